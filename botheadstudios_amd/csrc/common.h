@@ -1,0 +1,130 @@
+// Shared device definitions for the gfx950 particle-step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace isph {
+
+constexpr int kMaxMaterials = 16;           // gpu_sph.rs:156 MAX_MATERIALS
+constexpr uint32_t kNoParent = 0xffffffffu; // bh_gravity.wgsl:32
+constexpr float kG = 6.674e-11f;            // sph_step.wgsl:21, bh_gravity.wgsl:11
+constexpr double kG64 = 6.674e-11;          // orbit.rs:18
+constexpr float kPi = 3.14159265359f;       // sph_step.wgsl:20
+constexpr int kWave = 64;
+
+// Algorithmic flop constants per counted interaction (the roofline numerator).
+constexpr double kFlopsMonopole = 20.0;  // d(3) r2(6) rsqrt(1) r^-3(3) m*r^-3(1) accumulate(6)
+constexpr double kFlopsQuadrupole = 48.0;// monopole + Q.d(15) d.Q.d(5) quadrupole terms(8)
+constexpr double kFlopsDensityPair = 20.0;
+constexpr double kFlopsForcePair = 75.0;
+
+// Tillotson parameters (eos.rs:27-47 field order), f64.
+struct Till {
+    double rho0, a, b, cap_a, cap_b, e0, e_iv, e_cv, alpha, beta;
+};
+
+struct EosTable {
+    Till m[kMaxMaterials];
+};
+
+// ---- wave64 helpers --------------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    int l = __lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+__device__ __forceinline__ int prefix_count(uint64_t mask) { return __popcll(mask & lanemask_lt()); }
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Order-preserving float <-> uint map of bh_gravity.wgsl:49-57.
+__device__ __forceinline__ uint32_t bbox_key(float f) {
+    uint32_t k = __float_as_uint(f);
+    return (k >> 31) ? ~k : (k ^ 0x80000000u);
+}
+__device__ __forceinline__ float bbox_decode(uint32_t k) {
+    return __uint_as_float((k >> 31) ? (k ^ 0x80000000u) : ~k);
+}
+
+// ---- Tillotson, f64 (eos.rs:52-96; the GPU shader's f32 form is sph_step.wgsl:87-109) ----------------
+__device__ __forceinline__ double till_pressure(const Till& t, double rho_in, double u) {
+    double rho = fmax(rho_in, 1.0e-9);
+    double eta = rho / t.rho0;
+    double mu = eta - 1.0;
+    double omega = u / (t.e0 * eta * eta) + 1.0;
+    double pc = (t.a + t.b / omega) * rho * u + t.cap_a * mu + t.cap_b * mu * mu;
+    if (eta >= 1.0 || u <= t.e_iv) return pc;
+    double z = t.rho0 / rho - 1.0;
+    double pe = t.a * rho * u + (t.b * rho * u / omega + t.cap_a * mu * exp(-t.beta * z)) * exp(-t.alpha * z * z);
+    if (u >= t.e_cv) return pe;
+    return ((u - t.e_iv) * pe + (t.e_cv - u) * pc) / (t.e_cv - t.e_iv);
+}
+// Pressure and squared sound speed (central differences, relative step 1e-4 as eos.rs:88-96).
+__device__ __forceinline__ void till_p_c2(const Till& t, double rho_in, double u, double& p, double& c2) {
+    double rho = fmax(rho_in, 1.0e-9);
+    double dr = rho * 1.0e-4;
+    double du = fabs(u) * 1.0e-4 + 1.0;
+    double dp_drho = (till_pressure(t, rho + dr, u) - till_pressure(t, rho - dr, u)) / (2.0 * dr);
+    double dp_du = (till_pressure(t, rho, u + du) - till_pressure(t, rho, u - du)) / (2.0 * du);
+    p = till_pressure(t, rho, u);
+    c2 = fmax(dp_drho + p / (rho * rho) * dp_du, 0.0);
+}
+
+// ---- cubic spline with support h (sph_step.wgsl:73-86, atmosphere.rs:52-75) ---------------------------
+// W(r,h) and dW/dr from q = r/h and 1/h.
+__device__ __forceinline__ float spline_w(float q, float inv_h) {
+    float sig = (8.0f / kPi) * inv_h * inv_h * inv_h;
+    if (q < 0.5f) return sig * (1.0f - 6.0f * q * q + 6.0f * q * q * q);
+    if (q < 1.0f) {
+        float t = 1.0f - q;
+        return sig * 2.0f * t * t * t;
+    }
+    return 0.0f;
+}
+__device__ __forceinline__ float spline_dw(float q, float inv_h) {
+    float sig = (8.0f / kPi) * inv_h * inv_h * inv_h * inv_h;
+    if (q < 0.5f) return sig * (-12.0f * q + 18.0f * q * q);
+    if (q < 1.0f) {
+        float t = 1.0f - q;
+        return sig * (-6.0f * t * t);
+    }
+    return 0.0f;
+}
+
+// Compensated accumulation (Neumaier/Kahan) for the mixed-precision form.
+struct KahanF {
+    float s, c;
+    __device__ __forceinline__ void add(float x) {
+        float y = x - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// Map a linear block id so that consecutive work lands on the same XCD (8 XCDs, round-robin dispatch):
+// the blocks dispatched to XCD x receive the x-th contiguous slice of the work.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    uint32_t per = nblocks / kXcd;
+    uint32_t full = per * kXcd;
+    if (bid >= full) return bid;
+    return (bid % kXcd) * per + bid / kXcd;
+}
+
+}  // namespace isph

@@ -1,0 +1,290 @@
+// Barnes-Hut gravity over the LBVH: wave-cooperative group traversal, plus the exact direct sum.
+//
+// Reference: bh_gravity.wgsl:337-394 (cs_gravity_bh: per-particle stack walk, robust MAC
+// dist >= |bmax - bmin|/theta + |centre - com|, leaf buckets summed directly) and :313-328 / sph_step.wgsl:
+// 235-241 (direct sum). Here one wave64 walks the tree for 64 Morton-consecutive particles at once: a
+// node is accepted only when the MAC holds for every point of the group's bounding box (distance from the
+// node COM to the box, softened) and the node box does not overlap the group box, so each particle sees a
+// node set at least as refined as the reference's per-particle walk. Accepted cells and leaf particles are
+// appended to LDS lists with ballot/mbcnt compaction and evaluated 64 at a time with LDS broadcast reads.
+// The traversal stack lives in LDS; up to 64 nodes are popped per round (one per lane) and the pop width
+// narrows to depth-first near capacity, which bounds the stack by capacity - margin + tree depth.
+#include "state.h"
+
+namespace isph {
+
+constexpr int kStackCap = 1024;
+constexpr int kStackMargin = 128;  // > maximum Karras depth (30 code bits + 32 index tie-break bits)
+constexpr int kBuf = 128;
+
+struct Acc3 {
+    float x, y, z;
+};
+
+template <bool QUAD, bool COMP>
+__global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ spos, uint32_t n, uint32_t K,
+                                                     uint32_t L, const uint32_t* __restrict__ left,
+                                                     const uint32_t* __restrict__ right, const float4* __restrict__ nA,
+                                                     const float4* __restrict__ nB, const float4* __restrict__ nC,
+                                                     const float4* __restrict__ nQ0, const float2* __restrict__ nQ1,
+                                                     float soft2, float4* __restrict__ sgrav,
+                                                     unsigned long long* counters) {
+    __shared__ uint32_t stk[kStackCap];
+    __shared__ float4 cA[kBuf];
+    __shared__ float4 cQ0[QUAD ? kBuf : 1];
+    __shared__ float2 cQ1[QUAD ? kBuf : 1];
+    __shared__ float4 pA[kBuf];
+    __shared__ uint32_t pI[kBuf];
+
+    const int lane = threadIdx.x;
+    const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x);
+    const uint32_t t = g * kWave + lane;
+    const bool valid = t < n;
+    const float4 me = spos[valid ? t : n - 1];
+    const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
+    const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
+    const uint32_t leaf0 = L - 1;
+
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    KahanF kx{0.f, 0.f}, ky{0.f, 0.f}, kz{0.f, 0.f};
+    unsigned long long ncell = 0, npart = 0;
+
+    auto flush_cells = [&](int cnt) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        for (int j = 0; j < cnt; ++j) {
+            float4 c = cA[j];
+            float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+            float r2 = dx * dx + dy * dy + dz * dz + soft2;
+            float inv = __builtin_amdgcn_rsqf(r2);
+            float inv2 = inv * inv;
+            float w = c.w * inv * inv2;
+            if (QUAD) {
+                float4 q0 = cQ0[j];
+                float2 q1 = cQ1[j];
+                float qx = q0.x * dx + q0.w * dy + q1.x * dz;
+                float qy = q0.w * dx + q0.y * dy + q1.y * dz;
+                float qz = q1.x * dx + q1.y * dy + q0.z * dz;
+                float dqd = qx * dx + qy * dy + qz * dz;
+                float inv5 = inv * inv2 * inv2;
+                float s = w + 2.5f * dqd * inv5 * inv2;
+                bx += dx * s - qx * inv5;
+                by += dy * s - qy * inv5;
+                bz += dz * s - qz * inv5;
+            } else {
+                bx += dx * w;
+                by += dy * w;
+                bz += dz * w;
+            }
+        }
+        if (COMP) {
+            kx.add(bx);
+            ky.add(by);
+            kz.add(bz);
+        } else {
+            ax += bx;
+            ay += by;
+            az += bz;
+        }
+        ncell += cnt;
+    };
+    auto flush_parts = [&](int cnt) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        for (int j = 0; j < cnt; ++j) {
+            float4 c = pA[j];
+            float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+            float r2 = dx * dx + dy * dy + dz * dz + soft2;
+            float inv = __builtin_amdgcn_rsqf(r2);
+            float w = (pI[j] == t) ? 0.f : c.w * inv * inv * inv;  // s != t (bh_gravity.wgsl:362)
+            bx += dx * w;
+            by += dy * w;
+            bz += dz * w;
+        }
+        if (COMP) {
+            kx.add(bx);
+            ky.add(by);
+            kz.add(bz);
+        } else {
+            ax += bx;
+            ay += by;
+            az += bz;
+        }
+        npart += cnt;
+    };
+    // Drop the first 64 entries of a list after they were evaluated.
+    auto shift_cells = [&](int cnt) {
+        __syncthreads();
+        if (lane + kWave < cnt) {
+            cA[lane] = cA[lane + kWave];
+            if (QUAD) {
+                cQ0[lane] = cQ0[lane + kWave];
+                cQ1[lane] = cQ1[lane + kWave];
+            }
+        }
+        __syncthreads();
+    };
+    auto shift_parts = [&](int cnt) {
+        __syncthreads();
+        if (lane + kWave < cnt) {
+            pA[lane] = pA[lane + kWave];
+            pI[lane] = pI[lane + kWave];
+        }
+        __syncthreads();
+    };
+
+    int sp = 1, nc = 0, np = 0;
+    if (lane == 0) stk[0] = 0;  // root (a single leaf when L == 1)
+    __syncthreads();
+    while (sp > 0) {
+        int k = min(kWave, sp);
+        int room = kStackCap - kStackMargin - sp;
+        if (room < k) k = max(1, room);
+        sp -= k;
+        const bool has = lane < k;
+        const uint32_t node = has ? stk[sp + lane] : 0u;
+        __syncthreads();
+        const bool leaf = has && node >= leaf0;
+        bool accept = false, open = false;
+        float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (has && !leaf) {
+            A = nA[node];
+            const float4 B = nB[node];
+            const float4 C = nC[node];
+            float dx = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
+            float dy = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
+            float dz = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
+            float d2 = dx * dx + dy * dy + dz * dz + soft2;
+            bool disjoint = (B.x > gx1) | (C.x < gx0) | (B.y > gy1) | (C.y < gy0) | (B.z > gz1) | (C.z < gz0);
+            accept = disjoint && d2 >= C.w * C.w;
+            open = !accept;
+        }
+        const uint64_t mo = __ballot(open);
+        if (open) {
+            int pos = sp + 2 * prefix_count(mo);
+            stk[pos] = left[node];
+            stk[pos + 1] = right[node];
+        }
+        sp += 2 * __popcll(mo);
+        const uint64_t ma = __ballot(accept);
+        if (accept) {
+            int pos = nc + prefix_count(ma);
+            cA[pos] = A;
+            if (QUAD) {
+                cQ0[pos] = nQ0[node];
+                cQ1[pos] = nQ1[node];
+            }
+        }
+        nc += __popcll(ma);
+        const uint64_t ml = __ballot(leaf);
+        if (ml) {
+            const uint32_t c = leaf ? node - leaf0 : 0u;
+            const uint32_t start = c * K;
+            const uint32_t end = leaf ? min(start + K, n) : 0u;
+            for (uint32_t kk = 0; kk < K; ++kk) {
+                const bool present = leaf && start + kk < end;
+                const uint64_t mp = __ballot(present);
+                if (!mp) break;
+                if (present) {
+                    int pos = np + prefix_count(mp);
+                    pA[pos] = spos[start + kk];
+                    pI[pos] = start + kk;
+                }
+                np += __popcll(mp);
+                if (np >= kWave) {
+                    __syncthreads();
+                    flush_parts(kWave);
+                    shift_parts(np);
+                    np -= kWave;
+                }
+            }
+        }
+        __syncthreads();
+        if (nc >= kWave) {
+            flush_cells(kWave);
+            shift_cells(nc);
+            nc -= kWave;
+        }
+        if (np >= kWave) {
+            flush_parts(kWave);
+            shift_parts(np);
+            np -= kWave;
+        }
+    }
+    if (nc > 0) flush_cells(nc);
+    if (np > 0) flush_parts(np);
+    if (COMP) {
+        ax = kx.s;
+        ay = ky.s;
+        az = kz.s;
+    }
+    if (valid) sgrav[t] = make_float4(kG * ax, kG * ay, kG * az, 0.f);
+    const uint64_t mv = __ballot(valid);
+    if (lane == 0 && counters) {
+        unsigned long long nv = __popcll(mv);
+        atomicAdd(&counters[QUAD ? 2 : 0], ncell * nv);
+        atomicAdd(&counters[1], npart * nv);
+    }
+}
+
+// Exact softened direct sum in f32 (bh_gravity.wgsl:313-328), tiled through LDS.
+__global__ __launch_bounds__(256) void k_gravity_direct(const float4* __restrict__ spos, uint32_t n, float soft2,
+                                                        float4* __restrict__ sgrav) {
+    __shared__ float4 tile[256];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    float4 me = spos[i < n ? i : n - 1];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (uint32_t base = 0; base < n; base += 256) {
+        uint32_t j = base + threadIdx.x;
+        tile[threadIdx.x] = j < n ? spos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        uint32_t cnt = min(256u, n - base);
+        for (uint32_t jj = 0; jj < cnt; ++jj) {
+            float4 c = tile[jj];
+            float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+            float r2 = dx * dx + dy * dy + dz * dz + soft2;
+            float inv = __builtin_amdgcn_rsqf(r2);
+            float w = (base + jj == i) ? 0.f : c.w * inv * inv * inv;
+            ax += dx * w;
+            ay += dy * w;
+            az += dz * w;
+        }
+        __syncthreads();
+    }
+    if (i < n) sgrav[i] = make_float4(kG * ax, kG * ay, kG * az, 0.f);
+}
+
+__global__ void k_fill4(float4* p, uint32_t n, float4 v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_scatter_gravity(uint32_t n, const uint32_t* __restrict__ order, const float4* __restrict__ sgrav,
+                                  float4* __restrict__ acc) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) acc[order[t]] = sgrav[t];
+}
+
+void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s) {
+    uint32_t groups = (d.n + kWave - 1) / kWave;
+    auto args = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(groups), dim3(kWave), 0, s, d.spos, d.n, d.K, d.L, d.left, d.right, d.nA, d.nB,
+                           d.nC, d.nQ0, d.nQ1, p.soft2, d.sgrav, d.counters);
+    };
+    if (p.quad) {
+        if (p.mixed) args(k_gravity_tree<true, true>);
+        else args(k_gravity_tree<true, false>);
+    } else {
+        if (p.mixed) args(k_gravity_tree<false, true>);
+        else args(k_gravity_tree<false, false>);
+    }
+}
+void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_gravity_direct, dim3((d.n + 255) / 256), dim3(256), 0, s, d.spos, d.n, p.soft2, d.sgrav);
+}
+void launch_gravity_zero(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill4, dim3((d.n + 255) / 256), dim3(256), 0, s, d.sgrav, d.n, make_float4(0.f, 0.f, 0.f, 0.f));
+}
+void launch_scatter_gravity(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_gravity, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.vals, d.sgrav, d.acc);
+}
+
+}  // namespace isph

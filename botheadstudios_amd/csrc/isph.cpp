@@ -1,0 +1,774 @@
+// C-ABI host side of the MI355X particle step: handle lifetime, device memory, the per-evaluation
+// pipeline and the KDK drivers. Mirrors GpuSph (crates/engine/src/gpu_sph.rs:1086-1505) and GpuGravity
+// (crates/engine/src/gpu_gravity.rs:79-561); see include/isph.h for the per-entry-point mapping.
+#include "../../include/isph.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "state.h"
+
+using namespace isph;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(call)                                                                                        \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            throw HipError(std::string(#call) + ": " + hipGetErrorString(e_) + " (" + std::to_string(e_) + ")"); \
+    } while (0)
+
+struct EvPair {
+    int phase;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct isph_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Dev d;
+    StepParams p;
+    int gravity_mode = ISPH_GRAVITY_TREE;
+    int precision = ISPH_PRECISION_FP32;
+    float softening = 0.0f;
+    float dt_set = 0.0f;
+    float damp = 1.0f;
+    bool reuse_enabled = false;
+    bool reuse_valid = false;
+    uint32_t n_eos = 0;
+    std::string err;
+
+    // read-back
+    isph_particle* pinned = nullptr;
+    hipEvent_t rb_event = nullptr;
+    bool rb_inflight = false;
+    uint32_t rb_count = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<EvPair> ev_pairs;
+    double phase_ms[ISPH_NUM_PHASES] = {0};
+    double grav_launches = 0;
+
+    // gravity-API scratch
+    float4* gpos = nullptr;
+    float* gmass = nullptr;
+};
+
+namespace {
+
+template <class T>
+void dalloc(T** p, size_t count) {
+    HIP_OK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
+}
+
+void free_all(isph_ctx* c) {
+    Dev& d = c->d;
+    void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
+                    d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
+                    d.lcode, d.left, d.right, d.parent, d.ready, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->rb_event) (void)hipEventDestroy(c->rb_event);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+void alloc_all(isph_ctx* c, uint32_t cap) {
+    Dev& d = c->d;
+    d.cap = cap;
+    size_t nodes = 2 * (size_t)cap;
+    uint32_t sort_blocks = (cap + 4095) / 4096;
+    dalloc(&d.posh, cap);
+    dalloc(&d.velu, cap);
+    dalloc(&d.mass, cap);
+    dalloc(&d.mat, cap);
+    dalloc(&d.prov, cap);
+    dalloc(&d.rho, cap);
+    dalloc(&d.acc, cap);
+    dalloc(&d.keys, cap);
+    dalloc(&d.vals, cap);
+    dalloc(&d.keys_alt, cap);
+    dalloc(&d.vals_alt, cap);
+    dalloc(&d.hist, 256 * (size_t)sort_blocks);
+    d.nsort_blocks = sort_blocks;
+    dalloc(&d.spos, cap);
+    dalloc(&d.svel, cap);
+    dalloc(&d.sh, cap);
+    dalloc(&d.smat, cap);
+    dalloc(&d.srho, cap);
+    dalloc(&d.spr, cap);
+    dalloc(&d.scs, cap);
+    dalloc(&d.sgrav, cap);
+    dalloc(&d.lcode, cap);
+    dalloc(&d.left, cap);
+    dalloc(&d.right, cap);
+    dalloc(&d.parent, nodes);
+    dalloc(&d.ready, cap);
+    dalloc(&d.nA, nodes);
+    dalloc(&d.nB, nodes);
+    dalloc(&d.nC, nodes);
+    dalloc(&d.nQ0, nodes);
+    dalloc(&d.nQ1, nodes);
+    dalloc(&d.scal, 8);
+    dalloc(&d.dt, 1);
+    dalloc(&d.time, 1);
+    dalloc(&d.counters, 8);
+    dalloc(&d.red, 1024 * 16);
+    dalloc(&d.eos, kMaxMaterials);
+    HIP_OK(hipMalloc(&d.aos, (size_t)cap * sizeof(isph_particle)));
+    HIP_OK(hipHostMalloc((void**)&c->pinned, (size_t)cap * sizeof(isph_particle), hipHostMallocDefault));
+    HIP_OK(hipEventCreateWithFlags(&c->rb_event, hipEventDisableTiming));
+    HIP_OK(hipMemset(d.counters, 0, 8 * sizeof(unsigned long long)));
+    HIP_OK(hipMemset(d.time, 0, sizeof(double)));
+    HIP_OK(hipMemset(d.dt, 0, sizeof(float)));
+    HIP_OK(hipMemset(d.eos, 0, kMaxMaterials * sizeof(Till)));
+}
+
+void ensure64(isph_ctx* c) {
+    if (!c->d.pos64) {
+        dalloc(&c->d.pos64, 3 * (size_t)c->d.cap);
+        dalloc(&c->d.vel64, 3 * (size_t)c->d.cap);
+    }
+}
+
+// ---- timing -------------------------------------------------------------------------------------------
+hipEvent_t next_event(isph_ctx* c);
+
+void flush_timing(isph_ctx* c) {
+    if (c->ev_pairs.empty()) return;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (const EvPair& e : c->ev_pairs) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e.a, e.b));
+        c->phase_ms[e.phase] += ms;
+    }
+    c->ev_pairs.clear();
+    c->ev_used = 0;
+}
+
+hipEvent_t next_event(isph_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        if (c->ev_pool.size() < 8192) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        } else {
+            flush_timing(c);
+        }
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+struct Phase {
+    isph_ctx* c;
+    int phase;
+    hipEvent_t a = nullptr;
+    Phase(isph_ctx* c_, int ph) : c(c_), phase(ph) {
+        if (c->timing) {
+            a = next_event(c);
+            HIP_OK(hipEventRecord(a, c->stream));
+        }
+    }
+    ~Phase() {
+        if (!c->timing || !a) return;
+        try {
+            hipEvent_t b = next_event(c);
+            if (hipEventRecord(b, c->stream) == hipSuccess) c->ev_pairs.push_back({phase, a, b});
+        } catch (...) {
+            // a failed timing record drops this phase sample; the work itself was enqueued
+        }
+    }
+};
+
+// ---- pipeline -----------------------------------------------------------------------------------------
+void set_leaves(isph_ctx* c, uint32_t n) {
+    c->d.n = n;
+    c->d.L = (n + c->d.K - 1) / c->d.K;
+    if (c->d.L == 0) c->d.L = 1;
+}
+
+void build_sorted(isph_ctx* c, const Dev& d, const float4* pos, const float* mass, bool with_sph) {
+    Phase ph(c, ISPH_PHASE_SORT);
+    launch_reset_scalars(d, c->stream);
+    launch_bbox(d, pos, c->stream);
+    launch_morton(d, pos, c->stream);
+    launch_radix_sort(d, c->stream);
+    Dev g = d;
+    if (!with_sph) {
+        g.sh = nullptr;
+        g.svel = nullptr;
+        g.smat = nullptr;
+    }
+    launch_gather(g, pos, mass, c->stream);
+}
+
+void build_tree(isph_ctx* c, const Dev& d) {
+    {
+        Phase ph(c, ISPH_PHASE_TREE);
+        launch_karras(d, c->stream);
+    }
+    Phase ph(c, ISPH_PHASE_UPSWEEP);
+    launch_upsweep(d, c->p, c->stream);
+}
+
+void run_gravity(isph_ctx* c, const Dev& d, int mode) {
+    Phase ph(c, ISPH_PHASE_GRAVITY);
+    if (mode == ISPH_GRAVITY_TREE) {
+        launch_gravity_tree(d, c->p, c->stream);
+        c->grav_launches += 1;
+    } else if (mode == ISPH_GRAVITY_DIRECT) {
+        launch_gravity_direct(d, c->p, c->stream);
+        c->grav_launches += 1;
+    } else {
+        launch_gravity_zero(d, c->stream);
+    }
+}
+
+// One force evaluation (gpu_sph.rs:1468-1474): sort + tree + gravity + density/EOS + force.
+void eval(isph_ctx* c, bool rebuild) {
+    const Dev& d = c->d;
+    if (rebuild) {
+        build_sorted(c, d, d.posh, d.mass, true);
+        build_tree(c, d);
+        run_gravity(c, d, c->gravity_mode);
+        Phase ph(c, ISPH_PHASE_DENSITY);
+        launch_density(d, c->p, c->stream);
+    } else {
+        Phase ph(c, ISPH_PHASE_DENSITY);
+        launch_reset_scalars(d, c->stream);
+        launch_gather_vel(d, c->stream);
+        launch_eos(d, c->stream);
+    }
+    Phase ph(c, ISPH_PHASE_FORCE);
+    launch_force(d, c->p, c->stream);
+}
+
+void kdk(isph_ctx* c, bool cfl_mode, float cfl) {
+    const Dev& d = c->d;
+    eval(c, !(c->reuse_enabled && c->reuse_valid));
+    {
+        Phase ph(c, ISPH_PHASE_INTEGRATE);
+        if (cfl_mode) launch_set_dt_cfl(d, cfl, c->stream);
+        launch_kick_drift(d, c->stream);
+    }
+    eval(c, true);
+    {
+        Phase ph(c, ISPH_PHASE_INTEGRATE);
+        launch_kick(d, c->stream);
+        launch_advance_time(d, c->stream);
+    }
+    c->reuse_valid = true;
+}
+
+int fail(isph_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class F>
+int guarded(isph_ctx* c, F&& f) {
+    if (!c) return ISPH_EINVAL;
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(c, ISPH_EDEVICE, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(c, ISPH_ENOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(c, ISPH_EINVAL, e.what());
+    }
+}
+
+void widen_eos(isph_ctx* c, const isph_eos* eos, size_t n) {
+    Till t[kMaxMaterials];
+    std::memset(t, 0, sizeof(t));
+    for (size_t i = 0; i < n; ++i) {
+        t[i] = Till{eos[i].rho0, eos[i].a, eos[i].b, eos[i].cap_a, eos[i].cap_b, eos[i].e0, eos[i].e_iv, eos[i].e_cv,
+                    eos[i].alpha, eos[i].beta};
+    }
+    HIP_OK(hipMemcpy(c->d.eos, t, sizeof(t), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+extern "C" {
+
+isph_t* isph_create(uint32_t capacity, int device) {
+    isph_ctx* c = nullptr;
+    try {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            g_create_error = "no HIP device";
+            return nullptr;
+        }
+        if (device < 0 || device >= count) {
+            g_create_error = "device ordinal out of range";
+            return nullptr;
+        }
+        c = new isph_ctx();
+        c->device = device;
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        alloc_all(c, std::max<uint32_t>(capacity, 1));
+        c->d.K = 1;
+        set_leaves(c, 0);
+        g_create_error.clear();
+        return c;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        if (c) {
+            free_all(c);
+            delete c;
+        }
+        return nullptr;
+    }
+}
+
+void isph_destroy(isph_t* h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_all(h);
+    delete h;
+}
+
+const char* isph_last_error(const isph_t* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+uint32_t isph_count(const isph_t* h) { return h ? h->d.n : 0; }
+
+int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_eos* eos, size_t n_eos,
+                float softening) {
+    return guarded(h, [&]() -> int {
+        if (n > 0 && !particles) return fail(h, ISPH_EINVAL, "particles is NULL");
+        size_t ne = std::min<size_t>(n_eos, kMaxMaterials);  // gpu_sph.rs:1337
+        size_t take = std::min<size_t>(n, h->d.cap);          // gpu_store.rs:33
+        for (size_t i = 0; i < take; ++i)
+            if (particles[i].mat >= ne) return fail(h, ISPH_EINVAL, "particle material index outside the EOS table");
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (take > 0) {
+            HIP_OK(hipMemcpy(h->d.aos, particles, take * sizeof(isph_particle), hipMemcpyHostToDevice));
+            launch_unpack(h->d, (uint32_t)take, h->stream);
+        }
+        if (ne > 0) widen_eos(h, eos, ne);
+        h->n_eos = (uint32_t)ne;
+        set_leaves(h, (uint32_t)take);
+        h->softening = softening;
+        h->p.soft2 = softening * softening;
+        if (h->precision == ISPH_PRECISION_MIXED && take > 0) {
+            ensure64(h);
+            launch_sync64_from32(h->d, h->stream);
+        }
+        h->reuse_valid = false;
+        h->rb_inflight = false;
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return (int)take;
+    });
+}
+
+int isph_set_eos_f64(isph_t* h, const double* params10, size_t n_eos) {
+    return guarded(h, [&]() -> int {
+        if (!params10 || n_eos == 0 || n_eos > (size_t)kMaxMaterials) return fail(h, ISPH_EINVAL, "bad EOS table");
+        Till t[kMaxMaterials];
+        std::memset(t, 0, sizeof(t));
+        for (size_t i = 0; i < n_eos; ++i) {
+            const double* q = params10 + 10 * i;
+            t[i] = Till{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9]};
+        }
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipMemcpy(h->d.eos, t, sizeof(t), hipMemcpyHostToDevice));
+        h->n_eos = (uint32_t)n_eos;
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_dt(isph_t* h, float dt, float damp) {
+    return guarded(h, [&]() -> int {
+        h->dt_set = dt;
+        h->damp = damp;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_av(isph_t* h, float alpha, float beta) {
+    return guarded(h, [&]() -> int {
+        h->p.av_alpha = alpha;
+        h->p.av_beta = beta;
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int quadrupole) {
+    return guarded(h, [&]() -> int {
+        if (mode < ISPH_GRAVITY_TREE || mode > ISPH_GRAVITY_NONE) return fail(h, ISPH_EINVAL, "bad gravity mode");
+        if (!(theta > 0.0f) || leaf_size < 1 || leaf_size > 64) return fail(h, ISPH_EINVAL, "bad theta or leaf size");
+        h->gravity_mode = mode;
+        h->p.theta = theta;
+        h->p.quad = quadrupole ? 1 : 0;
+        h->d.K = leaf_size;
+        set_leaves(h, h->d.n);
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_precision(isph_t* h, int mode) {
+    return guarded(h, [&]() -> int {
+        if (mode != ISPH_PRECISION_FP32 && mode != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "bad precision");
+        HIP_OK(hipSetDevice(h->device));
+        if (mode == ISPH_PRECISION_MIXED) {
+            ensure64(h);
+            if (h->d.n > 0) launch_sync64_from32(h->d, h->stream);
+            h->p.mixed = 1;
+        } else {
+            h->p.mixed = 0;
+        }
+        h->precision = mode;
+        // The integrators take the f64 arrays only in mixed mode.
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_reuse(isph_t* h, int enable) {
+    return guarded(h, [&]() -> int {
+        h->reuse_enabled = enable != 0;
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+static Dev integ_view(isph_ctx* c) {
+    Dev d = c->d;
+    if (c->precision != ISPH_PRECISION_MIXED) {
+        d.pos64 = nullptr;
+        d.vel64 = nullptr;
+    }
+    return d;
+}
+
+int isph_encode_kdk(isph_t* h, uint32_t substeps) {
+    return guarded(h, [&]() -> int {
+        if (h->d.n == 0 || substeps == 0) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        Dev saved = h->d;
+        h->d = integ_view(h);
+        try {
+            launch_set_dt_value(h->d, h->dt_set, h->stream);
+            for (uint32_t s = 0; s < substeps; ++s) kdk(h, false, 0.0f);
+        } catch (...) {
+            h->d = saved;
+            throw;
+        }
+        h->d = saved;
+        HIP_OK(hipGetLastError());
+        return ISPH_OK;
+    });
+}
+
+int isph_step_cfl(isph_t* h, float cfl, uint32_t steps) {
+    return guarded(h, [&]() -> int {
+        if (!(cfl > 0.0f)) return fail(h, ISPH_EINVAL, "cfl must be > 0");
+        if (h->d.n == 0 || steps == 0) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        Dev saved = h->d;
+        h->d = integ_view(h);
+        try {
+            for (uint32_t s = 0; s < steps; ++s) kdk(h, true, cfl);
+        } catch (...) {
+            h->d = saved;
+            throw;
+        }
+        h->d = saved;
+        HIP_OK(hipGetLastError());
+        return ISPH_OK;
+    });
+}
+
+int isph_sim_time(isph_t* h, double* t, float* last_dt) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (t) HIP_OK(hipMemcpy(t, h->d.time, sizeof(double), hipMemcpyDeviceToHost));
+        if (last_dt) HIP_OK(hipMemcpy(last_dt, h->d.dt, sizeof(float), hipMemcpyDeviceToHost));
+        return ISPH_OK;
+    });
+}
+
+int isph_begin_readback(isph_t* h) {
+    return guarded(h, [&]() -> int {
+        if (h->d.n == 0 || h->rb_inflight) return ISPH_OK;  // gpu_store.rs:127-129
+        HIP_OK(hipSetDevice(h->device));
+        launch_pack(h->d, h->stream);
+        HIP_OK(hipMemcpyAsync(h->pinned, h->d.aos, (size_t)h->d.n * sizeof(isph_particle), hipMemcpyDeviceToHost,
+                              h->stream));
+        HIP_OK(hipEventRecord(h->rb_event, h->stream));
+        h->rb_inflight = true;
+        h->rb_count = h->d.n;
+        return ISPH_OK;
+    });
+}
+
+int isph_take_readback(isph_t* h, isph_particle* out, size_t cap) {
+    return guarded(h, [&]() -> int {
+        if (!h->rb_inflight) return ISPH_EAGAIN;
+        hipError_t q = hipEventQuery(h->rb_event);
+        if (q == hipErrorNotReady) return ISPH_EAGAIN;
+        HIP_OK(q);
+        size_t m = std::min<size_t>(h->rb_count, cap);
+        if (m > 0 && out) std::memcpy(out, h->pinned, m * sizeof(isph_particle));
+        h->rb_inflight = false;
+        return (int)h->rb_count;
+    });
+}
+
+int isph_readback_sync(isph_t* h, isph_particle* out, size_t cap) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        if (h->d.n == 0) return 0;
+        launch_pack(h->d, h->stream);
+        HIP_OK(hipMemcpyAsync(h->pinned, h->d.aos, (size_t)h->d.n * sizeof(isph_particle), hipMemcpyDeviceToHost,
+                              h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        size_t m = std::min<size_t>(h->d.n, cap);
+        if (m > 0 && out) std::memcpy(out, h->pinned, m * sizeof(isph_particle));
+        return (int)h->d.n;
+    });
+}
+
+int isph_synchronize(isph_t* h) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return ISPH_OK;
+    });
+}
+
+int isph_force_eval(isph_t* h, float* acc4, float* rho) {
+    return guarded(h, [&]() -> int {
+        if (h->d.n == 0) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        eval(h, true);
+        h->reuse_valid = false;
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        if (acc4) HIP_OK(hipMemcpy(acc4, h->d.acc, (size_t)h->d.n * sizeof(float4), hipMemcpyDeviceToHost));
+        if (rho) HIP_OK(hipMemcpy(rho, h->d.rho, (size_t)h->d.n * sizeof(float), hipMemcpyDeviceToHost));
+        return ISPH_OK;
+    });
+}
+
+int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass, size_t n, double softening,
+                               double* acc3, int mode) {
+    return guarded(h, [&]() -> int {
+        if (n == 0) return ISPH_OK;
+        if (!pos3 || !mass || !acc3) return fail(h, ISPH_EINVAL, "NULL array");
+        if (n > h->d.cap) return fail(h, ISPH_ERANGE, "n exceeds the handle capacity");
+        if (mode != ISPH_GRAVITY_TREE && mode != ISPH_GRAVITY_DIRECT) return fail(h, ISPH_EINVAL, "bad mode");
+        HIP_OK(hipSetDevice(h->device));
+        if (!h->gpos) {
+            dalloc(&h->gpos, h->d.cap);
+            dalloc(&h->gmass, h->d.cap);
+        }
+        // gpu_gravity.rs:340-345: bodies as f32 (pos.x, pos.y, pos.z, mass); soft2 = (s*s) as f32.
+        std::vector<float4> b(n);
+        std::vector<float> m(n);
+        for (size_t i = 0; i < n; ++i) {
+            b[i] = make_float4((float)pos3[3 * i], (float)pos3[3 * i + 1], (float)pos3[3 * i + 2], 0.f);
+            m[i] = (float)mass[i];
+        }
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipMemcpy(h->gpos, b.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(h->gmass, m.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        Dev g = h->d;
+        g.n = (uint32_t)n;
+        g.L = (g.n + g.K - 1) / g.K;
+        StepParams saved = h->p;
+        h->p.soft2 = (float)(softening * softening);
+        try {
+            build_sorted(h, g, h->gpos, h->gmass, false);
+            if (mode == ISPH_GRAVITY_TREE) build_tree(h, g);
+            run_gravity(h, g, mode);
+        } catch (...) {
+            h->p = saved;
+            throw;
+        }
+        h->p = saved;
+        Dev s = g;
+        s.acc = h->gpos;  // scratch for the scattered result (original order)
+        launch_scatter_gravity(s, h->stream);
+        std::vector<float4> out(n);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(out.data(), h->gpos, n * sizeof(float4), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            acc3[3 * i] = out[i].x;
+            acc3[3 * i + 1] = out[i].y;
+            acc3[3 * i + 2] = out[i].z;
+        }
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_diagnostics(isph_t* h, int with_pe, double* out7) {
+    return guarded(h, [&]() -> int {
+        if (!out7) return fail(h, ISPH_EINVAL, "NULL output");
+        for (int k = 0; k < 7; ++k) out7[k] = 0.0;
+        if (h->d.n == 0) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        Dev d = integ_view(h);
+        launch_diag(d, h->softening, with_pe, h->stream);
+        std::vector<double> red(1024 * 16);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(red.data(), d.red, red.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double f[13] = {0};
+        uint32_t pe_blocks = std::min<uint32_t>((d.n + 255) / 256, 1024);
+        for (int b = 0; b < 1024; ++b)
+            for (int k = 0; k < 12; ++k) f[k] += red[b * 16 + k];
+        if (with_pe)
+            for (uint32_t b = 0; b < pe_blocks; ++b) f[12] += red[b * 16 + 12];
+        double M = f[2];
+        out7[0] = f[0];
+        out7[1] = f[1];
+        out7[2] = f[12];
+        // L about the COM = sum m x cross v - (sum m x / M) cross (sum m v)
+        double cx = f[3] / M, cy = f[4] / M, cz = f[5] / M;
+        out7[3] = f[9] - (cy * f[8] - cz * f[7]);
+        out7[4] = f[10] - (cz * f[6] - cx * f[8]);
+        out7[5] = f[11] - (cx * f[7] - cy * f[6]);
+        out7[6] = M;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_timing(isph_t* h, int enable) {
+    return guarded(h, [&]() -> int {
+        if (!enable && h->timing) flush_timing(h);
+        h->timing = enable != 0;
+        return ISPH_OK;
+    });
+}
+
+int isph_phase_times(isph_t* h, double* ms, size_t n) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        flush_timing(h);
+        for (size_t k = 0; k < n && k < (size_t)ISPH_NUM_PHASES; ++k) ms[k] = h->phase_ms[k];
+        return ISPH_OK;
+    });
+}
+
+int isph_kernel_stats(isph_t* h, double* out7) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        flush_timing(h);
+        unsigned long long cnt[8];
+        HIP_OK(hipMemcpy(cnt, h->d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+        out7[0] = h->grav_launches;
+        out7[1] = h->phase_ms[ISPH_PHASE_GRAVITY];
+        out7[2] = (double)cnt[0];
+        out7[3] = (double)cnt[1];
+        out7[4] = (double)cnt[2];
+        out7[5] = (double)cnt[3];
+        out7[6] = (double)cnt[4];
+        return ISPH_OK;
+    });
+}
+
+int isph_reset_stats(isph_t* h) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        flush_timing(h);
+        for (double& x : h->phase_ms) x = 0.0;
+        h->grav_launches = 0;
+        HIP_OK(hipMemsetAsync(h->d.counters, 0, 8 * sizeof(unsigned long long), h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        return ISPH_OK;
+    });
+}
+
+int isph_probe_morton(isph_t* h, uint32_t* codes, float* lo3, float* hi3) {
+    return guarded(h, [&]() -> int {
+        if (h->d.n == 0) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        launch_reset_scalars(h->d, h->stream);
+        launch_bbox(h->d, h->d.posh, h->stream);
+        launch_morton(h->d, h->d.posh, h->stream);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        if (codes) HIP_OK(hipMemcpy(codes, h->d.keys, (size_t)h->d.n * 4, hipMemcpyDeviceToHost));
+        uint32_t k[6];
+        HIP_OK(hipMemcpy(k, h->d.scal, sizeof(k), hipMemcpyDeviceToHost));
+        for (int a = 0; a < 3; ++a) {
+            uint32_t lo = ((k[a] >> 31) ? (k[a] ^ 0x80000000u) : ~k[a]);
+            uint32_t hi = ((k[3 + a] >> 31) ? (k[3 + a] ^ 0x80000000u) : ~k[3 + a]);
+            if (lo3) std::memcpy(&lo3[a], &lo, 4);
+            if (hi3) std::memcpy(&hi3[a], &hi, 4);
+        }
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_probe_sort(isph_t* h, uint32_t* sorted_codes, uint32_t* order) {
+    return guarded(h, [&]() -> int {
+        if (h->d.n == 0) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        build_sorted(h, h->d, h->d.posh, h->d.mass, true);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        if (sorted_codes) HIP_OK(hipMemcpy(sorted_codes, h->d.keys, (size_t)h->d.n * 4, hipMemcpyDeviceToHost));
+        if (order) HIP_OK(hipMemcpy(order, h->d.vals, (size_t)h->d.n * 4, hipMemcpyDeviceToHost));
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_probe_tree(isph_t* h, uint32_t* left, uint32_t* right, uint32_t* parent, float* com4, float* bmin4,
+                    float* bmax4) {
+    return guarded(h, [&]() -> int {
+        if (h->d.n == 0) return 0;
+        HIP_OK(hipSetDevice(h->device));
+        build_sorted(h, h->d, h->d.posh, h->d.mass, true);
+        build_tree(h, h->d);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        uint32_t L = h->d.L;
+        size_t nodes = 2 * (size_t)L - 1;
+        if (left && L > 1) HIP_OK(hipMemcpy(left, h->d.left, (L - 1) * 4, hipMemcpyDeviceToHost));
+        if (right && L > 1) HIP_OK(hipMemcpy(right, h->d.right, (L - 1) * 4, hipMemcpyDeviceToHost));
+        if (parent) HIP_OK(hipMemcpy(parent, h->d.parent, nodes * 4, hipMemcpyDeviceToHost));
+        if (com4) HIP_OK(hipMemcpy(com4, h->d.nA, nodes * 16, hipMemcpyDeviceToHost));
+        if (bmin4) HIP_OK(hipMemcpy(bmin4, h->d.nB, nodes * 16, hipMemcpyDeviceToHost));
+        if (bmax4) HIP_OK(hipMemcpy(bmax4, h->d.nC, nodes * 16, hipMemcpyDeviceToHost));
+        h->reuse_valid = false;
+        return (int)L;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,240 @@
+// Bounding box, 30-bit Morton codes, LSD radix sort of (code, index) and the gather into Morton order.
+//
+// Reference: bh_gravity.wgsl:49-108 (bbox + Morton) and the CPU sort leg gpu_gravity.rs:351-388, which
+// this replaces with a device radix sort. The sort is stable and its input is in index order, so it
+// produces exactly the (code, index) order of `pairs.sort_unstable()` on (u32, u32) tuples.
+#include "state.h"
+
+namespace isph {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per block per pass
+
+__global__ void k_reset_scalars(uint32_t* scal, unsigned long long* counters, int reset_counters) {
+    int i = threadIdx.x;
+    if (i < 3) scal[i] = 0xffffffffu;              // min slots start at +inf key (bh_gravity.wgsl:64-65)
+    else if (i < 6) scal[i] = 0u;                  // max slots at -inf key
+    else if (i == 6) scal[i] = 0x7f800000u;        // Courant signal min: +inf bits
+    else if (i == 7) scal[i] = 0u;
+}
+
+// Block-reduced float-radix min/max (bh_gravity.wgsl:67-78 does one atomic per particle per axis).
+__global__ __launch_bounds__(256) void k_bbox(const float4* __restrict__ pos, uint32_t n, uint32_t* scal) {
+    __shared__ uint32_t red[6][4];
+    uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 p = pos[i];
+        uint32_t k0 = bbox_key(p.x), k1 = bbox_key(p.y), k2 = bbox_key(p.z);
+        lo[0] = min(lo[0], k0); lo[1] = min(lo[1], k1); lo[2] = min(lo[2], k2);
+        hi[0] = max(hi[0], k0); hi[1] = max(hi[1], k1); hi[2] = max(hi[2], k2);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], o));
+            hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], o));
+        }
+    }
+    int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        for (int k = 0; k < 3; ++k) {
+            red[k][w] = lo[k];
+            red[3 + k][w] = hi[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        uint32_t v = red[k][0];
+        for (int j = 1; j < (int)(blockDim.x / kWave); ++j) v = (k < 3) ? min(v, red[k][j]) : max(v, red[k][j]);
+        if (k < 3) atomicMin(&scal[k], v);
+        else atomicMax(&scal[k], v);
+    }
+}
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {  // bh_gravity.wgsl:82-89
+    v &= 0x000003ffu;
+    v = (v * 0x00010001u) & 0xff0000ffu;
+    v = (v * 0x00000101u) & 0x0f00f00fu;
+    v = (v * 0x00000011u) & 0xc30c30c3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// bh_gravity.wgsl:91-108: clamp((p - lo) / ext) with correctly rounded f32 division, floor(u * 1024).
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ pos, uint32_t n, const uint32_t* scal,
+                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ order) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float lo[3], ext[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = bbox_decode(scal[k]);
+        float hi = bbox_decode(scal[3 + k]);
+        ext[k] = fmaxf(__fsub_rn(hi, lo[k]), 1.0e-30f);
+    }
+    float4 p = pos[i];
+    float c[3] = {p.x, p.y, p.z};
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float u = __fdiv_rn(__fsub_rn(c[k], lo[k]), ext[k]);
+        u = fminf(fmaxf(u, 0.0f), 1.0f);
+        float f = floorf(__fmul_rn(u, 1024.0f));
+        f = fminf(fmaxf(f, 0.0f), 1023.0f);
+        q[k] = (uint32_t)f;
+    }
+    codes[i] = expand_bits(q[0]) * 4u + expand_bits(q[1]) * 2u + expand_bits(q[2]);
+    order[i] = i;
+}
+
+// ---- LSD radix sort, 8-bit digits ------------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                             int shift, uint32_t* __restrict__ hist, uint32_t nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int it = 0; it < kSortItems; ++it) {
+        uint32_t e = base + it * kSortThreads + threadIdx.x;
+        if (e < n) atomicAdd(&h[(keys[e] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of the digit-major histogram (256 * nb entries) by one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, uint32_t total) {
+    __shared__ uint32_t part[1024];
+    uint32_t per = (total + 1023) / 1024;
+    uint32_t b = threadIdx.x * per, e = min(b + per, total);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += hist[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint32_t v = threadIdx.x >= (uint32_t)o ? part[threadIdx.x - o] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t i = b; i < e; ++i) {
+        uint32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+// Stable scatter: within a tile, elements are ranked in index order per digit with wave64 match masks.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                                const uint32_t* __restrict__ vin,
+                                                                uint32_t* __restrict__ kout,
+                                                                uint32_t* __restrict__ vout, uint32_t n, int shift,
+                                                                const uint32_t* __restrict__ hist, uint32_t nb) {
+    constexpr int kWaves = kSortThreads / kWave;
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kWaves][256];
+    int tid = threadIdx.x, w = tid / kWave;
+    run[tid] = hist[tid * nb + blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) wcnt[k][tid] = 0;
+    __syncthreads();
+    uint32_t base = blockIdx.x * kSortTile;
+    for (int it = 0; it < kSortItems; ++it) {
+        uint32_t e = base + it * kSortThreads + tid;
+        bool valid = e < n;
+        uint32_t key = valid ? kin[e] : 0xffffffffu;
+        uint32_t val = valid ? vin[e] : 0u;
+        uint32_t dig = (key >> shift) & 0xffu;
+        uint64_t peers = ~0ull;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t bal = __ballot((dig >> b) & 1u);
+            peers &= ((dig >> b) & 1u) ? bal : ~bal;
+        }
+        int rank = __popcll(peers & lanemask_lt());
+        if (rank == 0) wcnt[w][dig] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t off = run[dig] + rank;
+        for (int k = 0; k < w; ++k) off += wcnt[k][dig];
+        if (valid) {
+            kout[off] = key;
+            vout[off] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) {
+            add += wcnt[k][tid];
+            wcnt[k][tid] = 0;
+        }
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+// Gather the state into Morton order; leaf cluster code = first code of each bucket (gpu_gravity.rs:386).
+__global__ __launch_bounds__(256) void k_gather(uint32_t n, uint32_t K, const uint32_t* __restrict__ order,
+                                                const uint32_t* __restrict__ codes, const float4* __restrict__ posh,
+                                                const float* __restrict__ mass, const float4* __restrict__ velu,
+                                                const uint32_t* __restrict__ mat, float4* __restrict__ spos,
+                                                float* __restrict__ sh, float4* __restrict__ svel,
+                                                uint32_t* __restrict__ smat, uint32_t* __restrict__ lcode) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t o = order[t];
+    float4 p = posh[o];
+    spos[t] = make_float4(p.x, p.y, p.z, mass[o]);
+    if (sh) sh[t] = p.w;
+    if (svel) svel[t] = velu[o];
+    if (smat) smat[t] = mat[o];
+    if (t % K == 0) lcode[t / K] = codes[t];
+}
+
+__global__ __launch_bounds__(256) void k_gather_vel(uint32_t n, const uint32_t* __restrict__ order,
+                                                    const float4* __restrict__ velu, float4* __restrict__ svel) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    svel[t] = velu[order[t]];
+}
+
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+void launch_reset_scalars(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, d.scal, d.counters, 0);
+}
+void launch_bbox(const Dev& d, const float4* pos, hipStream_t s) {
+    uint32_t blocks = std::min<uint32_t>(cdiv(d.n, 256), 2048);
+    hipLaunchKernelGGL(k_bbox, dim3(std::max<uint32_t>(blocks, 1)), dim3(256), 0, s, pos, d.n, d.scal);
+}
+void launch_morton(const Dev& d, const float4* pos, hipStream_t s) {
+    hipLaunchKernelGGL(k_morton, dim3(cdiv(d.n, 256)), dim3(256), 0, s, pos, d.n, d.scal, d.keys, d.vals);
+}
+void launch_radix_sort(const Dev& d, hipStream_t s) {
+    uint32_t nb = cdiv(d.n, kSortTile);
+    uint32_t* kin = d.keys;
+    uint32_t* vin = d.vals;
+    uint32_t* kout = d.keys_alt;
+    uint32_t* vout = d.vals_alt;
+    for (int shift = 0; shift < 32; shift += 8) {  // 4 passes: even count, result back in keys/vals
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, kin, d.n, shift, d.hist, nb);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, d.hist, 256u * nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, kin, vin, kout, vout, d.n, shift,
+                           d.hist, nb);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+}
+void launch_gather(const Dev& d, const float4* pos, const float* mass, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.K, d.vals, d.keys, pos, mass, d.velu,
+                       d.mat, d.spos, d.sh, d.svel, d.smat, d.lcode);
+}
+void launch_gather_vel(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_vel, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.vals, d.velu, d.svel);
+}
+
+}  // namespace isph

@@ -1,0 +1,581 @@
+// SPH density, per-particle FP64 Tillotson EOS, Courant signal, pressure + artificial-viscosity force with
+// du/dt, the KDK integrators, AoS pack/unpack and the f64 diagnostics.
+//
+// Reference: sph_step.wgsl:181-216 (density), :219-296 (forces + du/dt), :304-333 (KDK), :364-371 (signal);
+// the CPU path hydrostatic.rs:338-480 (same physics in f64, EOS once per particle with the eos.rs:88-96
+// sound speed). Neighbours are found through the LBVH instead of the reference's 2^15-cell hash table: one
+// wave walks the tree for 64 Morton-consecutive particles, keeping every node whose box comes within
+// (h_group_max + h_node_max)/2 of the group box, so the candidate set contains every pair with
+// r < (h_i + h_j)/2 exactly once; each lane then applies the exact radius test, as the reference's
+// cell-membership-guarded 27-cell scan does. Pressure and sound speed are computed once per particle (the
+// reference shader re-evaluates six Tillotson calls per neighbour pair, sph_step.wgsl:273-275).
+#include "state.h"
+
+namespace isph {
+
+constexpr int kStackCap = 1024;
+constexpr int kStackMargin = 128;
+constexpr int kBuf = 128;
+
+// AoS record, identical layout to isph_particle (gpu_sph.rs:27-37).
+struct ParticleRec {
+    float px, py, pz, h;
+    float vx, vy, vz, u;
+    float mass;
+    uint32_t mat;
+    float rho;
+    uint32_t prov;
+};
+static_assert(sizeof(ParticleRec) == 48, "SphParticle layout");
+
+// Wave-cooperative neighbour traversal. `flush(cnt)` is called with cand[0..cnt) holding candidate sorted
+// indices (cnt <= 64); it must evaluate them for the calling lane.
+template <class Flush>
+__device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, uint32_t n, uint32_t K, uint32_t L,
+                                               const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                                               const float4* __restrict__ nB, const float4* __restrict__ nC,
+                                               float gx0, float gy0, float gz0, float gx1, float gy1, float gz1,
+                                               float hgmax, Flush&& flush) {
+    const int lane = threadIdx.x;
+    const uint32_t leaf0 = L - 1;
+    int sp = 1, nq = 0;
+    if (lane == 0) stk[0] = 0;
+    __syncthreads();
+    auto shift = [&](int cnt) {
+        __syncthreads();
+        if (lane + kWave < cnt) cand[lane] = cand[lane + kWave];
+        __syncthreads();
+    };
+    while (sp > 0) {
+        int k = min(kWave, sp);
+        int room = kStackCap - kStackMargin - sp;
+        if (room < k) k = max(1, room);
+        sp -= k;
+        const bool has = lane < k;
+        const uint32_t node = has ? stk[sp + lane] : 0u;
+        __syncthreads();
+        bool keep = false;
+        if (has) {
+            const float4 B = nB[node];
+            const float4 C = nC[node];
+            float dx = fmaxf(fmaxf(B.x - gx1, gx0 - C.x), 0.f);
+            float dy = fmaxf(fmaxf(B.y - gy1, gy0 - C.y), 0.f);
+            float dz = fmaxf(fmaxf(B.z - gz1, gz0 - C.z), 0.f);
+            float rr = 0.5f * (hgmax + B.w);
+            keep = dx * dx + dy * dy + dz * dz < rr * rr;
+        }
+        const bool leaf = keep && node >= leaf0;
+        const bool open = keep && node < leaf0;
+        const uint64_t mo = __ballot(open);
+        if (open) {
+            int pos = sp + 2 * prefix_count(mo);
+            stk[pos] = left[node];
+            stk[pos + 1] = right[node];
+        }
+        sp += 2 * __popcll(mo);
+        const uint64_t ml = __ballot(leaf);
+        if (ml) {
+            const uint32_t c = leaf ? node - leaf0 : 0u;
+            const uint32_t start = c * K;
+            const uint32_t end = leaf ? min(start + K, n) : 0u;
+            for (uint32_t kk = 0; kk < K; ++kk) {
+                const bool present = leaf && start + kk < end;
+                const uint64_t mp = __ballot(present);
+                if (!mp) break;
+                if (present) cand[nq + prefix_count(mp)] = start + kk;
+                nq += __popcll(mp);
+                if (nq >= kWave) {
+                    __syncthreads();
+                    flush(kWave);
+                    shift(nq);
+                    nq -= kWave;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (nq > 0) {
+        __syncthreads();
+        flush(nq);
+        __syncthreads();
+    }
+}
+
+// Per-particle EOS + Courant signal epilogue shared by the density and the EOS-only kernels.
+__device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, float rho, float h, float4 v, uint32_t m,
+                                             const Till* __restrict__ eos, float* srho, float* spr, float* scs,
+                                             uint32_t* scal) {
+    float sig = INFINITY;
+    if (valid) {
+        const Till& T = eos[m];
+        double p, c2;
+        till_p_c2(T, (double)rho, (double)v.w, p, c2);
+        double r = (double)rho;
+        double c = sqrt(c2);
+        srho[t] = rho;
+        spr[t] = (float)(p / (r * r));
+        scs[t] = (float)c;
+        float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+        sig = h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
+    }
+    sig = wave_min(sig);
+    if (threadIdx.x == 0 && sig < INFINITY) atomicMin(&scal[6], __float_as_uint(sig));
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
+                                                const float4* __restrict__ svel, const uint32_t* __restrict__ smat,
+                                                uint32_t n, uint32_t K, uint32_t L, const uint32_t* __restrict__ left,
+                                                const uint32_t* __restrict__ right, const float4* __restrict__ nB,
+                                                const float4* __restrict__ nC, const uint32_t* __restrict__ order,
+                                                const Till* __restrict__ eos, float* __restrict__ srho,
+                                                float* __restrict__ spr, float* __restrict__ scs,
+                                                float* __restrict__ rho_orig, uint32_t* scal,
+                                                unsigned long long* counters) {
+    __shared__ uint32_t stk[kStackCap];
+    __shared__ uint32_t cand[kBuf];
+    __shared__ float4 cP[kWave];
+    __shared__ float cH[kWave];
+    const int lane = threadIdx.x;
+    const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x);
+    const uint32_t t = g * kWave + lane;
+    const bool valid = t < n;
+    const uint32_t tc = valid ? t : n - 1;
+    const float4 me = spos[tc];
+    const float hi = sh[tc];
+    const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
+    const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
+    const float hg = wave_max(hi);
+    float rho = 0.f;
+    KahanF kr{0.f, 0.f};
+    unsigned long long tested = 0;
+    neighbour_walk(stk, cand, n, K, L, left, right, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, [&](int cnt) {
+        if (lane < cnt) {
+            uint32_t j = cand[lane];
+            cP[lane] = spos[j];
+            cH[lane] = sh[j];
+        }
+        __syncthreads();
+        float acc = 0.f;
+        for (int jj = 0; jj < cnt; ++jj) {
+            float4 pj = cP[jj];
+            float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float hij = 0.5f * (hi + cH[jj]);
+            if (r2 < hij * hij && cand[jj] != t) {  // sph_step.wgsl:201,210
+                float r = sqrtf(r2);
+                float inv_h = 1.0f / hij;
+                acc += pj.w * spline_w(r * inv_h, inv_h);
+            }
+        }
+        if (COMP) kr.add(acc);
+        else rho += acc;
+        tested += cnt;
+    });
+    if (COMP) rho = kr.s;
+    const bool live = valid && me.w > 0.f;  // retired (mass 0) particles keep their density (:185-189)
+    float rho_i;
+    if (live) {
+        float inv_h = 1.0f / hi;
+        rho_i = me.w * spline_w(0.f, inv_h) + rho;
+        rho_orig[order[t]] = rho_i;
+    } else {
+        rho_i = valid ? rho_orig[order[t]] : 1.f;
+    }
+    eos_epilogue(valid, t, rho_i, hi, svel[tc], smat[tc], eos, srho, spr, scs, scal);
+    const uint64_t mv = __ballot(valid);
+    if (lane == 0 && counters) atomicAdd(&counters[3], tested * (unsigned long long)__popcll(mv));
+}
+
+// EOS + signal only (the density of the previous evaluation is reused: positions have not moved).
+__global__ __launch_bounds__(64) void k_eos(const float* __restrict__ sh, const float4* __restrict__ svel,
+                                            const uint32_t* __restrict__ smat, uint32_t n, const Till* __restrict__ eos,
+                                            float* __restrict__ srho, float* __restrict__ spr, float* __restrict__ scs,
+                                            uint32_t* scal) {
+    const uint32_t t = blockIdx.x * kWave + threadIdx.x;
+    const bool valid = t < n;
+    const uint32_t tc = valid ? t : n - 1;
+    eos_epilogue(valid, t, srho[tc], sh[tc], svel[tc], smat[tc], eos, srho, spr, scs, scal);
+}
+
+template <bool COMP>
+__global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
+                                              const float4* __restrict__ svel, const float* __restrict__ srho,
+                                              const float* __restrict__ spr, const float* __restrict__ scs, uint32_t n,
+                                              uint32_t K, uint32_t L, const uint32_t* __restrict__ left,
+                                              const uint32_t* __restrict__ right, const float4* __restrict__ nB,
+                                              const float4* __restrict__ nC, const uint32_t* __restrict__ order,
+                                              const float4* __restrict__ sgrav, float4* __restrict__ acc_orig,
+                                              float av_alpha, float av_beta, unsigned long long* counters) {
+    __shared__ uint32_t stk[kStackCap];
+    __shared__ uint32_t cand[kBuf];
+    __shared__ float4 cP[kWave];  // x, y, z, mass
+    __shared__ float4 cV[kWave];  // vx, vy, vz, rho
+    __shared__ float4 cS[kWave];  // h, P/rho^2, c
+    const int lane = threadIdx.x;
+    const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x);
+    const uint32_t t = g * kWave + lane;
+    const bool valid = t < n;
+    const uint32_t tc = valid ? t : n - 1;
+    const float4 me = spos[tc];
+    const float hi = sh[tc];
+    const float4 vi = svel[tc];
+    const float rhoi = srho[tc], pri = spr[tc], ci = scs[tc];
+    const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
+    const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
+    const float hg = wave_max(hi);
+    float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f;
+    KahanF kx{0.f, 0.f}, ky{0.f, 0.f}, kz{0.f, 0.f}, ke{0.f, 0.f};
+    unsigned long long tested = 0;
+    neighbour_walk(stk, cand, n, K, L, left, right, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, [&](int cnt) {
+        if (lane < cnt) {
+            uint32_t j = cand[lane];
+            cP[lane] = spos[j];
+            float4 v = svel[j];
+            cV[lane] = make_float4(v.x, v.y, v.z, srho[j]);
+            cS[lane] = make_float4(sh[j], spr[j], scs[j], 0.f);
+        }
+        __syncthreads();
+        for (int jj = 0; jj < cnt; ++jj) {
+            float4 pj = cP[jj];
+            float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;  // dpos = pos_i - pos_j
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float4 sj = cS[jj];
+            float hij = 0.5f * (hi + sj.x);
+            if (r2 < hij * hij && r2 > 1.0e-18f && cand[jj] != t) {  // r < hij && r > 1e-9 (:272)
+                float4 vj = cV[jj];
+                float r = sqrtf(r2);
+                float inv_h = 1.0f / hij;
+                float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+                float vr = dvx * dx + dvy * dy + dvz * dz;
+                float pij = 0.f;
+                if (vr < 0.f) {  // Monaghan viscosity (:279-284)
+                    float mu = hij * vr / (r2 + 0.01f * hij * hij);
+                    float cbar = 0.5f * (ci + sj.z);
+                    float rbar = 0.5f * (rhoi + vj.w);
+                    pij = (-av_alpha * cbar * mu + av_beta * mu * mu) / rbar;
+                }
+                float coeff = pri + sj.y + pij;
+                float gs = spline_dw(r * inv_h, inv_h) / r;
+                float gx = dx * gs, gy = dy * gs, gz = dz * gs;
+                float f = -coeff * pj.w;
+                float e = 0.5f * pj.w * coeff * (dvx * gx + dvy * gy + dvz * gz);
+                if (COMP) {
+                    kx.add(gx * f);
+                    ky.add(gy * f);
+                    kz.add(gz * f);
+                    ke.add(e);
+                } else {
+                    ax += gx * f;
+                    ay += gy * f;
+                    az += gz * f;
+                    de += e;
+                }
+            }
+        }
+        tested += cnt;
+    });
+    if (COMP) {
+        ax = kx.s;
+        ay = ky.s;
+        az = kz.s;
+        de = ke.s;
+    }
+    if (valid) {
+        float4 gr = sgrav[t];
+        float4 out = me.w > 0.f ? make_float4(gr.x + ax, gr.y + ay, gr.z + az, de) : make_float4(0.f, 0.f, 0.f, 0.f);
+        acc_orig[order[t]] = out;
+    }
+    const uint64_t mv = __ballot(valid);
+    if (lane == 0 && counters) atomicAdd(&counters[4], tested * (unsigned long long)__popcll(mv));
+}
+
+// ---- KDK (sph_step.wgsl:304-333; hydrostatic.rs:451-465), retired particles are not integrated ----------
+__global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, float4* velu,
+                                                    const float* __restrict__ mass, const float4* __restrict__ acc,
+                                                    const float* __restrict__ dtp, double* pos64, double* vel64) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(mass[i] > 0.f)) return;
+    const float dt = *dtp;
+    const float4 a = acc[i];
+    float4 v = velu[i];
+    float4 p = posh[i];
+    v.w = fmaxf(v.w + a.w * (0.5f * dt), 0.0f);
+    if (pos64) {
+        double hd = 0.5 * (double)dt;
+        double vx = vel64[3 * i] + (double)a.x * hd, vy = vel64[3 * i + 1] + (double)a.y * hd,
+               vz = vel64[3 * i + 2] + (double)a.z * hd;
+        double x = pos64[3 * i] + vx * (double)dt, y = pos64[3 * i + 1] + vy * (double)dt,
+               z = pos64[3 * i + 2] + vz * (double)dt;
+        vel64[3 * i] = vx;
+        vel64[3 * i + 1] = vy;
+        vel64[3 * i + 2] = vz;
+        pos64[3 * i] = x;
+        pos64[3 * i + 1] = y;
+        pos64[3 * i + 2] = z;
+        v.x = (float)vx;
+        v.y = (float)vy;
+        v.z = (float)vz;
+        p.x = (float)x;
+        p.y = (float)y;
+        p.z = (float)z;
+    } else {
+        v.x += a.x * (0.5f * dt);
+        v.y += a.y * (0.5f * dt);
+        v.z += a.z * (0.5f * dt);
+        p.x += v.x * dt;
+        p.y += v.y * dt;
+        p.z += v.z * dt;
+    }
+    velu[i] = v;
+    posh[i] = p;
+}
+
+__global__ __launch_bounds__(256) void k_kick(uint32_t n, float4* velu, const float* __restrict__ mass,
+                                              const float4* __restrict__ acc, const float* __restrict__ dtp,
+                                              double* vel64) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(mass[i] > 0.f)) return;
+    const float dt = *dtp;
+    const float4 a = acc[i];
+    float4 v = velu[i];
+    v.w = fmaxf(v.w + a.w * (0.5f * dt), 0.0f);
+    if (vel64) {
+        double hd = 0.5 * (double)dt;
+        double vx = vel64[3 * i] + (double)a.x * hd, vy = vel64[3 * i + 1] + (double)a.y * hd,
+               vz = vel64[3 * i + 2] + (double)a.z * hd;
+        vel64[3 * i] = vx;
+        vel64[3 * i + 1] = vy;
+        vel64[3 * i + 2] = vz;
+        v.x = (float)vx;
+        v.y = (float)vy;
+        v.z = (float)vz;
+    } else {
+        v.x += a.x * (0.5f * dt);
+        v.y += a.y * (0.5f * dt);
+        v.z += a.z * (0.5f * dt);
+    }
+    velu[i] = v;
+}
+
+// dt = cfl * min_i h_i / max(c_i + |v_i|, 1)  (impact-run main.rs:345-356 done on the device).
+__global__ void k_set_dt_cfl(const uint32_t* scal, float cfl, float* dt) {
+    float m = __uint_as_float(scal[6]);
+    *dt = (m < INFINITY) ? cfl * m : 0.0f;
+}
+__global__ void k_set_dt_value(float v, float* dt) { *dt = v; }
+__global__ void k_advance_time(const float* dt, double* time) { *time += (double)*dt; }
+
+__global__ __launch_bounds__(256) void k_unpack(uint32_t n, const ParticleRec* __restrict__ rec, float4* posh,
+                                                float4* velu, float* mass, uint32_t* mat, uint32_t* prov, float* rho) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ParticleRec r = rec[i];
+    posh[i] = make_float4(r.px, r.py, r.pz, r.h);
+    velu[i] = make_float4(r.vx, r.vy, r.vz, r.u);
+    mass[i] = r.mass;
+    mat[i] = r.mat;
+    prov[i] = r.prov;
+    rho[i] = r.rho;
+}
+__global__ __launch_bounds__(256) void k_pack(uint32_t n, ParticleRec* __restrict__ rec, const float4* posh,
+                                              const float4* velu, const float* mass, const uint32_t* mat,
+                                              const uint32_t* prov, const float* rho) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = posh[i], v = velu[i];
+    ParticleRec r;
+    r.px = p.x; r.py = p.y; r.pz = p.z; r.h = p.w;
+    r.vx = v.x; r.vy = v.y; r.vz = v.z; r.u = v.w;
+    r.mass = mass[i];
+    r.mat = mat[i];
+    r.rho = rho[i];
+    r.prov = prov[i];
+    rec[i] = r;
+}
+__global__ __launch_bounds__(256) void k_sync64(uint32_t n, const float4* posh, const float4* velu, double* pos64,
+                                                double* vel64) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = posh[i], v = velu[i];
+    pos64[3 * i] = p.x; pos64[3 * i + 1] = p.y; pos64[3 * i + 2] = p.z;
+    vel64[3 * i] = v.x; vel64[3 * i + 1] = v.y; vel64[3 * i + 2] = v.z;
+}
+
+// ---- f64 diagnostics: per-block partials, summed on the host in a fixed order ------------------------------
+constexpr int kDiagBlocks = 1024;
+constexpr int kDiagFields = 16;
+
+__device__ __forceinline__ void load_pos64(uint32_t i, const float4* posh, const double* pos64, double* x) {
+    if (pos64) {
+        for (int k = 0; k < 3; ++k) x[k] = pos64[3 * i + k];
+    } else {
+        float4 p = posh[i];
+        x[0] = p.x; x[1] = p.y; x[2] = p.z;
+    }
+}
+
+__device__ __forceinline__ void load_state64(uint32_t i, const float4* posh, const float4* velu, const double* pos64,
+                                             const double* vel64, double* x, double* v) {
+    if (pos64) {
+        for (int k = 0; k < 3; ++k) {
+            x[k] = pos64[3 * i + k];
+            v[k] = vel64[3 * i + k];
+        }
+    } else {
+        float4 p = posh[i], q = velu[i];
+        x[0] = p.x; x[1] = p.y; x[2] = p.z;
+        v[0] = q.x; v[1] = q.y; v[2] = q.z;
+    }
+}
+
+// Fields: 0 KE, 1 IE, 2 mass, 3-5 sum m x, 6-8 sum m v, 9-11 sum m (x cross v), 12 PE partial.
+__global__ __launch_bounds__(256) void k_diag_sums(uint32_t n, const float4* posh, const float4* velu,
+                                                   const double* pos64, const double* vel64, const float* mass,
+                                                   double* red) {
+    __shared__ double part[4][12];
+    double acc[12];
+    for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double x[3], v[3];
+        load_state64(i, posh, velu, pos64, vel64, x, v);
+        double m = mass[i];
+        double u = velu[i].w;
+        acc[0] += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        acc[1] += m * u;
+        acc[2] += m;
+        for (int k = 0; k < 3; ++k) {
+            acc[3 + k] += m * x[k];
+            acc[6 + k] += m * v[k];
+        }
+        acc[9] += m * (x[1] * v[2] - x[2] * v[1]);
+        acc[10] += m * (x[2] * v[0] - x[0] * v[2]);
+        acc[11] += m * (x[0] * v[1] - x[1] * v[0]);
+    }
+    int w = threadIdx.x / kWave;
+    for (int k = 0; k < 12; ++k) {
+        double s = wave_sum_f64(acc[k]);
+        if (lane_id() == 0) part[w][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        double s = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        red[blockIdx.x * kDiagFields + threadIdx.x] = s;
+    }
+}
+
+// Softened potential energy, exact O(N^2) in f64: PE = -G/2 sum_i m_i sum_{j != i} m_j / sqrt(r^2 + s^2)
+// (gpu_sph.rs:940-961 sums i<j on the CPU). 1/sqrt by an f32 seed plus two Newton steps in f64.
+__global__ __launch_bounds__(256) void k_diag_pe(uint32_t n, const float4* posh, const double* pos64,
+                                                 const float* mass, double soft2, double* red) {
+    __shared__ double tx[256], ty[256], tz[256], tm[256];
+    double pe_block = 0.0;
+    for (uint32_t ib = blockIdx.x * 256; ib < n; ib += gridDim.x * 256) {
+        uint32_t i = ib + threadIdx.x;
+        double xi[3] = {0, 0, 0};
+        double mi = 0.0;
+        if (i < n) {
+            load_pos64(i, posh, pos64, xi);
+            mi = mass[i];
+        }
+        double phi = 0.0;
+        for (uint32_t jb = 0; jb < n; jb += 256) {
+            uint32_t j = jb + threadIdx.x;
+            __syncthreads();
+            if (j < n) {
+                double xj[3];
+                load_pos64(j, posh, pos64, xj);
+                tx[threadIdx.x] = xj[0];
+                ty[threadIdx.x] = xj[1];
+                tz[threadIdx.x] = xj[2];
+                tm[threadIdx.x] = mass[j];
+            } else {
+                tx[threadIdx.x] = ty[threadIdx.x] = tz[threadIdx.x] = 0.0;
+                tm[threadIdx.x] = 0.0;
+            }
+            __syncthreads();
+            uint32_t cnt = min(256u, n - jb);
+            for (uint32_t jj = 0; jj < cnt; ++jj) {
+                double dx = tx[jj] - xi[0], dy = ty[jj] - xi[1], dz = tz[jj] - xi[2];
+                double r2 = dx * dx + dy * dy + dz * dz + soft2;
+                double y = (double)__builtin_amdgcn_rsqf((float)r2);
+                y = y * (1.5 - 0.5 * r2 * y * y);
+                y = y * (1.5 - 0.5 * r2 * y * y);
+                phi += (jb + jj == i) ? 0.0 : tm[jj] * y;
+            }
+        }
+        pe_block += mi * phi;
+    }
+    __shared__ double part[4];
+    double s = wave_sum_f64(pe_block);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) red[blockIdx.x * kDiagFields + 12] = -0.5 * kG64 * (part[0] + part[1] + part[2] + part[3]);
+}
+
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
+    uint32_t groups = cdiv(d.n, kWave);
+    if (p.mixed)
+        hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
+                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.scal,
+                           d.counters);
+    else
+        hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
+                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.scal,
+                           d.counters);
+}
+void launch_eos(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_eos, dim3(cdiv(d.n, kWave)), dim3(kWave), 0, s, d.sh, d.svel, d.smat, d.n, d.eos, d.srho,
+                       d.spr, d.scs, d.scal);
+}
+void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
+    uint32_t groups = cdiv(d.n, kWave);
+    if (p.mixed)
+        hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
+                           d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc, p.av_alpha, p.av_beta,
+                           d.counters);
+    else
+        hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
+                           d.scs, d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc, p.av_alpha,
+                           p.av_beta, d.counters);
+}
+void launch_kick_drift(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,
+                       d.pos64, d.vel64);
+}
+void launch_kick(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_kick, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.velu, d.mass, d.acc, d.dt, d.vel64);
+}
+void launch_set_dt_cfl(const Dev& d, float cfl, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_dt_cfl, dim3(1), dim3(1), 0, s, d.scal, cfl, d.dt);
+}
+void launch_set_dt_value(const Dev& d, float dt, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_dt_value, dim3(1), dim3(1), 0, s, dt, d.dt);
+}
+void launch_advance_time(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, s, d.dt, d.time);
+}
+void launch_unpack(const Dev& d, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_unpack, dim3(cdiv(n, 256)), dim3(256), 0, s, n, (const ParticleRec*)d.aos, d.posh, d.velu,
+                       d.mass, d.mat, d.prov, d.rho);
+}
+void launch_pack(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, (ParticleRec*)d.aos, d.posh, d.velu, d.mass,
+                       d.mat, d.prov, d.rho);
+}
+void launch_sync64_from32(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_sync64, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.pos64, d.vel64);
+}
+void launch_diag(const Dev& d, float soft, int with_pe, hipStream_t s) {
+    hipLaunchKernelGGL(k_diag_sums, dim3(kDiagBlocks), dim3(256), 0, s, d.n, d.posh, d.velu, d.pos64, d.vel64, d.mass,
+                       d.red);
+    if (with_pe) {
+        uint32_t blocks = std::min<uint32_t>(cdiv(d.n, 256), kDiagBlocks);
+        hipLaunchKernelGGL(k_diag_pe, dim3(blocks), dim3(256), 0, s, d.n, d.posh, d.pos64, d.mass,
+                           (double)soft * (double)soft, d.red);
+    }
+}
+
+}  // namespace isph

@@ -1,0 +1,113 @@
+// Device buffers of one stepper handle and the launcher interface between the host code and the kernels.
+#pragma once
+
+#include <algorithm>
+
+#include "common.h"
+
+namespace isph {
+
+// All device arrays. "orig" arrays are in upload order; "sorted" arrays in Morton order of the last
+// evaluation (order[t] = upload index of sorted slot t).
+struct Dev {
+    uint32_t n = 0;     // live particles
+    uint32_t cap = 0;   // capacity
+    uint32_t K = 1;     // leaf bucket size
+    uint32_t L = 1;     // leaves = ceil(n / K)
+
+    // master state, upload order
+    float4* posh = nullptr;   // x, y, z, h
+    float4* velu = nullptr;   // vx, vy, vz, u
+    float* mass = nullptr;
+    uint32_t* mat = nullptr;
+    uint32_t* prov = nullptr;
+    float* rho = nullptr;
+    float4* acc = nullptr;    // ax, ay, az, du/dt of the last evaluation
+    double* pos64 = nullptr;  // mixed precision: x, y, z per particle (3n)
+    double* vel64 = nullptr;  // mixed precision: vx, vy, vz per particle (3n)
+
+    // Morton + sort
+    uint32_t* keys = nullptr;  // codes
+    uint32_t* vals = nullptr;  // order
+    uint32_t* keys_alt = nullptr;
+    uint32_t* vals_alt = nullptr;
+    uint32_t* hist = nullptr;  // 256 * sort blocks
+    uint32_t nsort_blocks = 0;
+
+    // sorted working copies
+    float4* spos = nullptr;  // x, y, z, mass
+    float4* svel = nullptr;  // vx, vy, vz, u
+    float* sh = nullptr;
+    uint32_t* smat = nullptr;
+    float* srho = nullptr;
+    float* spr = nullptr;    // P / rho^2
+    float* scs = nullptr;    // sound speed
+    float4* sgrav = nullptr; // gravitational acceleration, sorted
+    uint32_t* lcode = nullptr;  // leaf cluster codes [L]
+
+    // tree: internal [0, L-1), leaves [L-1, 2L-1)
+    uint32_t* left = nullptr;
+    uint32_t* right = nullptr;
+    uint32_t* parent = nullptr;
+    uint32_t* ready = nullptr;
+    float4* nA = nullptr;  // com.xyz, mass
+    float4* nB = nullptr;  // bmin.xyz, max h
+    float4* nC = nullptr;  // bmax.xyz, MAC radius size/theta + |centre - com|
+    float4* nQ0 = nullptr; // quadrupole xx, yy, zz, xy
+    float2* nQ1 = nullptr; // quadrupole xz, yz
+
+    // scalars
+    uint32_t* scal = nullptr;  // [0..5] bbox keys, [6] Courant signal min (float bits), [7] spare
+    float* dt = nullptr;       // current dt
+    double* time = nullptr;    // accumulated simulated time
+    unsigned long long* counters = nullptr;  // [0] cells, [1] particles, [2] quad cells, [3] dens pairs, [4] force pairs
+    double* red = nullptr;     // reduction partials (diagnostics)
+
+    Till* eos = nullptr;       // EOS table [kMaxMaterials]
+
+    // read-back / upload staging
+    void* aos = nullptr;       // device AoS records
+};
+
+struct StepParams {
+    float soft2 = 0.0f;
+    float theta = 0.5f;
+    float av_alpha = 1.0f;
+    float av_beta = 2.0f;
+    int quad = 0;
+    int mixed = 0;
+};
+
+// ---- launchers (each .hip file implements its own) ----
+void launch_reset_scalars(const Dev& d, hipStream_t s);
+void launch_bbox(const Dev& d, const float4* pos, hipStream_t s);
+void launch_morton(const Dev& d, const float4* pos, hipStream_t s);
+void launch_radix_sort(const Dev& d, hipStream_t s);  // 30-bit keys, 4 passes; result in keys/vals
+void launch_gather(const Dev& d, const float4* pos, const float* mass, hipStream_t s);
+void launch_gather_vel(const Dev& d, hipStream_t s);
+
+void launch_karras(const Dev& d, hipStream_t s);
+void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s);
+
+void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s);
+void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s);
+void launch_gravity_zero(const Dev& d, hipStream_t s);
+void launch_scatter_gravity(const Dev& d, hipStream_t s);  // sgrav (sorted) -> acc (orig), for the gravity API
+
+void launch_density(const Dev& d, const StepParams& p, hipStream_t s);
+void launch_eos(const Dev& d, hipStream_t s);
+void launch_force(const Dev& d, const StepParams& p, hipStream_t s);
+
+void launch_kick_drift(const Dev& d, hipStream_t s);
+void launch_kick(const Dev& d, hipStream_t s);
+void launch_set_dt_cfl(const Dev& d, float cfl, hipStream_t s);
+void launch_set_dt_value(const Dev& d, float dt, hipStream_t s);
+void launch_advance_time(const Dev& d, hipStream_t s);
+
+void launch_unpack(const Dev& d, uint32_t n, hipStream_t s);
+void launch_pack(const Dev& d, hipStream_t s);
+void launch_sync64_from32(const Dev& d, hipStream_t s);
+
+void launch_diag(const Dev& d, float soft, int with_pe, hipStream_t s);  // writes d.red[0..7]
+
+}  // namespace isph

@@ -1,0 +1,171 @@
+/*
+ * isph.h — C-ABI of the MI355X particle physics step (LBVH Barnes-Hut gravity + SPH/Tillotson + KDK).
+ *
+ * Drop-in boundary for integrity-engine's GPU SPH stepper and GPU gravity. Each entry point names the
+ * reference interface it replaces (paths relative to integrity-engine/). The wgpu `&Device`, `&Queue` and
+ * `&mut CommandEncoder` arguments of the reference cannot cross to HIP: they are replaced by the opaque
+ * handle, which owns one HIP stream. "encode" enqueues on that stream; completion is observed through
+ * the read-back calls. One handle is used by one thread at a time (not internally locked). No C++
+ * exception crosses this boundary: every call returns a status (>= 0 ok, < 0 an ISPH_E* code) and
+ * isph_last_error() describes the last failure.
+ */
+#ifndef ISPH_H
+#define ISPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-particle record, 48 bytes: `SphParticle` crates/engine/src/gpu_sph.rs:27-37 == WGSL `Particle`
+ * shaders/sph_step.wgsl:45-49. SI units. */
+typedef struct isph_particle {
+    float pos[3];
+    float h;
+    float vel[3];
+    float u;
+    float mass;
+    uint32_t mat;
+    float rho;
+    uint32_t prov;
+} isph_particle;
+
+/* Tillotson parameters, 48 bytes: `SphEos` gpu_sph.rs:42-55 == WGSL `Eos` sph_step.wgsl:50-54. */
+typedef struct isph_eos {
+    float rho0, a, b, cap_a, cap_b, e0, e_iv, e_cv, alpha, beta, _p0, _p1;
+} isph_eos;
+
+#ifdef __cplusplus
+static_assert(sizeof(isph_particle) == 48, "SphParticle is 48 bytes (gpu_sph.rs:1520-1598)");
+static_assert(sizeof(isph_eos) == 48, "SphEos is 48 bytes");
+static_assert(offsetof(isph_particle, h) == 12 && offsetof(isph_particle, vel) == 16 &&
+                  offsetof(isph_particle, u) == 28 && offsetof(isph_particle, mass) == 32 &&
+                  offsetof(isph_particle, mat) == 36 && offsetof(isph_particle, rho) == 40 &&
+                  offsetof(isph_particle, prov) == 44,
+              "field offsets of the WGSL Particle struct");
+#endif
+
+typedef struct isph_ctx isph_t;
+
+/* Status codes. */
+#define ISPH_OK 0
+#define ISPH_EAGAIN (-11) /* read-back not ready yet (take_readback's `None`) */
+#define ISPH_EINVAL (-22)
+#define ISPH_ENOMEM (-12)
+#define ISPH_EDEVICE (-5) /* a HIP runtime call failed */
+#define ISPH_ENODEV (-19)
+#define ISPH_ERANGE (-34)
+
+/* Gravity modes (isph_set_gravity). */
+#define ISPH_GRAVITY_TREE 0   /* LBVH Barnes-Hut: bh_gravity.wgsl:122-394 */
+#define ISPH_GRAVITY_DIRECT 1 /* exact softened direct sum: sph_step.wgsl:235-241 */
+#define ISPH_GRAVITY_NONE 2   /* SPH forces only (parity probes) */
+
+/* Precision modes (isph_set_precision). */
+#define ISPH_PRECISION_FP32 0  /* plain: f32 state, f32 pair sums (the reference's GPU arithmetic) */
+#define ISPH_PRECISION_MIXED 1 /* f64 position/velocity state, f32 pairs with compensated sums */
+
+/* Phase indices of isph_phase_times. */
+#define ISPH_PHASE_SORT 0      /* bbox + Morton + radix sort + gather */
+#define ISPH_PHASE_TREE 1      /* Karras build */
+#define ISPH_PHASE_UPSWEEP 2   /* moments */
+#define ISPH_PHASE_GRAVITY 3
+#define ISPH_PHASE_DENSITY 4   /* density + per-particle EOS + Courant signal */
+#define ISPH_PHASE_FORCE 5     /* SPH force + du/dt (+ gravity add, scatter) */
+#define ISPH_PHASE_INTEGRATE 6 /* kick-drift, kick, dt */
+#define ISPH_NUM_PHASES 7
+
+/* ---- stepper (GpuSph, gpu_sph.rs:1086-1505) ------------------------------------------------------ */
+
+/* GpuSph::new(device, capacity) gpu_sph.rs:1116. `device` is the HIP ordinal (the adapter choice of
+ * gpu_host.rs:31). Capacity is clamped to >= 1 like the reference (:1117). NULL on failure. */
+isph_t* isph_create(uint32_t capacity, int device);
+void isph_destroy(isph_t* h);
+/* Last error message of `h` (or of the calling thread's last isph_create when h is NULL). */
+const char* isph_last_error(const isph_t* h);
+
+/* GpuSph::upload(queue, particles, eos, softening) gpu_sph.rs:1323-1340: particles beyond capacity are
+ * dropped (gpu_store.rs:33), the EOS table is truncated to 16 entries (:1337). Returns the count kept.
+ * Every particle's `mat` must index the table (ISPH_EINVAL otherwise). */
+int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_eos* eos, size_t n_eos,
+                float softening);
+/* Replace the EOS table with f64 parameters (10 per material, eos.rs:27-47 field order). The f32 table
+ * of isph_upload is otherwise widened exactly; this lets the per-particle FP64 EOS use the catalogue
+ * values unrounded, as the f64 CPU path (`eos::Tillotson`) does. */
+int isph_set_eos_f64(isph_t* h, const double* params10, size_t n_eos);
+/* GpuSph::count gpu_sph.rs:1449 */
+uint32_t isph_count(const isph_t* h);
+/* GpuSph::set_dt(queue, dt, damp) gpu_sph.rs:1343 */
+int isph_set_dt(isph_t* h, float dt, float damp);
+/* GpuSph::set_av(queue, alpha, beta) gpu_sph.rs:1353 */
+int isph_set_av(isph_t* h, float alpha, float beta);
+/* Gravity configuration (the reference hard-codes THETA=0.5, BUCKET_K=1: gpu_gravity.rs:55,62).
+ * leaf_size in [1, 64]; quadrupole != 0 adds the node quadrupole to accepted cells. */
+int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int quadrupole);
+int isph_set_precision(isph_t* h, int mode);
+/* Reuse the tree, gravity and density of the previous evaluation when positions have not moved since
+ * (eval 2 of step n and eval 1 of step n+1 see identical positions). Results are bit-identical. */
+int isph_set_reuse(isph_t* h, int enable);
+
+/* GpuSph::encode_kdk(enc, substeps) gpu_sph.rs:1484-1491: `substeps` KDK steps at the set_dt dt, each
+ * = eval -> half-kick + drift -> eval -> half-kick (sph_step.wgsl:304-333). Enqueued, returns at once. */
+int isph_encode_kdk(isph_t* h, uint32_t substeps);
+/* The adaptive-dt KDK loop of tools/impact-run/src/main.rs:394-434 with the Courant reduction
+ * (sph_step.wgsl:364-371, main.rs:345-356) done on the device: no host round trip per step. */
+int isph_step_cfl(isph_t* h, float cfl, uint32_t steps);
+/* Simulated time accumulated by isph_step_cfl / isph_encode_kdk (seconds, f64), and the last dt. */
+int isph_sim_time(isph_t* h, double* t, float* last_dt);
+
+/* GpuSph::begin_readback gpu_sph.rs:1497 / gpu_store.rs:126-153: no-op when empty or in flight. */
+int isph_begin_readback(isph_t* h);
+/* GpuSph::take_readback gpu_sph.rs:1502 / gpu_store.rs:156-170: the snapshot's particle count, or
+ * ISPH_EAGAIN while pending or idle (the reference's `None`). Never blocks. */
+int isph_take_readback(isph_t* h, isph_particle* out, size_t cap);
+/* Blocking read-back for native callers (the offline driver's read_particles, main.rs:332-343). */
+int isph_readback_sync(isph_t* h, isph_particle* out, size_t cap);
+int isph_synchronize(isph_t* h);
+
+/* One force evaluation at the current state (the reference's force_eval gpu_sph.rs:1468-1474:
+ * density + forces). Writes, in upload order: acc4 = (ax, ay, az, du/dt) per particle, rho. Either
+ * output may be NULL. */
+int isph_force_eval(isph_t* h, float* acc4, float* rho);
+
+/* ---- gravity (GpuGravity, gpu_gravity.rs:79-561) --------------------------------------------------- */
+/* GpuGravity::accelerations (mode ISPH_GRAVITY_DIRECT, :187) / accelerations_tree (mode
+ * ISPH_GRAVITY_TREE, :207, theta and leaf size from isph_set_gravity): f64 in, f32 on the device, f64
+ * out. n must not exceed the handle's capacity. Does not touch the stepper's particle state. */
+int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass, size_t n, double softening,
+                               double* acc3, int mode);
+
+/* ---- diagnostics ---------------------------------------------------------------------------------- */
+/* Totals in f64 over the live particles: out[0]=KE, [1]=IE, [2]=softened PE (exact O(N^2) device sum
+ * when with_pe != 0, else 0), [3..5]=angular momentum about the COM, [6]=total mass.
+ * gpu_sph.rs:940-961, impact-run main.rs:452-472, 528-536. */
+int isph_diagnostics(isph_t* h, int with_pe, double* out7);
+/* Accumulated per-phase milliseconds (ISPH_PHASE_*) since the last reset, when timing is enabled. */
+int isph_set_timing(isph_t* h, int enable);
+int isph_phase_times(isph_t* h, double* ms, size_t n);
+/* Gravity-kernel statistics since the last reset: out[0] = launches, [1] = total kernel ms (HIP events on
+ * the handle's stream), [2] = monopole cell interactions, [3] = particle-particle interactions,
+ * [4] = quadrupole cell interactions, [5] = density pairs tested, [6] = force pairs tested. */
+int isph_kernel_stats(isph_t* h, double* out7);
+int isph_reset_stats(isph_t* h);
+
+/* ---- stage probes (parity of the bit-exact stages against the oracle) ------------------------------- */
+/* bbox + Morton of the current positions: codes in upload order, lo3/hi3 the f32 bbox (cs_bbox/cs_morton,
+ * bh_gravity.wgsl:67-108). */
+int isph_probe_morton(isph_t* h, uint32_t* codes, float* lo3, float* hi3);
+/* Sorted (code, index) pairs (the CPU sort of gpu_gravity.rs:378-380, done by the device radix sort). */
+int isph_probe_sort(isph_t* h, uint32_t* sorted_codes, uint32_t* order);
+/* Karras tree and moments over L = ceil(n / leaf_size) leaves (bh_gravity.wgsl:122-241): left/right
+ * [L-1], parent [2L-1], com4 [2L-1][4] (xyz, mass), bmin4/bmax4 [2L-1][4] (xyz, w = max h / MAC radius).
+ * Returns L. Any output may be NULL. */
+int isph_probe_tree(isph_t* h, uint32_t* left, uint32_t* right, uint32_t* parent, float* com4, float* bmin4,
+                    float* bmax4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISPH_H */

@@ -1,0 +1,94 @@
+"""Shared generators (the reference tests' own seeds and RNGs) and comparison metrics."""
+import numpy as np
+
+from botheadstudios_amd import materials, scene
+
+G = 6.674e-11
+
+
+def splitmix_stream(seed):
+    """bhtree.rs:280-286 / neighbors.rs:98-104 splitmix64 -> [0,1) doubles, sequential."""
+    rng = scene.SplitMix64(seed)
+    return lambda k: rng.uniform(k)
+
+
+def splitmix_cloud(n, seed, box=1.0e6):
+    """bhtree.rs:359-364 / gpu-bh-verify build_cloud (main.rs:124-137): per body x, y, z then mass."""
+    u = splitmix_stream(seed)(4 * n).reshape(n, 4)
+    pos = u[:, :3] * box
+    mass = 1.0e18 * (0.5 + u[:, 3])
+    return pos, mass
+
+
+def xorshift_debris(n, seed):
+    """gpu_gravity.rs:751-768 debris_cloud: xorshift64, normalize(rng3) * 1.7e6 * (0.2 + |rng|)."""
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    out = np.zeros((n, 3))
+
+    def rng():
+        nonlocal s
+        s ^= (s << 13) & 0xFFFFFFFFFFFFFFFF
+        s ^= s >> 7
+        s ^= (s << 17) & 0xFFFFFFFFFFFFFFFF
+        return (s >> 40) / float(1 << 24) - 0.5
+
+    for i in range(n):
+        v = np.array([rng(), rng(), rng()])
+        nv = np.linalg.norm(v)
+        v = v / nv if nv > 0 else np.zeros(3)
+        out[i] = v * (1.7e6 * (0.2 + abs(rng())))
+    return out, np.full(n, 5.0e19)
+
+
+def particles_from(pos, mass, h=1.0, u=0.0, vel=None, mat=0):
+    n = len(mass)
+    rec = np.zeros(n, dtype=scene.PARTICLE_DTYPE)
+    rec["pos"] = np.asarray(pos, np.float32)
+    rec["mass"] = np.asarray(mass, np.float32)
+    rec["h"] = h
+    rec["u"] = u
+    rec["mat"] = mat
+    if vel is not None:
+        rec["vel"] = np.asarray(vel, np.float32)
+    return rec
+
+
+def rms_rel(a, b):
+    """gpu-bh-verify compare (main.rs:163-180): ||sum err|| / ||sum ref|| and the worst particle."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    e = ((a - b) ** 2).sum(axis=1)
+    r = (b ** 2).sum(axis=1)
+    return float(np.sqrt(e.sum() / max(r.sum(), 1e-300))), float(np.max(np.sqrt(e) / np.maximum(np.sqrt(r), 1e-30)))
+
+
+def rel_err_floor(x, ref, floor_frac=1e-3):
+    """Per-particle relative error with denominator max(|ref|, floor_frac * RMS(|ref|)) (vectors or scalars)."""
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    if x.ndim == 1:
+        x = x[:, None]
+        ref = ref[:, None]
+    num = np.sqrt(((x - ref) ** 2).sum(axis=1))
+    mag = np.sqrt((ref ** 2).sum(axis=1))
+    den = np.maximum(mag, floor_frac * np.sqrt(np.mean(mag ** 2)))
+    den = np.maximum(den, 1e-300)
+    return num / den
+
+
+def with_random_velocities(sc, amp_frac=0.05, seed=0x5EED):
+    """Add seeded random velocities (a fraction of the target sound speed scale) so the artificial
+    viscosity and du/dt are exercised (sph-verify build_config gives +/-200 m/s, main.rs:279)."""
+    rng = scene.SplitMix64(seed)
+    v = (rng.uniform(3 * len(sc.particles)).reshape(-1, 3) * 2 - 1) * 300.0
+    rec = sc.particles.copy()
+    rec["vel"] = (rec["vel"].astype(np.float64) + v).astype(np.float32)
+    return rec
+
+
+def hydro_from(rec, eos64, softening, gravity="none"):
+    from oracle import oracle
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, eos64, softening)
+    hb.configure(theta=0.5, av_alpha=1.0, av_beta=2.0, gravity=gravity)
+    return hb

@@ -1,0 +1,302 @@
+"""GPU parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Bit-exact: Morton codes + bbox, the sorted (code, index) permutation, the Karras topology (every leaf
+size). Tolerance: gravity against the f64 direct sum within the reference's own bounds
+(gpu_gravity.rs:716-750, 852-924; gpu-bh-verify main.rs:527-575), SPH density / acceleration / du/dt
+against the f64 CPU path (hydrostatic.rs:338-446) within 1e-5 relative per particle (denominator floored
+at 1e-3 of the RMS magnitude), KDK trajectories and the device Courant dt against the CPU step.
+"""
+import numpy as np
+import pytest
+
+from botheadstudios_amd import isph, materials, scene
+from oracle import oracle
+
+from helpers import (hydro_from, particles_from, rel_err_floor, rms_rel, splitmix_cloud, with_random_velocities,
+                     xorshift_debris)
+
+pytestmark = pytest.mark.gpu
+
+EOS = materials.eos_table()
+EOS64 = materials.eos_f64()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return scene.two_body_8k()
+
+
+def _sim(rec, soft, cap=None, eos64=True):
+    s = isph.GpuSph(cap or len(rec))
+    assert s.upload(rec, EOS, soft) == len(rec)
+    if eos64:
+        s.set_eos_f64(EOS64)
+    return s
+
+
+def _dup_cloud():
+    # many coincident / quantised positions: exercises equal codes and the index tie-break
+    pos, mass = splitmix_cloud(3000, 0x51A17EED)
+    pos = np.round(pos / 2.0e5) * 2.0e5
+    pos[100:140] = pos[7]
+    return pos, mass
+
+
+CLOUDS = {
+    "bh_verify_1500": lambda: splitmix_cloud(1500, 0xABCD1234),
+    "debris_6000": lambda: xorshift_debris(6000, 0x7EE52707),
+    "duplicates_3000": _dup_cloud,
+}
+
+
+@pytest.mark.parametrize("name", list(CLOUDS))
+def test_morton_and_bbox_bit_exact(name):
+    pos, mass = CLOUDS[name]()
+    rec = particles_from(pos, mass)
+    with _sim(rec, 5e3, eos64=False) as s:
+        codes, lo, hi = s.probe_morton()
+    olo, ohi = oracle.bbox_f32(rec["pos"])
+    assert np.array_equal(lo.view(np.uint32), olo.view(np.uint32))
+    assert np.array_equal(hi.view(np.uint32), ohi.view(np.uint32))
+    assert np.array_equal(codes, oracle.morton_f32(rec["pos"], olo, ohi))
+
+
+def test_morton_c1_scene_bit_exact(c1):
+    with _sim(c1.particles, c1.softening) as s:
+        codes, lo, hi = s.probe_morton()
+    olo, ohi = oracle.bbox_f32(c1.particles["pos"])
+    assert np.array_equal(codes, oracle.morton_f32(c1.particles["pos"], olo, ohi))
+
+
+@pytest.mark.parametrize("name", list(CLOUDS))
+def test_sort_permutation_bit_exact(name):
+    pos, mass = CLOUDS[name]()
+    rec = particles_from(pos, mass)
+    with _sim(rec, 5e3, eos64=False) as s:
+        sc, order = s.probe_sort()
+    olo, ohi = oracle.bbox_f32(rec["pos"])
+    osc, oorder = oracle.sort_pairs(oracle.morton_f32(rec["pos"], olo, ohi))
+    assert np.array_equal(sc, osc)
+    assert np.array_equal(order, oorder)
+
+
+def _verify_tree(left, right, parent, L):
+    """gpu-bh-verify verify_tree (main.rs:420-459): reachability once + parent pointers."""
+    if L == 1:
+        return True
+    nodes = 2 * L - 1
+    seen = np.zeros(nodes, np.int64)
+    stack = [0]
+    assert parent[0] == 0xFFFFFFFF
+    while stack:
+        i = stack.pop()
+        seen[i] += 1
+        if i < L - 1:
+            l, r = int(left[i]), int(right[i])
+            assert parent[l] == i and parent[r] == i
+            stack += [l, r]
+    return bool(np.all(seen == 1))
+
+
+@pytest.mark.parametrize("K", [1, 4, 16, 64])
+@pytest.mark.parametrize("name", ["bh_verify_1500", "duplicates_3000"])
+def test_karras_topology_bit_exact(name, K):
+    pos, mass = CLOUDS[name]()
+    rec = particles_from(pos, mass)
+    with _sim(rec, 5e3, eos64=False) as s:
+        s.set_gravity(isph.GRAVITY_TREE, 0.5, K)
+        t = s.probe_tree(K)
+    olo, ohi = oracle.bbox_f32(rec["pos"])
+    osc, oorder = oracle.sort_pairs(oracle.morton_f32(rec["pos"], olo, ohi))
+    ol, orr, op = oracle.karras(oracle.cluster_codes(osc, K))
+    assert np.array_equal(t["left"], ol)
+    assert np.array_equal(t["right"], orr)
+    assert np.array_equal(t["parent"], op)
+    assert _verify_tree(t["left"], t["right"], t["parent"], t["L"])
+    # root moments (gpu-bh-verify main.rs:527-550): mass and COM to 1e-4, AABB exactly the bbox
+    root = t["com"][0].astype(np.float64)
+    m = rec["mass"].astype(np.float64)
+    p = rec["pos"].astype(np.float64)
+    assert abs(root[3] - m.sum()) / m.sum() < 1e-4
+    com = (p * m[:, None]).sum(0) / m.sum()
+    assert np.linalg.norm(root[:3] - com) / max(np.linalg.norm(com), 1.0) < 1e-4
+    assert np.array_equal(t["bmin"][0, :3], olo) and np.array_equal(t["bmax"][0, :3], ohi)
+
+
+def test_gravity_direct_matches_cpu_sum():
+    """gpu_gravity.rs:716-750: worst relative error < 1e-3 (f32 vs f64), N=1500 xorshift cloud."""
+    pos, mass = xorshift_debris(1500, 0x51ED2707)
+    soft = 2.5e4
+    with isph.GpuSph(2000) as s:
+        gpu = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_DIRECT)
+    cpu = oracle.direct_gravity(pos, mass, soft)
+    assert np.isfinite(gpu).all()
+    assert rms_rel(gpu, cpu)[1] < 1e-3
+
+
+def test_gravity_tree_theta_bounds():
+    """gpu_gravity.rs:852-890 (N=6000: RMS < 1e-2 vs octree and vs exact, bitwise repeatable) and
+    :897-924 (theta -> 0 recovers the direct sum to RMS < 1e-4)."""
+    pos, mass = xorshift_debris(6000, 0x7EE52707)
+    soft = 2.5e4
+    exact = oracle.direct_gravity(pos, mass, soft)
+    octree = oracle.octree_accel(pos, mass, 0.5, soft)
+    with isph.GpuSph(6000) as s:
+        s.set_gravity(isph.GRAVITY_TREE, 0.5, 1)
+        a = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_TREE)
+        b = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_TREE)
+        s.set_gravity(isph.GRAVITY_TREE, 1e-4, 1)
+        opened = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_TREE)
+    assert np.isfinite(a).all()
+    assert rms_rel(a, octree)[0] < 1e-2
+    assert rms_rel(a, exact)[0] < 1e-2
+    assert np.array_equal(a, b)
+    assert rms_rel(opened, exact)[0] < 1e-4
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.25, 0.1])
+def test_gravity_tree_at_least_as_accurate_as_reference_walk(theta):
+    """The group MAC accepts a node only when the reference's per-particle MAC holds for every group member,
+    so the error against the exact sum must not exceed the reference LBVH walk's (oracle restatement of
+    cs_gravity_bh, whose numbers reproduce docs/37:21-22) by more than f32 noise."""
+    pos, mass = splitmix_cloud(1500, 0xABCD1234)
+    soft = 5e3
+    exact = oracle.direct_gravity(pos, mass, soft)
+    ref = oracle.lbvh_gravity_f32(pos, mass, soft, theta)
+    with isph.GpuSph(1500) as s:
+        s.set_gravity(isph.GRAVITY_TREE, theta, 1)
+        a = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_TREE)
+    assert rms_rel(a, exact)[0] <= rms_rel(ref, exact)[0] * 1.05 + 2e-6
+
+
+@pytest.mark.parametrize("K", [1, 8])
+def test_sph_density_force_dudt_parity(c1, K):
+    """rho, SPH acceleration and du/dt vs the f64 CPU path (HydroBody, gravity off) on the 8192-particle
+    two-body scene with random velocities: per-particle relative error < 1e-5."""
+    rec = with_random_velocities(c1)
+    hb = hydro_from(rec, EOS64, c1.softening, gravity="none")
+    hb.compute_density()
+    acc_ref, du_ref = hb.forces()
+    rho_ref = hb.state()[3]
+    with _sim(rec, c1.softening) as s:
+        s.set_gravity(isph.GRAVITY_NONE, 0.5, K)
+        acc, du, rho = s.force_eval()
+    e_rho = rel_err_floor(rho, rho_ref)
+    e_acc = rel_err_floor(acc, acc_ref)
+    e_du = rel_err_floor(du, du_ref)
+    print(f"rho max {e_rho.max():.2e} acc max {e_acc.max():.2e} (rms {np.sqrt(np.mean(e_acc**2)):.2e}) "
+          f"du max {e_du.max():.2e}")
+    assert e_rho.max() < 1e-5
+    assert e_acc.max() < 1e-5
+    assert e_du.max() < 1e-5
+
+
+def test_direct_gravity_in_step_parity(c1):
+    """Full force (direct gravity + SPH) vs the f64 CPU path with direct gravity: < 1e-5 per particle."""
+    rec = with_random_velocities(c1)
+    hb = hydro_from(rec, EOS64, c1.softening, gravity="direct")
+    hb.compute_density()
+    acc_ref, du_ref = hb.forces()
+    with _sim(rec, c1.softening) as s:
+        s.set_gravity(isph.GRAVITY_DIRECT, 0.5, 1)
+        acc, du, rho = s.force_eval()
+    e = rel_err_floor(acc, acc_ref)
+    print(f"acc max {e.max():.2e}")
+    assert e.max() < 1e-5
+
+
+def test_tree_gravity_in_step_theta_bound(c1):
+    """Tree gravity inside the step vs the reference CPU path (octree theta=0.5): the gravity part is an
+    approximation on both sides, so the bound is the reference's theta bound, RMS < 1e-2."""
+    rec = c1.particles
+    hb = hydro_from(rec, EOS64, c1.softening, gravity="octree")
+    hb.compute_density()
+    acc_ref, _ = hb.forces()
+    with _sim(rec, c1.softening) as s:
+        acc, _, _ = s.force_eval()
+    assert rms_rel(acc, acc_ref)[0] < 1e-2
+
+
+def test_courant_dt_matches_cpu(c1):
+    """Device Courant reduction vs HydroBody::courant_dt (hydrostatic.rs:472-480)."""
+    rec = with_random_velocities(c1)
+    hb = hydro_from(rec, EOS64, c1.softening, gravity="none")
+    hb.compute_density()
+    dt_ref = hb.courant_dt(0.3)
+    with _sim(rec, c1.softening) as s:
+        s.step_cfl(0.3, 1)
+        t, dt = s.sim_time()
+    assert abs(dt - dt_ref) / dt_ref < 1e-5
+    assert t == pytest.approx(dt, rel=1e-7)
+
+
+def test_kdk_trajectory_tracks_cpu_step(c1):
+    """10 KDK steps at the CPU path's Courant dt (direct gravity on both sides): positions, velocities and
+    u track the f64 HydroBody::step within RMS 1e-4 relative (sph-verify bound 5e-3, main.rs:603)."""
+    rec = with_random_velocities(c1)
+    hb = hydro_from(rec, EOS64, c1.softening, gravity="direct")
+    hb.compute_density()
+    dt = hb.courant_dt(0.3)
+    for _ in range(10):
+        hb.step(dt)
+    pos_ref, vel_ref, u_ref, _ = hb.state()
+    with _sim(rec, c1.softening) as s:
+        s.set_gravity(isph.GRAVITY_DIRECT, 0.5, 1)
+        s.set_dt(dt, 1.0)
+        s.encode_kdk(10)
+        out = s.readback_sync()
+    p0 = rec["pos"].astype(np.float64)
+    disp_ref = pos_ref - p0
+    disp = out["pos"].astype(np.float64) - p0
+    e_pos = np.sqrt(((disp - disp_ref) ** 2).sum() / (disp_ref ** 2).sum())
+    e_vel = rms_rel(out["vel"], vel_ref)[0]
+    e_u = np.sqrt(((out["u"] - u_ref) ** 2).sum() / (u_ref ** 2).sum())
+    print(f"displacement {e_pos:.2e} vel {e_vel:.2e} u {e_u:.2e}")
+    assert e_pos < 1e-4 and e_vel < 1e-4 and e_u < 1e-4
+
+
+def test_energy_and_angular_momentum_diagnostics(c1):
+    rec = with_random_velocities(c1)
+    with _sim(rec, c1.softening) as s:
+        d = s.diagnostics(with_pe=True)
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    ke, ie, pe = oracle.total_energy(pos, vel, mass, u, c1.softening)
+    L = oracle.angular_momentum(pos, vel, mass)
+    assert d["ke"] == pytest.approx(ke, rel=1e-12)
+    assert d["ie"] == pytest.approx(ie, rel=1e-12)
+    assert d["pe"] == pytest.approx(pe, rel=1e-10)
+    assert np.allclose(d["L"], L, rtol=1e-9, atol=1e-9 * np.abs(L).max())
+
+
+def test_readback_semantics_and_clamps(c1):
+    """gpu_store.rs:27-35, 126-170 and gpu_sph.rs:1323-1340: capacity clamp, two-phase read-back."""
+    rec = c1.particles
+    with isph.GpuSph(5000) as s:
+        assert s.take_readback() is None               # idle -> None
+        assert s.upload(rec, EOS, c1.softening) == 5000  # clamped to capacity
+        assert s.count() == 5000
+        s.begin_readback()
+        snap = None
+        for _ in range(100000):
+            snap = s.take_readback()
+            if snap is not None:
+                break
+        assert snap is not None and len(snap) == 5000
+        assert np.array_equal(snap["pos"], rec["pos"][:5000])
+        assert np.array_equal(snap["prov"], rec["prov"][:5000])
+        assert s.take_readback() is None               # consumed
+        bad = rec[:10].copy()
+        bad["mat"][3] = 7
+        with pytest.raises(isph.IsphError):
+            s.upload(bad, EOS, c1.softening)
+
+
+def test_retired_particles_are_inert(c1):
+    """Mass-0 particles are not integrated and contribute nothing (sph_step.wgsl:185-189, 309-313)."""
+    rec = c1.particles.copy()
+    rec["mass"][:50] = 0.0
+    with _sim(rec, c1.softening) as s:
+        s.step_cfl(0.3, 2)
+        out = s.readback_sync()
+    assert np.array_equal(out["pos"][:50], rec["pos"][:50])
+    assert np.isfinite(out["pos"]).all() and np.isfinite(out["vel"]).all()
