@@ -1,0 +1,170 @@
+"""Particle-step throughput of the MI355X LBVH + SPH step on the 1,048,576-particle two-body impact.
+
+Workload (one GPU): the seeded two-body giant impact (917,504-particle target + 131,072-particle impactor,
+30 % iron / 70 % granite, h for 48 neighbours, Barnes-Hut theta = 0.5, Plummer softening 0.01, AV 1/2),
+plain FP32, adaptive Courant dt at CFL 0.3 computed on the device. A step is one full KDK step: tree build,
+gravity, density + EOS and SPH force twice (eval -> half-kick + drift -> eval -> half-kick), as the
+reference's CPU step and GPU driver do. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: throughput (particle-steps/s), per-phase ms, roofline of the dominant kernel
+(gravity traversal, flops from counted interactions, duration from HIP events on the handle's stream),
+energy / angular-momentum drift over the timed run (exact f64 PE on the device), and the CPU baseline
+(the oracle's single-threaded restatement of HydroBody::step, one force evaluation of the same scene).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32")
+    ap.add_argument("--reuse", action="store_true", help="reuse tree/gravity/density between evals at equal positions")
+    ap.add_argument("--quadrupole", action="store_true")
+    ap.add_argument("--leaf", type=int, default=1)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--cfl", type=float, default=0.3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-energy", action="store_true", help="skip the O(N^2) energy diagnostics")
+    ap.add_argument("--compare-mixed", action="store_true", help="also run the mixed-precision form")
+    return ap.parse_args()
+
+
+def lscpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc):
+    """One force evaluation (density + forces_and_dudt = half a KDK step) of the oracle's restatement of
+    HydroBody (hydrostatic.rs:338-446, octree gravity theta 0.5) on the same scene, 1 thread."""
+    from botheadstudios_amd import scene
+    from oracle import oracle
+    pos, vel, mass, u, h, mat = scene.f64_state(sc.particles)
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, sc.eos64, sc.softening)
+    t0 = time.perf_counter()
+    hb.compute_density()
+    hb.forces()
+    dt = time.perf_counter() - t0
+    n = len(mass)
+    return {"value": n / (2.0 * dt), "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one force evaluation (density + forces_and_dudt, half a KDK step) of the N={n} scene, "
+                      f"{dt:.1f} s on 1 thread of {lscpu_model()}"}
+
+
+def run_gpu(sc, args, precision, steps, warmup):
+    from botheadstudios_amd import isph
+    rec = sc.particles
+    n = len(rec)
+    s = isph.GpuSph(n)
+    s.upload(rec, sc.eos, sc.softening)
+    s.set_eos_f64(sc.eos64)
+    s.set_gravity(isph.GRAVITY_TREE, args.theta, args.leaf, args.quadrupole)
+    s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
+    s.set_reuse(args.reuse)
+    s.step_cfl(args.cfl, warmup)
+    s.synchronize()
+    d0 = None if args.no_energy else s.diagnostics(with_pe=True)
+    t_sim0, _ = s.sim_time()
+    s.reset_stats()
+    s.set_timing(True)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step_cfl(args.cfl, steps)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    phases = s.phase_times()
+    stats = s.kernel_stats()
+    s.set_timing(False)
+    t_sim1, dt_last = s.sim_time()
+    d1 = None if args.no_energy else s.diagnostics(with_pe=True)
+    s.close()
+    out = {"elapsed": elapsed, "phases": phases, "stats": stats, "sim_time": t_sim1 - t_sim0, "dt_last": dt_last}
+    if d0 is not None:
+        e0 = d0["ke"] + d0["ie"] + d0["pe"]
+        e1 = d1["ke"] + d1["ie"] + d1["pe"]
+        l0 = np.linalg.norm(d0["L"])
+        out["dE_rel"] = abs(e1 - e0) / abs(e0)
+        out["dL_rel"] = float(np.linalg.norm(d1["L"] - d0["L"]) / l0)
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the particle step here is a single-GPU build (got --gpus {args.gpus}). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    from botheadstudios_amd import scene
+
+    sc = scene.two_body_1m() if args.n == 1048576 else scene.two_body(args.n)
+    n = len(sc.particles)
+    r = run_gpu(sc, args, args.precision, args.steps, args.warmup)
+    st = r["stats"]
+    launches = max(st["gravity_launches"], 1.0)
+    flops = st["gravity_flops"]
+    grav_s = st["gravity_ms"] / 1e3
+    achieved = flops / grav_s / 1e12 if grav_s > 0 else 0.0
+    value = n * args.steps / r["elapsed"]
+    line = {
+        "metric": "particle-steps/s @N=1M (plain FP32 vs mixed, dE<1e-4); adjoint/forward cost",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["elapsed"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "fp32" else "f32-pairs/f64-state",
+        "data": "synthetic",
+        "config": {"workload": f"two-body giant impact, N={n} ({sc.n_target} target + {sc.n_impactor} impactor), "
+                               f"LBVH Barnes-Hut theta={args.theta}, leaf={args.leaf}, "
+                               f"{'quadrupole' if args.quadrupole else 'monopole'}, SPH 48 neighbours, "
+                               f"Tillotson granite/iron, KDK at CFL {args.cfl}",
+                   "n": n, "precision": args.precision, "reuse": bool(args.reuse)},
+        "roofline": {"bound": "mfma", "engine": "valu-fp32", "kernel": "k_gravity_tree",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops / launches,
+                     "avg_launch_ms": st["gravity_ms"] / launches},
+        "phases_ms_per_step": {k: v / args.steps for k, v in r["phases"].items()},
+        "interactions_per_particle_eval": (st["cell_interactions"] + st["particle_interactions"] +
+                                           st["quad_interactions"]) / (launches * n),
+        "sim_time_s": r["sim_time"],
+    }
+    if "dE_rel" in r:
+        line["dE_rel"] = r["dE_rel"]
+        line["dL_rel"] = r["dL_rel"]
+    if args.compare_mixed and args.precision == "fp32":
+        rm = run_gpu(sc, args, "mixed", args.steps, args.warmup)
+        line["mixed"] = {"value": n * args.steps / rm["elapsed"], "ms_per_step": rm["elapsed"] * 1e3 / args.steps,
+                         "dE_rel": rm.get("dE_rel"), "dL_rel": rm.get("dL_rel")}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(sc)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

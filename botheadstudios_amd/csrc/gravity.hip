@@ -7,13 +7,14 @@
 // node COM to the box, softened) and the node box does not overlap the group box, so each particle sees a
 // node set at least as refined as the reference's per-particle walk. Accepted cells and leaf particles are
 // appended to LDS lists with ballot/mbcnt compaction and evaluated 64 at a time with LDS broadcast reads.
-// The traversal stack lives in LDS; up to 64 nodes are popped per round (one per lane) and the pop width
-// narrows to depth-first near capacity, which bounds the stack by capacity - margin + tree depth.
+// The traversal stack lives in LDS; up to 64 nodes are popped per round (one per lane), which leaves at
+// most ~64 pending nodes per tree level (2048 entries cover the tree depth with room to spare); the pop
+// width narrows to depth-first near capacity, which bounds the stack by capacity - margin + tree depth.
 #include "state.h"
 
 namespace isph {
 
-constexpr int kStackCap = 1024;
+constexpr int kStackCap = 2048;
 constexpr int kStackMargin = 128;  // > maximum Karras depth (30 code bits + 32 index tie-break bits)
 constexpr int kBuf = 128;
 
@@ -142,10 +143,12 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
         const bool has = lane < k;
         const uint32_t node = has ? stk[sp + lane] : 0u;
         __syncthreads();
-        const bool leaf = has && node >= leaf0;
+        // Leaves of one particle always go to the particle list (bh_gravity.wgsl:356-369); bucketed leaves
+        // (K > 1) are tested like internal nodes first and expanded only when the MAC fails.
+        const bool is_leaf = has && node >= leaf0;
         bool accept = false, open = false;
         float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (has && !leaf) {
+        if (has && (!is_leaf || K > 1)) {
             A = nA[node];
             const float4 B = nB[node];
             const float4 C = nC[node];
@@ -155,8 +158,9 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
             float d2 = dx * dx + dy * dy + dz * dz + soft2;
             bool disjoint = (B.x > gx1) | (C.x < gx0) | (B.y > gy1) | (C.y < gy0) | (B.z > gz1) | (C.z < gz0);
             accept = disjoint && d2 >= C.w * C.w;
-            open = !accept;
+            open = !accept && !is_leaf;
         }
+        const bool leaf = is_leaf && !accept;
         const uint64_t mo = __ballot(open);
         if (open) {
             int pos = sp + 2 * prefix_count(mo);
@@ -226,28 +230,45 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
 }
 
 // Exact softened direct sum in f32 (bh_gravity.wgsl:313-328), tiled through LDS.
+template <bool COMP>
 __global__ __launch_bounds__(256) void k_gravity_direct(const float4* __restrict__ spos, uint32_t n, float soft2,
                                                         float4* __restrict__ sgrav) {
     __shared__ float4 tile[256];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     float4 me = spos[i < n ? i : n - 1];
     float ax = 0.f, ay = 0.f, az = 0.f;
+    KahanF kx{0.f, 0.f}, ky{0.f, 0.f}, kz{0.f, 0.f};
     for (uint32_t base = 0; base < n; base += 256) {
         uint32_t j = base + threadIdx.x;
         tile[threadIdx.x] = j < n ? spos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
         uint32_t cnt = min(256u, n - base);
+        float bx = 0.f, by = 0.f, bz = 0.f;
         for (uint32_t jj = 0; jj < cnt; ++jj) {
             float4 c = tile[jj];
             float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
             float r2 = dx * dx + dy * dy + dz * dz + soft2;
             float inv = __builtin_amdgcn_rsqf(r2);
             float w = (base + jj == i) ? 0.f : c.w * inv * inv * inv;
-            ax += dx * w;
-            ay += dy * w;
-            az += dz * w;
+            bx += dx * w;
+            by += dy * w;
+            bz += dz * w;
+        }
+        if (COMP) {
+            kx.add(bx);
+            ky.add(by);
+            kz.add(bz);
+        } else {
+            ax += bx;
+            ay += by;
+            az += bz;
         }
         __syncthreads();
+    }
+    if (COMP) {
+        ax = kx.s;
+        ay = ky.s;
+        az = kz.s;
     }
     if (i < n) sgrav[i] = make_float4(kG * ax, kG * ay, kG * az, 0.f);
 }
@@ -278,7 +299,12 @@ void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s) {
     }
 }
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_gravity_direct, dim3((d.n + 255) / 256), dim3(256), 0, s, d.spos, d.n, p.soft2, d.sgrav);
+    if (p.mixed)
+        hipLaunchKernelGGL(k_gravity_direct<true>, dim3((d.n + 255) / 256), dim3(256), 0, s, d.spos, d.n, p.soft2,
+                           d.sgrav);
+    else
+        hipLaunchKernelGGL(k_gravity_direct<false>, dim3((d.n + 255) / 256), dim3(256), 0, s, d.spos, d.n, p.soft2,
+                           d.sgrav);
 }
 void launch_gravity_zero(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_fill4, dim3((d.n + 255) / 256), dim3(256), 0, s, d.sgrav, d.n, make_float4(0.f, 0.f, 0.f, 0.f));

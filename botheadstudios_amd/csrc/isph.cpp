@@ -83,7 +83,7 @@ void free_all(isph_ctx* c) {
     Dev& d = c->d;
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
-                    d.lcode, d.left, d.right, d.parent, d.ready, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.srho64, d.spr64, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -124,6 +124,8 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.right, cap);
     dalloc(&d.parent, nodes);
     dalloc(&d.ready, cap);
+    dalloc(&d.nrange, cap);
+    dalloc(&d.cross, cap);
     dalloc(&d.nA, nodes);
     dalloc(&d.nB, nodes);
     dalloc(&d.nC, nodes);
@@ -148,6 +150,8 @@ void ensure64(isph_ctx* c) {
     if (!c->d.pos64) {
         dalloc(&c->d.pos64, 3 * (size_t)c->d.cap);
         dalloc(&c->d.vel64, 3 * (size_t)c->d.cap);
+        dalloc(&c->d.srho64, c->d.cap);
+        dalloc(&c->d.spr64, c->d.cap);
     }
 }
 
@@ -168,7 +172,7 @@ void flush_timing(isph_ctx* c) {
 
 hipEvent_t next_event(isph_ctx* c) {
     if (c->ev_used == c->ev_pool.size()) {
-        if (c->ev_pool.size() < 8192) {
+        if (c->ev_pool.size() < 16384) {
             hipEvent_t e;
             HIP_OK(hipEventCreate(&e));
             c->ev_pool.push_back(e);
@@ -466,6 +470,8 @@ static Dev integ_view(isph_ctx* c) {
     if (c->precision != ISPH_PRECISION_MIXED) {
         d.pos64 = nullptr;
         d.vel64 = nullptr;
+        d.srho64 = nullptr;
+        d.spr64 = nullptr;
     }
     return d;
 }
@@ -684,6 +690,7 @@ int isph_phase_times(isph_t* h, double* ms, size_t n) {
 }
 
 int isph_kernel_stats(isph_t* h, double* out7) {
+    // out7 holds 8 entries (see isph.h)
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));
         flush_timing(h);
@@ -696,6 +703,7 @@ int isph_kernel_stats(isph_t* h, double* out7) {
         out7[4] = (double)cnt[2];
         out7[5] = (double)cnt[3];
         out7[6] = (double)cnt[4];
+        out7[7] = ((double)cnt[0] + (double)cnt[1]) * kFlopsMonopole + (double)cnt[2] * kFlopsQuadrupole;
         return ISPH_OK;
     });
 }

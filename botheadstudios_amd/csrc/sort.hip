@@ -16,7 +16,7 @@ __global__ void k_reset_scalars(uint32_t* scal, unsigned long long* counters, in
     if (i < 3) scal[i] = 0xffffffffu;              // min slots start at +inf key (bh_gravity.wgsl:64-65)
     else if (i < 6) scal[i] = 0u;                  // max slots at -inf key
     else if (i == 6) scal[i] = 0x7f800000u;        // Courant signal min: +inf bits
-    else if (i == 7) scal[i] = 0u;
+    else if (i == 7) scal[i] = 0u;                 // chunk-crossing internal nodes of the next tree build
 }
 
 // Block-reduced float-radix min/max (bh_gravity.wgsl:67-78 does one atomic per particle per axis).

@@ -13,7 +13,7 @@
 
 namespace isph {
 
-constexpr int kStackCap = 1024;
+constexpr int kStackCap = 2048;
 constexpr int kStackMargin = 128;
 constexpr int kBuf = 128;
 
@@ -102,19 +102,23 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
 }
 
 // Per-particle EOS + Courant signal epilogue shared by the density and the EOS-only kernels.
-__device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, float rho, float h, float4 v, uint32_t m,
+__device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, double rho, float h, float4 v, uint32_t m,
                                              const Till* __restrict__ eos, float* srho, float* spr, float* scs,
-                                             uint32_t* scal) {
+                                             double* srho64, double* spr64, uint32_t* scal) {
     float sig = INFINITY;
     if (valid) {
         const Till& T = eos[m];
         double p, c2;
-        till_p_c2(T, (double)rho, (double)v.w, p, c2);
-        double r = (double)rho;
+        till_p_c2(T, rho, (double)v.w, p, c2);
+        double r = rho;
         double c = sqrt(c2);
-        srho[t] = rho;
+        srho[t] = (float)rho;
         spr[t] = (float)(p / (r * r));
         scs[t] = (float)c;
+        if (srho64) {
+            srho64[t] = rho;
+            spr64[t] = p / (r * r);
+        }
         float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
         sig = h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
     }
@@ -122,7 +126,7 @@ __device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, float rho, 
     if (threadIdx.x == 0 && sig < INFINITY) atomicMin(&scal[6], __float_as_uint(sig));
 }
 
-template <bool COMP>
+template <bool F64>
 __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                 const float4* __restrict__ svel, const uint32_t* __restrict__ smat,
                                                 uint32_t n, uint32_t K, uint32_t L, const uint32_t* __restrict__ left,
@@ -130,7 +134,8 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                                                 const float4* __restrict__ nC, const uint32_t* __restrict__ order,
                                                 const Till* __restrict__ eos, float* __restrict__ srho,
                                                 float* __restrict__ spr, float* __restrict__ scs,
-                                                float* __restrict__ rho_orig, uint32_t* scal,
+                                                float* __restrict__ rho_orig, double* __restrict__ srho64,
+                                                double* __restrict__ spr64, uint32_t* scal,
                                                 unsigned long long* counters) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ uint32_t cand[kBuf];
@@ -146,8 +151,12 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
     const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
     const float hg = wave_max(hi);
+    // Plain: f32 pair arithmetic and sum (the reference shader's arithmetic). Mixed (F64): the density feeds
+    // the Tillotson pressure, whose cold branch P = A (rho/rho0 - 1) + ... cancels near rho0 and amplifies a
+    // relative density error by A / P (~1e3), so the true-pair kernel sums run in f64 (the coordinate
+    // differences of f32 positions are exact; only ~50 pairs per particle pass the radius test).
     float rho = 0.f;
-    KahanF kr{0.f, 0.f};
+    double rho64 = 0.0;
     unsigned long long tested = 0;
     neighbour_walk(stk, cand, n, K, L, left, right, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, [&](int cnt) {
         if (lane < cnt) {
@@ -163,26 +172,38 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
             float r2 = dx * dx + dy * dy + dz * dz;
             float hij = 0.5f * (hi + cH[jj]);
             if (r2 < hij * hij && cand[jj] != t) {  // sph_step.wgsl:201,210
-                float r = sqrtf(r2);
-                float inv_h = 1.0f / hij;
-                acc += pj.w * spline_w(r * inv_h, inv_h);
+                if (F64) {
+                    double ddx = dx, ddy = dy, ddz = dz;
+                    double h2 = 0.5 * ((double)hi + (double)cH[jj]);
+                    double q = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / h2;
+                    double sig = 8.0 / (3.14159265358979323846 * h2 * h2 * h2);
+                    double w = q < 0.5 ? sig * (6.0 * (q * q * q - q * q) + 1.0)
+                                       : (q < 1.0 ? sig * 2.0 * (1.0 - q) * (1.0 - q) * (1.0 - q) : 0.0);
+                    rho64 += (double)pj.w * w;
+                } else {
+                    float r = sqrtf(r2);
+                    float inv_h = 1.0f / hij;
+                    acc += pj.w * spline_w(r * inv_h, inv_h);
+                }
             }
         }
-        if (COMP) kr.add(acc);
-        else rho += acc;
+        rho += acc;
         tested += cnt;
     });
-    if (COMP) rho = kr.s;
     const bool live = valid && me.w > 0.f;  // retired (mass 0) particles keep their density (:185-189)
-    float rho_i;
+    double rho_i;
     if (live) {
-        float inv_h = 1.0f / hi;
-        rho_i = me.w * spline_w(0.f, inv_h) + rho;
-        rho_orig[order[t]] = rho_i;
+        if (F64) {
+            rho_i = (double)me.w * (8.0 / (3.14159265358979323846 * (double)hi * (double)hi * (double)hi)) + rho64;
+        } else {
+            float inv_h = 1.0f / hi;
+            rho_i = (double)(me.w * spline_w(0.f, inv_h) + rho);
+        }
+        rho_orig[order[t]] = (float)rho_i;
     } else {
-        rho_i = valid ? rho_orig[order[t]] : 1.f;
+        rho_i = valid ? (double)rho_orig[order[t]] : 1.0;
     }
-    eos_epilogue(valid, t, rho_i, hi, svel[tc], smat[tc], eos, srho, spr, scs, scal);
+    eos_epilogue(valid, t, rho_i, hi, svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scal);
     const uint64_t mv = __ballot(valid);
     if (lane == 0 && counters) atomicAdd(&counters[3], tested * (unsigned long long)__popcll(mv));
 }
@@ -191,14 +212,15 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
 __global__ __launch_bounds__(64) void k_eos(const float* __restrict__ sh, const float4* __restrict__ svel,
                                             const uint32_t* __restrict__ smat, uint32_t n, const Till* __restrict__ eos,
                                             float* __restrict__ srho, float* __restrict__ spr, float* __restrict__ scs,
-                                            uint32_t* scal) {
+                                            double* __restrict__ srho64, double* __restrict__ spr64, uint32_t* scal) {
     const uint32_t t = blockIdx.x * kWave + threadIdx.x;
     const bool valid = t < n;
     const uint32_t tc = valid ? t : n - 1;
-    eos_epilogue(valid, t, srho[tc], sh[tc], svel[tc], smat[tc], eos, srho, spr, scs, scal);
+    const double rho = srho64 ? srho64[tc] : (double)srho[tc];
+    eos_epilogue(valid, t, rho, sh[tc], svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scal);
 }
 
-template <bool COMP>
+template <bool F64>
 __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
                                               const float4* __restrict__ svel, const float* __restrict__ srho,
                                               const float* __restrict__ spr, const float* __restrict__ scs, uint32_t n,
@@ -206,12 +228,15 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
                                               const uint32_t* __restrict__ right, const float4* __restrict__ nB,
                                               const float4* __restrict__ nC, const uint32_t* __restrict__ order,
                                               const float4* __restrict__ sgrav, float4* __restrict__ acc_orig,
+                                              const double* __restrict__ srho64, const double* __restrict__ spr64,
                                               float av_alpha, float av_beta, unsigned long long* counters) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ uint32_t cand[kBuf];
     __shared__ float4 cP[kWave];  // x, y, z, mass
     __shared__ float4 cV[kWave];  // vx, vy, vz, rho
     __shared__ float4 cS[kWave];  // h, P/rho^2, c
+    __shared__ double cR64[F64 ? kWave : 1];
+    __shared__ double cPr64[F64 ? kWave : 1];
     const int lane = threadIdx.x;
     const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x);
     const uint32_t t = g * kWave + lane;
@@ -224,8 +249,13 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
     const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
     const float hg = wave_max(hi);
+    // Plain: f32 pair arithmetic and sums (the reference shader's). Mixed (F64): the true pairs (about 50
+    // per particle) are evaluated and summed in f64 from exact f32 coordinate differences, with the f64
+    // density and P / rho^2 of the EOS pass; du/dt cancels across pairs, so f32 pair terms cap its
+    // relative accuracy near 1e-4.
     float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f;
-    KahanF kx{0.f, 0.f}, ky{0.f, 0.f}, kz{0.f, 0.f}, ke{0.f, 0.f};
+    double ax64 = 0.0, ay64 = 0.0, az64 = 0.0, de64 = 0.0;
+    const double rhoi64 = F64 ? srho64[tc] : 0.0, pri64 = F64 ? spr64[tc] : 0.0;
     unsigned long long tested = 0;
     neighbour_walk(stk, cand, n, K, L, left, right, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, [&](int cnt) {
         if (lane < cnt) {
@@ -234,9 +264,46 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
             float4 v = svel[j];
             cV[lane] = make_float4(v.x, v.y, v.z, srho[j]);
             cS[lane] = make_float4(sh[j], spr[j], scs[j], 0.f);
+            if (F64) {
+                cR64[lane] = srho64[j];
+                cPr64[lane] = spr64[j];
+            }
         }
         __syncthreads();
         for (int jj = 0; jj < cnt; ++jj) {
+            if (F64) {
+                float4 pj = cP[jj];
+                float4 sj = cS[jj];
+                double dx = (double)(me.x - pj.x), dy = (double)(me.y - pj.y), dz = (double)(me.z - pj.z);
+                double r2 = dx * dx + dy * dy + dz * dz;
+                double hij = 0.5 * ((double)hi + (double)sj.x);
+                if (r2 < hij * hij && r2 > 1.0e-18 && cand[jj] != t) {
+                    float4 vj = cV[jj];
+                    double r = sqrt(r2);
+                    double dvx = (double)vi.x - (double)vj.x, dvy = (double)vi.y - (double)vj.y,
+                           dvz = (double)vi.z - (double)vj.z;
+                    double vr = dvx * dx + dvy * dy + dvz * dz;
+                    double pij = 0.0;
+                    if (vr < 0.0) {
+                        double mu = hij * vr / (r2 + 0.01 * hij * hij);
+                        double cbar = 0.5 * ((double)ci + (double)sj.z);
+                        double rbar = 0.5 * (rhoi64 + cR64[jj]);
+                        pij = (-(double)av_alpha * cbar * mu + (double)av_beta * mu * mu) / rbar;
+                    }
+                    double coeff = pri64 + cPr64[jj] + pij;
+                    double q = r / hij;
+                    double h2 = hij * hij;
+                    double sig = 8.0 / (3.14159265358979323846 * (h2 * h2));
+                    double dw = q < 0.5 ? sig * (18.0 * q * q - 12.0 * q) : (q < 1.0 ? sig * -6.0 * (1.0 - q) * (1.0 - q) : 0.0);
+                    double gx = dx / r * dw, gy = dy / r * dw, gz = dz / r * dw;
+                    double mj = (double)pj.w;
+                    ax64 += gx * (-coeff * mj);
+                    ay64 += gy * (-coeff * mj);
+                    az64 += gz * (-coeff * mj);
+                    de64 += 0.5 * mj * coeff * (dvx * gx + dvy * gy + dvz * gz);
+                }
+                continue;
+            }
             float4 pj = cP[jj];
             float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;  // dpos = pos_i - pos_j
             float r2 = dx * dx + dy * dy + dz * dz;
@@ -259,27 +326,19 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
                 float gs = spline_dw(r * inv_h, inv_h) / r;
                 float gx = dx * gs, gy = dy * gs, gz = dz * gs;
                 float f = -coeff * pj.w;
-                float e = 0.5f * pj.w * coeff * (dvx * gx + dvy * gy + dvz * gz);
-                if (COMP) {
-                    kx.add(gx * f);
-                    ky.add(gy * f);
-                    kz.add(gz * f);
-                    ke.add(e);
-                } else {
-                    ax += gx * f;
-                    ay += gy * f;
-                    az += gz * f;
-                    de += e;
-                }
+                ax += gx * f;
+                ay += gy * f;
+                az += gz * f;
+                de += 0.5f * pj.w * coeff * (dvx * gx + dvy * gy + dvz * gz);
             }
         }
         tested += cnt;
     });
-    if (COMP) {
-        ax = kx.s;
-        ay = ky.s;
-        az = kz.s;
-        de = ke.s;
+    if (F64) {
+        ax = (float)ax64;
+        ay = (float)ay64;
+        az = (float)az64;
+        de = (float)de64;
     }
     if (valid) {
         float4 gr = sgrav[t];
@@ -519,27 +578,27 @@ void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = cdiv(d.n, kWave);
     if (p.mixed)
         hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
-                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.scal,
-                           d.counters);
+                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.srho64,
+                           d.spr64, d.scal, d.counters);
     else
         hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
-                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.scal,
-                           d.counters);
+                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho,
+                           (double*)nullptr, (double*)nullptr, d.scal, d.counters);
 }
 void launch_eos(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_eos, dim3(cdiv(d.n, kWave)), dim3(kWave), 0, s, d.sh, d.svel, d.smat, d.n, d.eos, d.srho,
-                       d.spr, d.scs, d.scal);
+                       d.spr, d.scs, d.srho64, d.spr64, d.scal);
 }
 void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = cdiv(d.n, kWave);
     if (p.mixed)
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
-                           d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc, p.av_alpha, p.av_beta,
-                           d.counters);
+                           d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
+                           p.av_alpha, p.av_beta, d.counters);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
-                           d.scs, d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc, p.av_alpha,
-                           p.av_beta, d.counters);
+                           d.scs, d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc,
+                           (const double*)nullptr, (const double*)nullptr, p.av_alpha, p.av_beta, d.counters);
 }
 void launch_kick_drift(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,

@@ -42,6 +42,8 @@ struct Dev {
     float* srho = nullptr;
     float* spr = nullptr;    // P / rho^2
     float* scs = nullptr;    // sound speed
+    double* srho64 = nullptr;  // mixed precision: f64 density and P / rho^2 (sorted)
+    double* spr64 = nullptr;
     float4* sgrav = nullptr; // gravitational acceleration, sorted
     uint32_t* lcode = nullptr;  // leaf cluster codes [L]
 
@@ -50,6 +52,8 @@ struct Dev {
     uint32_t* right = nullptr;
     uint32_t* parent = nullptr;
     uint32_t* ready = nullptr;
+    uint2* nrange = nullptr;    // [first, last] leaf range of each internal node
+    uint32_t* cross = nullptr;  // internal nodes whose range crosses an upsweep chunk boundary
     float4* nA = nullptr;  // com.xyz, mass
     float4* nB = nullptr;  // bmin.xyz, max h
     float4* nC = nullptr;  // bmax.xyz, MAC radius size/theta + |centre - com|
@@ -57,7 +61,7 @@ struct Dev {
     float2* nQ1 = nullptr; // quadrupole xz, yz
 
     // scalars
-    uint32_t* scal = nullptr;  // [0..5] bbox keys, [6] Courant signal min (float bits), [7] spare
+    uint32_t* scal = nullptr;  // [0..5] bbox keys, [6] Courant signal min (float bits), [7] crossing-node count
     float* dt = nullptr;       // current dt
     double* time = nullptr;    // accumulated simulated time
     unsigned long long* counters = nullptr;  // [0] cells, [1] particles, [2] quad cells, [3] dens pairs, [4] force pairs

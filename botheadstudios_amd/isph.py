@@ -196,10 +196,10 @@ class GpuSph:
         return dict(zip(PHASES, ms.tolist()))
 
     def kernel_stats(self):
-        out = np.zeros(7, np.float64)
+        out = np.zeros(8, np.float64)
         self._check(self._lib.isph_kernel_stats(self._h, _ptr(out)))
         keys = ("gravity_launches", "gravity_ms", "cell_interactions", "particle_interactions",
-                "quad_interactions", "density_pairs_tested", "force_pairs_tested")
+                "quad_interactions", "density_pairs_tested", "force_pairs_tested", "gravity_flops")
         return dict(zip(keys, out.tolist()))
 
     def reset_stats(self):

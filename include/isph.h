@@ -149,8 +149,9 @@ int isph_set_timing(isph_t* h, int enable);
 int isph_phase_times(isph_t* h, double* ms, size_t n);
 /* Gravity-kernel statistics since the last reset: out[0] = launches, [1] = total kernel ms (HIP events on
  * the handle's stream), [2] = monopole cell interactions, [3] = particle-particle interactions,
- * [4] = quadrupole cell interactions, [5] = density pairs tested, [6] = force pairs tested. */
-int isph_kernel_stats(isph_t* h, double* out7);
+ * [4] = quadrupole cell interactions, [5] = density pairs tested, [6] = force pairs tested,
+ * [7] = algorithmic gravity flops (interactions times the per-interaction flop constants). */
+int isph_kernel_stats(isph_t* h, double* out8);
 int isph_reset_stats(isph_t* h);
 
 /* ---- stage probes (parity of the bit-exact stages against the oracle) ------------------------------- */

@@ -541,7 +541,12 @@ void orc_cluster_codes(const uint32_t* sorted_codes, uint64_t n, uint32_t K, uin
 }
 
 // Karras build: bh_gravity.wgsl:122-184. left/right sized L-1 (internal nodes), parent sized 2L-1.
-void orc_karras(const uint32_t* codes, uint32_t L, uint32_t* left, uint32_t* right, uint32_t* parent) {
+// reference_split != 0 keeps the shader's midpoint split for all-equal-code ranges (:159-160), which is
+// inconsistent with its index-tie-broken determineRange and yields an invalid tree (nodes with two parents)
+// on runs of >= 3 equal codes; 0 applies Karras 2012's split on the index-extended key (the binary search of
+// :162-172 with the extended delta), identical to the shader whenever first and last codes differ.
+void orc_karras_variant(const uint32_t* codes, uint32_t L, uint32_t* left, uint32_t* right, uint32_t* parent,
+                        int reference_split) {
     for (uint32_t i = 0; i < 2 * L - 1; ++i) parent[i] = NO_PARENT;  // cs_tree_reset
     Karras kt{codes, (int32_t)L};
     for (int32_t i = 0; i < (int32_t)L - 1; ++i) {
@@ -559,8 +564,17 @@ void orc_karras(const uint32_t* codes, uint32_t L, uint32_t* left, uint32_t* rig
         int32_t first = std::min(i, j), last = std::max(i, j);
         int32_t split = first;
         uint32_t fc = codes[first], lc = codes[last];
-        if (fc == lc) {
+        if (fc == lc && reference_split) {
             split = (first + last) >> 1;
+        } else if (fc == lc) {
+            int cpfx = kt.delta(first, last);
+            int32_t step = last - first;
+            for (;;) {
+                step = (step + 1) >> 1;
+                int32_t ns = split + step;
+                if (ns < last && kt.delta(first, ns) > cpfx) split = ns;
+                if (step <= 1) break;
+            }
         } else {
             int cpfx = clz32(fc ^ lc);
             int32_t step = last - first;
@@ -582,6 +596,10 @@ void orc_karras(const uint32_t* codes, uint32_t L, uint32_t* left, uint32_t* rig
         parent[li] = (uint32_t)i;
         parent[ri] = (uint32_t)i;
     }
+}
+
+void orc_karras(const uint32_t* codes, uint32_t L, uint32_t* left, uint32_t* right, uint32_t* parent) {
+    orc_karras_variant(codes, L, left, right, parent, 0);
 }
 
 // Bottom-up moments (f32): bh_gravity.wgsl:260-298 (the sweep's leaf/internal arithmetic; the converged

