@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--quadrupole", action="store_true")
     ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--group-extent", type=float, default=4.0)
+    ap.add_argument("--chunk-extent", type=float, default=0.0)
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-energy", action="store_true", help="skip the O(N^2) energy diagnostics")
@@ -81,6 +83,7 @@ def run_gpu(sc, args, precision, steps, warmup):
     s.set_eos_f64(sc.eos64)
     s.set_gravity(isph.GRAVITY_TREE, args.theta, args.leaf, args.quadrupole)
     s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
+    s.set_walk(args.group_extent, args.chunk_extent)
     s.set_reuse(args.reuse)
     s.step_cfl(args.cfl, warmup)
     s.synchronize()
@@ -95,6 +98,7 @@ def run_gpu(sc, args, precision, steps, warmup):
     elapsed = time.perf_counter() - t0
     phases = s.phase_times()
     stats = s.kernel_stats()
+    stats.update(s.walk_stats())
     s.set_timing(False)
     t_sim1, dt_last = s.sim_time()
     d1 = None if args.no_energy else s.diagnostics(with_pe=True)
@@ -153,6 +157,7 @@ def main():
         "interactions_per_particle_eval": (st["cell_interactions"] + st["particle_interactions"] +
                                            st["quad_interactions"]) / (launches * n),
         "sim_time_s": r["sim_time"],
+        "walk": {k: v for k, v in st.items()},
     }
     if "dE_rel" in r:
         line["dE_rel"] = r["dE_rel"]

@@ -14,37 +14,39 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libisph.so")
 SOURCES = ["sort.hip", "tree.hip", "gravity.hip", "sph.hip", "isph.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Werror=return-type", "-Werror=uninitialized",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def _compile(src):
-    obj = os.path.join(CSRC, "_build", src + ".o")
+def _compile(src, defines=(), tag=""):
+    obj = os.path.join(CSRC, "_build" + tag, src + ".o")
     os.makedirs(os.path.dirname(obj), exist_ok=True)
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + [f"-D{d}" for d in defines] + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
     return obj, r.stderr
 
 
-def build(verbose=False):
+def build(verbose=False, out=OUT, defines=()):
+    """Compile and link; `defines` are extra -D macros (tuning variants), `out` the library path."""
+    tag = "_" + "_".join(d.replace("=", "") for d in defines) if defines else ""
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda s: _compile(s, defines, tag), SOURCES))
     objs = [o for o, _ in results]
     if verbose:
         for (o, err), src in zip(results, SOURCES):
             lines = [l for l in err.splitlines() if "packed-fp32-ops" not in l and l.strip()]
             if lines:
                 print(f"[{src}]\n" + "\n".join(lines))
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

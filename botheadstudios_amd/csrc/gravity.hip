@@ -3,18 +3,22 @@
 // Reference: bh_gravity.wgsl:337-394 (cs_gravity_bh: per-particle stack walk, robust MAC
 // dist >= |bmax - bmin|/theta + |centre - com|, leaf buckets summed directly) and :313-328 / sph_step.wgsl:
 // 235-241 (direct sum). Here one wave64 walks the tree for 64 Morton-consecutive particles at once: a
-// node is accepted only when the MAC holds for every point of the group's bounding box (distance from the
-// node COM to the box, softened) and the node box does not overlap the group box, so each particle sees a
-// node set at least as refined as the reference's per-particle walk. Accepted cells and leaf particles are
+// node is accepted only when the MAC holds for every point of the group's bounding box (unsoftened
+// distance from the node COM to the box), so each particle sees a node set at least as refined as the
+// reference's per-particle walk. Accepted cells and leaf particles are
 // appended to LDS lists with ballot/mbcnt compaction and evaluated 64 at a time with LDS broadcast reads.
 // The traversal stack lives in LDS; up to 64 nodes are popped per round (one per lane), which leaves at
-// most ~64 pending nodes per tree level (2048 entries cover the tree depth with room to spare); the pop
-// width narrows to depth-first near capacity, which bounds the stack by capacity - margin + tree depth.
+// most ~64 pending nodes per tree level (for compact groups the deepest stack measured at N = 1M is ~520
+// entries); the pop width narrows to depth-first near capacity, which bounds the stack by
+// capacity - margin + tree depth.
 #include "state.h"
 
 namespace isph {
 
-constexpr int kStackCap = 2048;
+#ifndef ISPH_GRAVITY_STACK
+#define ISPH_GRAVITY_STACK 1024
+#endif
+constexpr int kStackCap = ISPH_GRAVITY_STACK;
 constexpr int kStackMargin = 128;  // > maximum Karras depth (30 code bits + 32 index tie-break bits)
 constexpr int kBuf = 128;
 
@@ -29,7 +33,8 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                                                      const float4* __restrict__ nB, const float4* __restrict__ nC,
                                                      const float4* __restrict__ nQ0, const float2* __restrict__ nQ1,
                                                      float soft2, float4* __restrict__ sgrav,
-                                                     unsigned long long* counters) {
+                                                     unsigned long long* counters, const uint2* __restrict__ groups,
+                                                     uint32_t* scal, const uint4* __restrict__ nG, uint32_t nslots) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 cA[kBuf];
     __shared__ float4 cQ0[QUAD ? kBuf : 1];
@@ -38,13 +43,23 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
     __shared__ uint32_t pI[kBuf];
 
     const int lane = threadIdx.x;
-    const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x);
-    const uint32_t t = g * kWave + lane;
-    const bool valid = t < n;
-    const float4 me = spos[valid ? t : n - 1];
+    const uint32_t leaf0 = L - 1;
+    const uint32_t ngroups = nslots + scal[8];
+    unsigned long long tot_cell = 0, tot_part = 0, rounds = 0, popped = 0;
+    int max_sp = 1;
+    // Work queue over walk groups: every wave leaves the loop once the queue head passes the group count.
+    for (;;) {
+    uint32_t gi = 0;
+    if (lane == 0) gi = atomicAdd(&scal[9], 1u);
+    gi = __builtin_amdgcn_readfirstlane(gi);
+    if (gi >= ngroups) break;
+    const uint2 grp = groups[gi];
+    if (grp.y == 0) continue;  // a chunk slot covered by seam pieces
+    const uint32_t t = grp.x + lane;
+    const bool valid = (uint32_t)lane < grp.y;
+    const float4 me = spos[valid ? t : grp.x];
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
     const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
-    const uint32_t leaf0 = L - 1;
 
     float ax = 0.f, ay = 0.f, az = 0.f;
     KahanF kx{0.f, 0.f}, ky{0.f, 0.f}, kz{0.f, 0.f};
@@ -133,6 +148,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
     };
 
     int sp = 1, nc = 0, np = 0;
+    __syncthreads();
     if (lane == 0) stk[0] = 0;  // root (a single leaf when L == 1)
     __syncthreads();
     while (sp > 0) {
@@ -140,6 +156,8 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
         int room = kStackCap - kStackMargin - sp;
         if (room < k) k = max(1, room);
         sp -= k;
+        rounds += 1;
+        popped += k;
         const bool has = lane < k;
         const uint32_t node = has ? stk[sp + lane] : 0u;
         __syncthreads();
@@ -148,26 +166,38 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
         const bool is_leaf = has && node >= leaf0;
         bool accept = false, open = false;
         float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+        uint32_t lc = 0, rc = 0;
+        // Packed 32-byte record: (com, mass) then (MAC radius, left, right). One-particle leaves are never
+        // tested (bh_gravity.wgsl:356-369) and use the first half only: it is the particle itself. The MAC
+        // uses the unsoftened distance from the COM to the group box, d_min <= |x - com| for every member,
+        // so acceptance implies the reference's softened per-particle MAC for each of them, and
+        // d_min >= |bmax - bmin| / theta with theta < 1 keeps group particles outside accepted nodes.
+        if (has) {
+            const uint4 r0 = nG[2 * node];
+            A = make_float4(__uint_as_float(r0.x), __uint_as_float(r0.y), __uint_as_float(r0.z),
+                            __uint_as_float(r0.w));
+        }
         if (has && (!is_leaf || K > 1)) {
-            A = nA[node];
-            const float4 B = nB[node];
-            const float4 C = nC[node];
+            const uint4 r1 = nG[2 * node + 1];
+            const float rcrit = __uint_as_float(r1.x);
+            lc = r1.y;
+            rc = r1.z;
             float dx = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
             float dy = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
             float dz = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
-            float d2 = dx * dx + dy * dy + dz * dz + soft2;
-            bool disjoint = (B.x > gx1) | (C.x < gx0) | (B.y > gy1) | (C.y < gy0) | (B.z > gz1) | (C.z < gz0);
-            accept = disjoint && d2 >= C.w * C.w;
+            float d2 = dx * dx + dy * dy + dz * dz;
+            accept = d2 > 0.f && d2 >= rcrit * rcrit;
             open = !accept && !is_leaf;
         }
         const bool leaf = is_leaf && !accept;
         const uint64_t mo = __ballot(open);
         if (open) {
             int pos = sp + 2 * prefix_count(mo);
-            stk[pos] = left[node];
-            stk[pos + 1] = right[node];
+            stk[pos] = lc;
+            stk[pos + 1] = rc;
         }
         sp += 2 * __popcll(mo);
+        max_sp = max(max_sp, sp);
         const uint64_t ma = __ballot(accept);
         if (accept) {
             int pos = nc + prefix_count(ma);
@@ -189,7 +219,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                 if (!mp) break;
                 if (present) {
                     int pos = np + prefix_count(mp);
-                    pA[pos] = spos[start + kk];
+                    pA[pos] = K == 1 ? A : spos[start + kk];
                     pI[pos] = start + kk;
                 }
                 np += __popcll(mp);
@@ -221,11 +251,15 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
         az = kz.s;
     }
     if (valid) sgrav[t] = make_float4(kG * ax, kG * ay, kG * az, 0.f);
-    const uint64_t mv = __ballot(valid);
+    tot_cell += ncell * grp.y;
+    tot_part += npart * grp.y;
+    }
     if (lane == 0 && counters) {
-        unsigned long long nv = __popcll(mv);
-        atomicAdd(&counters[QUAD ? 2 : 0], ncell * nv);
-        atomicAdd(&counters[1], npart * nv);
+        atomicAdd(&counters[QUAD ? 2 : 0], tot_cell);
+        atomicAdd(&counters[1], tot_part);
+        atomicAdd(&counters[5], rounds);
+        atomicAdd(&counters[6], popped);
+        atomicMax(&counters[7], (unsigned long long)max_sp);
     }
 }
 
@@ -285,10 +319,11 @@ __global__ void k_scatter_gravity(uint32_t n, const uint32_t* __restrict__ order
 }
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s) {
-    uint32_t groups = (d.n + kWave - 1) / kWave;
+    uint32_t waves = std::min<uint32_t>(kWalkWaves, d.n);
     auto args = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(groups), dim3(kWave), 0, s, d.spos, d.n, d.K, d.L, d.left, d.right, d.nA, d.nB,
-                           d.nC, d.nQ0, d.nQ1, p.soft2, d.sgrav, d.counters);
+        hipLaunchKernelGGL(kern, dim3(waves), dim3(kWave), 0, s, d.spos, d.n, d.K, d.L, d.left, d.right, d.nA, d.nB,
+                           d.nC, d.nQ0, d.nQ1, p.soft2, d.sgrav, d.counters, d.groups, d.scal, d.nG,
+                           walk_chunk_slots(d, p));
     };
     if (p.quad) {
         if (p.mixed) args(k_gravity_tree<true, true>);

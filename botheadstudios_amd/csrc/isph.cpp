@@ -83,7 +83,7 @@ void free_all(isph_ctx* c) {
     Dev& d = c->d;
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
-                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.srho64, d.spr64, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.gflag, d.srho64, d.spr64, d.scs64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -126,21 +126,25 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.ready, cap);
     dalloc(&d.nrange, cap);
     dalloc(&d.cross, cap);
+    dalloc(&d.groups, (size_t)cap + cap / kGroupMax + 16);
+    dalloc(&d.gflag, cap / kGroupMax + 16);
+    dalloc(&d.nG, 2 * nodes);
+    dalloc(&d.nLR, cap);
     dalloc(&d.nA, nodes);
     dalloc(&d.nB, nodes);
     dalloc(&d.nC, nodes);
     dalloc(&d.nQ0, nodes);
     dalloc(&d.nQ1, nodes);
-    dalloc(&d.scal, 8);
+    dalloc(&d.scal, 16);
     dalloc(&d.dt, 1);
     dalloc(&d.time, 1);
-    dalloc(&d.counters, 8);
+    dalloc(&d.counters, 16);
     dalloc(&d.red, 1024 * 16);
     dalloc(&d.eos, kMaxMaterials);
     HIP_OK(hipMalloc(&d.aos, (size_t)cap * sizeof(isph_particle)));
     HIP_OK(hipHostMalloc((void**)&c->pinned, (size_t)cap * sizeof(isph_particle), hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&c->rb_event, hipEventDisableTiming));
-    HIP_OK(hipMemset(d.counters, 0, 8 * sizeof(unsigned long long)));
+    HIP_OK(hipMemset(d.counters, 0, 16 * sizeof(unsigned long long)));
     HIP_OK(hipMemset(d.time, 0, sizeof(double)));
     HIP_OK(hipMemset(d.dt, 0, sizeof(float)));
     HIP_OK(hipMemset(d.eos, 0, kMaxMaterials * sizeof(Till)));
@@ -152,6 +156,7 @@ void ensure64(isph_ctx* c) {
         dalloc(&c->d.vel64, 3 * (size_t)c->d.cap);
         dalloc(&c->d.srho64, c->d.cap);
         dalloc(&c->d.spr64, c->d.cap);
+        dalloc(&c->d.scs64, c->d.cap);
     }
 }
 
@@ -233,6 +238,7 @@ void build_tree(isph_ctx* c, const Dev& d) {
     }
     Phase ph(c, ISPH_PHASE_UPSWEEP);
     launch_upsweep(d, c->p, c->stream);
+    launch_groups(d, c->p, c->stream);
 }
 
 void run_gravity(isph_ctx* c, const Dev& d, int mode) {
@@ -259,7 +265,7 @@ void eval(isph_ctx* c, bool rebuild) {
         launch_density(d, c->p, c->stream);
     } else {
         Phase ph(c, ISPH_PHASE_DENSITY);
-        launch_reset_scalars(d, c->stream);
+        launch_reset_scalars(d, c->stream, 1);
         launch_gather_vel(d, c->stream);
         launch_eos(d, c->stream);
     }
@@ -439,6 +445,16 @@ int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int q
     });
 }
 
+int isph_set_walk(isph_t* h, float group_extent, float chunk_extent) {
+    return guarded(h, [&]() -> int {
+        if (!(group_extent > 0.0f) || chunk_extent < 0.0f) return fail(h, ISPH_EINVAL, "bad walk extents");
+        h->p.group_extent = group_extent;
+        h->p.chunk_extent = chunk_extent;
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
 int isph_set_precision(isph_t* h, int mode) {
     return guarded(h, [&]() -> int {
         if (mode != ISPH_PRECISION_FP32 && mode != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "bad precision");
@@ -472,6 +488,7 @@ static Dev integ_view(isph_ctx* c) {
         d.vel64 = nullptr;
         d.srho64 = nullptr;
         d.spr64 = nullptr;
+        d.scs64 = nullptr;
     }
     return d;
 }
@@ -694,7 +711,7 @@ int isph_kernel_stats(isph_t* h, double* out7) {
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));
         flush_timing(h);
-        unsigned long long cnt[8];
+        unsigned long long cnt[16];
         HIP_OK(hipMemcpy(cnt, h->d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
         out7[0] = h->grav_launches;
         out7[1] = h->phase_ms[ISPH_PHASE_GRAVITY];
@@ -708,13 +725,24 @@ int isph_kernel_stats(isph_t* h, double* out7) {
     });
 }
 
+int isph_walk_stats(isph_t* h, double* out9) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        unsigned long long cnt[16];
+        HIP_OK(hipMemcpy(cnt, h->d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+        for (int k = 0; k < 9; ++k) out9[k] = (double)cnt[5 + k];
+        return ISPH_OK;
+    });
+}
+
 int isph_reset_stats(isph_t* h) {
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));
         flush_timing(h);
         for (double& x : h->phase_ms) x = 0.0;
         h->grav_launches = 0;
-        HIP_OK(hipMemsetAsync(h->d.counters, 0, 8 * sizeof(unsigned long long), h->stream));
+        HIP_OK(hipMemsetAsync(h->d.counters, 0, 16 * sizeof(unsigned long long), h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         return ISPH_OK;
     });

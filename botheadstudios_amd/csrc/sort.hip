@@ -11,12 +11,19 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per block per pass
 
-__global__ void k_reset_scalars(uint32_t* scal, unsigned long long* counters, int reset_counters) {
+// mode 0: everything before a tree build; mode 1: only what a force evaluation on the existing tree uses
+// (Courant minimum, force-walk queue head).
+__global__ void k_reset_scalars(uint32_t* scal, int mode) {
     int i = threadIdx.x;
+    if (mode == 1) {
+        if (i == 6) scal[i] = 0x7f800000u;
+        if (i == 11) scal[i] = 0u;
+        return;
+    }
     if (i < 3) scal[i] = 0xffffffffu;              // min slots start at +inf key (bh_gravity.wgsl:64-65)
     else if (i < 6) scal[i] = 0u;                  // max slots at -inf key
     else if (i == 6) scal[i] = 0x7f800000u;        // Courant signal min: +inf bits
-    else if (i == 7) scal[i] = 0u;                 // chunk-crossing internal nodes of the next tree build
+    else if (i < 12) scal[i] = 0u;                 // crossing nodes, walk groups, work-queue heads
 }
 
 // Block-reduced float-radix min/max (bh_gravity.wgsl:67-78 does one atomic per particle per axis).
@@ -106,26 +113,37 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (256 * nb entries) by one block of 1024 threads.
+// Exclusive scan of the digit-major histogram (256 * nb entries) by one block of 1024 threads, through LDS
+// tiles of 16384 entries: coalesced loads, a contiguous 16-entry run per thread, a block scan of the runs.
 __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, uint32_t total) {
+    constexpr uint32_t kTile = 16384, kPer = kTile / 1024;
+    __shared__ uint32_t tile[kTile];
     __shared__ uint32_t part[1024];
-    uint32_t per = (total + 1023) / 1024;
-    uint32_t b = threadIdx.x * per, e = min(b + per, total);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += hist[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = threadIdx.x >= (uint32_t)o ? part[threadIdx.x - o] : 0u;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < total; base += kTile) {
+        uint32_t cnt = min(kTile, total - base);
+        for (uint32_t k = threadIdx.x; k < kTile; k += 1024) tile[k] = k < cnt ? hist[base + k] : 0u;
         __syncthreads();
-        part[threadIdx.x] += v;
+        uint32_t s = 0;
+        for (uint32_t k = 0; k < kPer; ++k) s += tile[threadIdx.x * kPer + k];
+        part[threadIdx.x] = s;
         __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
-    for (uint32_t i = b; i < e; ++i) {
-        uint32_t v = hist[i];
-        hist[i] = run;
-        run += v;
+        for (int o = 1; o < 1024; o <<= 1) {
+            uint32_t v = threadIdx.x >= (uint32_t)o ? part[threadIdx.x - o] : 0u;
+            __syncthreads();
+            part[threadIdx.x] += v;
+            __syncthreads();
+        }
+        uint32_t run = carry + part[threadIdx.x] - s;
+        for (uint32_t k = 0; k < kPer; ++k) {
+            uint32_t v = tile[threadIdx.x * kPer + k];
+            tile[threadIdx.x * kPer + k] = run;
+            run += v;
+        }
+        carry += part[1023];
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < cnt; k += 1024) hist[base + k] = tile[k];
+        __syncthreads();
     }
 }
 
@@ -204,8 +222,8 @@ __global__ __launch_bounds__(256) void k_gather_vel(uint32_t n, const uint32_t* 
 
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-void launch_reset_scalars(const Dev& d, hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, d.scal, d.counters, 0);
+void launch_reset_scalars(const Dev& d, hipStream_t s, int mode) {
+    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, d.scal, mode);
 }
 void launch_bbox(const Dev& d, const float4* pos, hipStream_t s) {
     uint32_t blocks = std::min<uint32_t>(cdiv(d.n, 256), 2048);

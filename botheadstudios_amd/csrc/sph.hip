@@ -7,13 +7,14 @@
 // wave walks the tree for 64 Morton-consecutive particles, keeping every node whose box comes within
 // (h_group_max + h_node_max)/2 of the group box, so the candidate set contains every pair with
 // r < (h_i + h_j)/2 exactly once; each lane then applies the exact radius test, as the reference's
-// cell-membership-guarded 27-cell scan does. Pressure and sound speed are computed once per particle (the
+// cell-membership-guarded 27-cell scan does. Waves take walk groups (compact subtrees of <= 64 particles)
+// from a work queue. Pressure and sound speed are computed once per particle (the
 // reference shader re-evaluates six Tillotson calls per neighbour pair, sph_step.wgsl:273-275).
 #include "state.h"
 
 namespace isph {
 
-constexpr int kStackCap = 2048;
+constexpr int kStackCap = 1024;
 constexpr int kStackMargin = 128;
 constexpr int kBuf = 128;
 
@@ -32,13 +33,15 @@ static_assert(sizeof(ParticleRec) == 48, "SphParticle layout");
 // indices (cnt <= 64); it must evaluate them for the calling lane.
 template <class Flush>
 __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, uint32_t n, uint32_t K, uint32_t L,
-                                               const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                                               const uint2* __restrict__ nlr,
                                                const float4* __restrict__ nB, const float4* __restrict__ nC,
                                                float gx0, float gy0, float gz0, float gx1, float gy1, float gz1,
-                                               float hgmax, Flush&& flush) {
+                                               float hgmax, unsigned long long* stats, Flush&& flush) {
     const int lane = threadIdx.x;
     const uint32_t leaf0 = L - 1;
     int sp = 1, nq = 0;
+    unsigned long long rounds = 0, popped = 0;
+    int max_sp = 1;
     if (lane == 0) stk[0] = 0;
     __syncthreads();
     auto shift = [&](int cnt) {
@@ -51,13 +54,21 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
         int room = kStackCap - kStackMargin - sp;
         if (room < k) k = max(1, room);
         sp -= k;
+        rounds += 1;
+        popped += k;
         const bool has = lane < k;
         const uint32_t node = has ? stk[sp + lane] : 0u;
         __syncthreads();
         bool keep = false;
+        uint32_t lc = 0, rc = 0;
         if (has) {
             const float4 B = nB[node];
             const float4 C = nC[node];
+            if (node < leaf0) {  // child indices travel with the box: one memory round trip per round
+                const uint2 ch = nlr[node];
+                lc = ch.x;
+                rc = ch.y;
+            }
             float dx = fmaxf(fmaxf(B.x - gx1, gx0 - C.x), 0.f);
             float dy = fmaxf(fmaxf(B.y - gy1, gy0 - C.y), 0.f);
             float dz = fmaxf(fmaxf(B.z - gz1, gz0 - C.z), 0.f);
@@ -69,10 +80,11 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
         const uint64_t mo = __ballot(open);
         if (open) {
             int pos = sp + 2 * prefix_count(mo);
-            stk[pos] = left[node];
-            stk[pos + 1] = right[node];
+            stk[pos] = lc;
+            stk[pos + 1] = rc;
         }
         sp += 2 * __popcll(mo);
+        max_sp = max(max_sp, sp);
         const uint64_t ml = __ballot(leaf);
         if (ml) {
             const uint32_t c = leaf ? node - leaf0 : 0u;
@@ -99,12 +111,17 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
         flush(nq);
         __syncthreads();
     }
+    if (lane == 0 && stats) {
+        atomicAdd(&stats[0], rounds);
+        atomicAdd(&stats[1], popped);
+        atomicMax(&stats[2], (unsigned long long)max_sp);
+    }
 }
 
 // Per-particle EOS + Courant signal epilogue shared by the density and the EOS-only kernels.
 __device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, double rho, float h, float4 v, uint32_t m,
                                              const Till* __restrict__ eos, float* srho, float* spr, float* scs,
-                                             double* srho64, double* spr64, uint32_t* scal) {
+                                             double* srho64, double* spr64, double* scs64, uint32_t* scal) {
     float sig = INFINITY;
     if (valid) {
         const Till& T = eos[m];
@@ -118,6 +135,7 @@ __device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, double rho,
         if (srho64) {
             srho64[t] = rho;
             spr64[t] = p / (r * r);
+            scs64[t] = c;
         }
         float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
         sig = h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
@@ -129,23 +147,33 @@ __device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, double rho,
 template <bool F64>
 __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                 const float4* __restrict__ svel, const uint32_t* __restrict__ smat,
-                                                uint32_t n, uint32_t K, uint32_t L, const uint32_t* __restrict__ left,
-                                                const uint32_t* __restrict__ right, const float4* __restrict__ nB,
+                                                uint32_t n, uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
+                                                const float4* __restrict__ nB,
                                                 const float4* __restrict__ nC, const uint32_t* __restrict__ order,
                                                 const Till* __restrict__ eos, float* __restrict__ srho,
                                                 float* __restrict__ spr, float* __restrict__ scs,
                                                 float* __restrict__ rho_orig, double* __restrict__ srho64,
-                                                double* __restrict__ spr64, uint32_t* scal,
-                                                unsigned long long* counters) {
+                                                double* __restrict__ spr64, double* __restrict__ scs64, uint32_t* scal,
+                                                unsigned long long* counters, const uint2* __restrict__ groups,
+                                                uint32_t nslots) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ uint32_t cand[kBuf];
     __shared__ float4 cP[kWave];
     __shared__ float cH[kWave];
     const int lane = threadIdx.x;
-    const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x);
-    const uint32_t t = g * kWave + lane;
-    const bool valid = t < n;
-    const uint32_t tc = valid ? t : n - 1;
+    const uint32_t ngroups = nslots + scal[8];
+    unsigned long long tested_all = 0;
+    for (;;) {
+    uint32_t gi = 0;
+    if (lane == 0) gi = atomicAdd(&scal[10], 1u);
+    gi = __builtin_amdgcn_readfirstlane(gi);
+    if (gi >= ngroups) break;
+    const uint2 grp = groups[gi];
+    if (grp.y == 0) continue;  // a chunk slot covered by seam pieces
+    const uint32_t t = grp.x + lane;
+    const bool valid = (uint32_t)lane < grp.y;
+    const uint32_t tc = valid ? t : grp.x;
+    __syncthreads();
     const float4 me = spos[tc];
     const float hi = sh[tc];
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
@@ -153,12 +181,13 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
     const float hg = wave_max(hi);
     // Plain: f32 pair arithmetic and sum (the reference shader's arithmetic). Mixed (F64): the density feeds
     // the Tillotson pressure, whose cold branch P = A (rho/rho0 - 1) + ... cancels near rho0 and amplifies a
-    // relative density error by A / P (~1e3), so the true-pair kernel sums run in f64 (the coordinate
-    // differences of f32 positions are exact; only ~50 pairs per particle pass the radius test).
+    // relative density error by A / P (~1e3), so the true-pair kernel sums run in f64, from f64 differences
+    // of the f32 coordinates (only ~50 pairs per particle pass the radius test).
     float rho = 0.f;
     double rho64 = 0.0;
     unsigned long long tested = 0;
-    neighbour_walk(stk, cand, n, K, L, left, right, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, [&](int cnt) {
+    neighbour_walk(stk, cand, n, K, L, nlr, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
+                   counters ? counters + 8 : nullptr, [&](int cnt) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
             cP[lane] = spos[j];
@@ -173,7 +202,8 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
             float hij = 0.5f * (hi + cH[jj]);
             if (r2 < hij * hij && cand[jj] != t) {  // sph_step.wgsl:201,210
                 if (F64) {
-                    double ddx = dx, ddy = dy, ddz = dz;
+                    double ddx = (double)me.x - (double)pj.x, ddy = (double)me.y - (double)pj.y,
+                           ddz = (double)me.z - (double)pj.z;
                     double h2 = 0.5 * ((double)hi + (double)cH[jj]);
                     double q = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / h2;
                     double sig = 8.0 / (3.14159265358979323846 * h2 * h2 * h2);
@@ -203,33 +233,36 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
     } else {
         rho_i = valid ? (double)rho_orig[order[t]] : 1.0;
     }
-    eos_epilogue(valid, t, rho_i, hi, svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scal);
-    const uint64_t mv = __ballot(valid);
-    if (lane == 0 && counters) atomicAdd(&counters[3], tested * (unsigned long long)__popcll(mv));
+    eos_epilogue(valid, t, rho_i, hi, svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scs64, scal);
+    tested_all += tested * grp.y;
+    }
+    if (lane == 0 && counters) atomicAdd(&counters[3], tested_all);
 }
 
 // EOS + signal only (the density of the previous evaluation is reused: positions have not moved).
 __global__ __launch_bounds__(64) void k_eos(const float* __restrict__ sh, const float4* __restrict__ svel,
                                             const uint32_t* __restrict__ smat, uint32_t n, const Till* __restrict__ eos,
                                             float* __restrict__ srho, float* __restrict__ spr, float* __restrict__ scs,
-                                            double* __restrict__ srho64, double* __restrict__ spr64, uint32_t* scal) {
+                                            double* __restrict__ srho64, double* __restrict__ spr64,
+                                            double* __restrict__ scs64, uint32_t* scal) {
     const uint32_t t = blockIdx.x * kWave + threadIdx.x;
     const bool valid = t < n;
     const uint32_t tc = valid ? t : n - 1;
     const double rho = srho64 ? srho64[tc] : (double)srho[tc];
-    eos_epilogue(valid, t, rho, sh[tc], svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scal);
+    eos_epilogue(valid, t, rho, sh[tc], svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scs64, scal);
 }
 
 template <bool F64>
 __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
                                               const float4* __restrict__ svel, const float* __restrict__ srho,
                                               const float* __restrict__ spr, const float* __restrict__ scs, uint32_t n,
-                                              uint32_t K, uint32_t L, const uint32_t* __restrict__ left,
-                                              const uint32_t* __restrict__ right, const float4* __restrict__ nB,
+                                              uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
+                                              const float4* __restrict__ nB,
                                               const float4* __restrict__ nC, const uint32_t* __restrict__ order,
                                               const float4* __restrict__ sgrav, float4* __restrict__ acc_orig,
                                               const double* __restrict__ srho64, const double* __restrict__ spr64,
-                                              float av_alpha, float av_beta, unsigned long long* counters) {
+                                              const double* __restrict__ scs64, float av_alpha, float av_beta, unsigned long long* counters,
+                                              const uint2* __restrict__ groups, uint32_t* scal, uint32_t nslots) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ uint32_t cand[kBuf];
     __shared__ float4 cP[kWave];  // x, y, z, mass
@@ -237,11 +270,21 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
     __shared__ float4 cS[kWave];  // h, P/rho^2, c
     __shared__ double cR64[F64 ? kWave : 1];
     __shared__ double cPr64[F64 ? kWave : 1];
+    __shared__ double cC64[F64 ? kWave : 1];
     const int lane = threadIdx.x;
-    const uint32_t g = xcd_swizzle(blockIdx.x, gridDim.x);
-    const uint32_t t = g * kWave + lane;
-    const bool valid = t < n;
-    const uint32_t tc = valid ? t : n - 1;
+    const uint32_t ngroups = nslots + scal[8];
+    unsigned long long tested_all = 0;
+    for (;;) {
+    uint32_t gi = 0;
+    if (lane == 0) gi = atomicAdd(&scal[11], 1u);
+    gi = __builtin_amdgcn_readfirstlane(gi);
+    if (gi >= ngroups) break;
+    const uint2 grp = groups[gi];
+    if (grp.y == 0) continue;  // a chunk slot covered by seam pieces
+    const uint32_t t = grp.x + lane;
+    const bool valid = (uint32_t)lane < grp.y;
+    const uint32_t tc = valid ? t : grp.x;
+    __syncthreads();
     const float4 me = spos[tc];
     const float hi = sh[tc];
     const float4 vi = svel[tc];
@@ -250,14 +293,15 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
     const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
     const float hg = wave_max(hi);
     // Plain: f32 pair arithmetic and sums (the reference shader's). Mixed (F64): the true pairs (about 50
-    // per particle) are evaluated and summed in f64 from exact f32 coordinate differences, with the f64
+    // per particle) are evaluated and summed in f64 from f64 coordinate differences, with the f64
     // density and P / rho^2 of the EOS pass; du/dt cancels across pairs, so f32 pair terms cap its
     // relative accuracy near 1e-4.
     float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f;
     double ax64 = 0.0, ay64 = 0.0, az64 = 0.0, de64 = 0.0;
-    const double rhoi64 = F64 ? srho64[tc] : 0.0, pri64 = F64 ? spr64[tc] : 0.0;
+    const double rhoi64 = F64 ? srho64[tc] : 0.0, pri64 = F64 ? spr64[tc] : 0.0, ci64 = F64 ? scs64[tc] : 0.0;
     unsigned long long tested = 0;
-    neighbour_walk(stk, cand, n, K, L, left, right, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, [&](int cnt) {
+    neighbour_walk(stk, cand, n, K, L, nlr, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
+                   counters ? counters + 11 : nullptr, [&](int cnt) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
             cP[lane] = spos[j];
@@ -267,6 +311,7 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
             if (F64) {
                 cR64[lane] = srho64[j];
                 cPr64[lane] = spr64[j];
+                cC64[lane] = scs64[j];
             }
         }
         __syncthreads();
@@ -274,7 +319,8 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
             if (F64) {
                 float4 pj = cP[jj];
                 float4 sj = cS[jj];
-                double dx = (double)(me.x - pj.x), dy = (double)(me.y - pj.y), dz = (double)(me.z - pj.z);
+                double dx = (double)me.x - (double)pj.x, dy = (double)me.y - (double)pj.y,
+                       dz = (double)me.z - (double)pj.z;
                 double r2 = dx * dx + dy * dy + dz * dz;
                 double hij = 0.5 * ((double)hi + (double)sj.x);
                 if (r2 < hij * hij && r2 > 1.0e-18 && cand[jj] != t) {
@@ -286,7 +332,7 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
                     double pij = 0.0;
                     if (vr < 0.0) {
                         double mu = hij * vr / (r2 + 0.01 * hij * hij);
-                        double cbar = 0.5 * ((double)ci + (double)sj.z);
+                        double cbar = 0.5 * (ci64 + cC64[jj]);
                         double rbar = 0.5 * (rhoi64 + cR64[jj]);
                         pij = (-(double)av_alpha * cbar * mu + (double)av_beta * mu * mu) / rbar;
                     }
@@ -345,8 +391,9 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
         float4 out = me.w > 0.f ? make_float4(gr.x + ax, gr.y + ay, gr.z + az, de) : make_float4(0.f, 0.f, 0.f, 0.f);
         acc_orig[order[t]] = out;
     }
-    const uint64_t mv = __ballot(valid);
-    if (lane == 0 && counters) atomicAdd(&counters[4], tested * (unsigned long long)__popcll(mv));
+    tested_all += tested * grp.y;
+    }
+    if (lane == 0 && counters) atomicAdd(&counters[4], tested_all);
 }
 
 // ---- KDK (sph_step.wgsl:304-333; hydrostatic.rs:451-465), retired particles are not integrated ----------
@@ -575,30 +622,33 @@ __global__ __launch_bounds__(256) void k_diag_pe(uint32_t n, const float4* posh,
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
-    uint32_t groups = cdiv(d.n, kWave);
+    uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
     if (p.mixed)
         hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
-                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.srho64,
-                           d.spr64, d.scal, d.counters);
+                           d.L, d.nLR, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.srho64,
+                           d.spr64, d.scs64, d.scal, d.counters, d.groups, walk_chunk_slots(d, p));
     else
         hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
-                           d.L, d.left, d.right, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho,
-                           (double*)nullptr, (double*)nullptr, d.scal, d.counters);
+                           d.L, d.nLR, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho,
+                           (double*)nullptr, (double*)nullptr, (double*)nullptr, d.scal, d.counters, d.groups,
+                           walk_chunk_slots(d, p));
 }
 void launch_eos(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_eos, dim3(cdiv(d.n, kWave)), dim3(kWave), 0, s, d.sh, d.svel, d.smat, d.n, d.eos, d.srho,
-                       d.spr, d.scs, d.srho64, d.spr64, d.scal);
+                       d.spr, d.scs, d.srho64, d.spr64, d.scs64, d.scal);
 }
 void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
-    uint32_t groups = cdiv(d.n, kWave);
+    uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
     if (p.mixed)
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
-                           d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
-                           p.av_alpha, p.av_beta, d.counters);
+                           d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
+                           d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, walk_chunk_slots(d, p));
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
-                           d.scs, d.n, d.K, d.L, d.left, d.right, d.nB, d.nC, d.vals, d.sgrav, d.acc,
-                           (const double*)nullptr, (const double*)nullptr, p.av_alpha, p.av_beta, d.counters);
+                           d.scs, d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc,
+                           (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
+                           p.av_beta, d.counters,
+                           d.groups, d.scal, walk_chunk_slots(d, p));
 }
 void launch_kick_drift(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,

@@ -44,6 +44,7 @@ struct Dev {
     float* scs = nullptr;    // sound speed
     double* srho64 = nullptr;  // mixed precision: f64 density and P / rho^2 (sorted)
     double* spr64 = nullptr;
+    double* scs64 = nullptr;
     float4* sgrav = nullptr; // gravitational acceleration, sorted
     uint32_t* lcode = nullptr;  // leaf cluster codes [L]
 
@@ -54,6 +55,10 @@ struct Dev {
     uint32_t* ready = nullptr;
     uint2* nrange = nullptr;    // [first, last] leaf range of each internal node
     uint32_t* cross = nullptr;  // internal nodes whose range crosses an upsweep chunk boundary
+    uint2* groups = nullptr;    // walk groups: (first sorted particle, count); chunk slots then seam pieces
+    uint32_t* gflag = nullptr;  // 64-particle chunks that straddle a Morton seam
+    uint4* nG = nullptr;   // gravity walk record, 2 x 16 B: (com.xyz, mass), (MAC radius bits, left, right, 0)
+    uint2* nLR = nullptr;  // children of internal nodes, for the SPH walks
     float4* nA = nullptr;  // com.xyz, mass
     float4* nB = nullptr;  // bmin.xyz, max h
     float4* nC = nullptr;  // bmax.xyz, MAC radius size/theta + |centre - com|
@@ -61,7 +66,9 @@ struct Dev {
     float2* nQ1 = nullptr; // quadrupole xz, yz
 
     // scalars
-    uint32_t* scal = nullptr;  // [0..5] bbox keys, [6] Courant signal min (float bits), [7] crossing-node count
+    // [0..5] bbox keys, [6] Courant signal min (float bits), [7] crossing-node count, [8] walk groups,
+    // [9..11] work-queue heads of the gravity, density and force walks
+    uint32_t* scal = nullptr;
     float* dt = nullptr;       // current dt
     double* time = nullptr;    // accumulated simulated time
     unsigned long long* counters = nullptr;  // [0] cells, [1] particles, [2] quad cells, [3] dens pairs, [4] force pairs
@@ -80,10 +87,15 @@ struct StepParams {
     float av_beta = 2.0f;
     int quad = 0;
     int mixed = 0;
+    float group_extent = 4.0f;  // subtree walk groups: box diagonal <= this many times the largest h
+    float chunk_extent = 0.0f;  // 64-particle chunk groups: box diagonal limit in the same unit (0 = off)
 };
 
+constexpr uint32_t kGroupMax = 64;  // particles per walk group (one per lane)
+constexpr uint32_t kWalkWaves = 8192; // resident waves of the work-queue walks
+
 // ---- launchers (each .hip file implements its own) ----
-void launch_reset_scalars(const Dev& d, hipStream_t s);
+void launch_reset_scalars(const Dev& d, hipStream_t s, int mode = 0);
 void launch_bbox(const Dev& d, const float4* pos, hipStream_t s);
 void launch_morton(const Dev& d, const float4* pos, hipStream_t s);
 void launch_radix_sort(const Dev& d, hipStream_t s);  // 30-bit keys, 4 passes; result in keys/vals
@@ -92,6 +104,8 @@ void launch_gather_vel(const Dev& d, hipStream_t s);
 
 void launch_karras(const Dev& d, hipStream_t s);
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s);
+void launch_groups(const Dev& d, const StepParams& p, hipStream_t s);
+uint32_t walk_chunk_slots(const Dev& d, const StepParams& p);  // leading chunk slots of the group list
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s);
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s);

@@ -5,16 +5,18 @@
 // :186-241 (single-pass climb). The reference engine replaced its climb with 32+ level-synchronous sweeps
 // because WGSL cannot order the sibling reads (bh_gravity.wgsl:192-199, gpu_gravity.rs:484-519). Here the
 // climb runs in two launches whose arrivals are ordered with workgroup-scope acquire/release only (a
-// device-scope fence writes back the XCD-local L2 on every climber): pass 1 climbs inside chunks of 256
-// leaves, pass 2 (one workgroup) finishes the nodes that straddle chunks. Each node is computed from its
+// device-scope fence writes back the XCD-local L2 on every climber): pass 1 climbs inside chunks of 1024
+// leaves, pass 2 inside chunks of 65536 leaves, pass 3 (one workgroup) finishes the top of the tree. Each node is computed from its
 // two children in a fixed left/right order, so the result does not depend on arrival order.
 #include "state.h"
 
 namespace isph {
 
-// Leaves per upsweep chunk: internal nodes whose leaf range lies inside one chunk are finished by that
-// chunk's workgroup; the few whose range crosses a chunk boundary by one final workgroup.
-constexpr uint32_t kUpChunk = 256;
+// Upsweep levels: internal nodes whose leaf range lies inside one kUpChunk-leaf chunk are finished by that
+// chunk's workgroup (pass 1); nodes crossing those chunks but inside one kUpChunk2-leaf chunk by that
+// chunk's workgroup (pass 2); the few crossing kUpChunk2 chunks by a single workgroup (pass 3).
+constexpr uint32_t kUpChunk = 1024;
+constexpr uint32_t kUpChunk2 = 65536;
 
 __device__ __forceinline__ int delta_codes(const uint32_t* __restrict__ codes, int L, int i, int j) {
     if (j < 0 || j >= L) return -1;
@@ -28,8 +30,8 @@ __device__ __forceinline__ int delta_codes(const uint32_t* __restrict__ codes, i
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, uint32_t L,
                                                 uint32_t* __restrict__ left, uint32_t* __restrict__ right,
                                                 uint32_t* __restrict__ parent, uint32_t* __restrict__ ready,
-                                                uint2* __restrict__ nrange, uint32_t* __restrict__ cross,
-                                                uint32_t* scal) {
+                                                uint2* __restrict__ nrange, uint2* __restrict__ nlr,
+                                                uint32_t* __restrict__ cross, uint32_t* scal) {
     uint32_t ii = blockIdx.x * blockDim.x + threadIdx.x;
     if (ii == 0) parent[0] = kNoParent;
     if (ii + 1 >= L) return;  // L-1 internal nodes
@@ -66,6 +68,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     parent[li] = ii;
     parent[ri] = ii;
     nrange[ii] = make_uint2((uint32_t)first, (uint32_t)last);
+    nlr[ii] = make_uint2(li, ri);
     if ((uint32_t)first / kUpChunk != (uint32_t)last / kUpChunk) cross[atomicAdd(&scal[7], 1u)] = ii;
 }
 
@@ -104,6 +107,7 @@ struct TreeView {
     const uint32_t* parent;
     const uint2* nrange;
     uint32_t* ready;
+    uint4* nG;
     float4* nA;
     float4* nB;
     float4* nC;
@@ -111,10 +115,14 @@ struct TreeView {
     float2* nQ1;
 };
 
-__device__ __forceinline__ void store_node(const TreeView& tv, uint32_t i, const NodeMom& m, int quad) {
+__device__ __forceinline__ void store_node(const TreeView& tv, uint32_t i, const NodeMom& m, int quad, uint32_t l,
+                                           uint32_t r) {
     tv.nA[i] = m.a;
     tv.nB[i] = m.b;
     tv.nC[i] = m.c;
+    tv.nG[2 * i] = make_uint4(__float_as_uint(m.a.x), __float_as_uint(m.a.y), __float_as_uint(m.a.z),
+                              __float_as_uint(m.a.w));
+    tv.nG[2 * i + 1] = make_uint4(__float_as_uint(m.c.w), l, r, 0u);
     if (quad) {
         tv.nQ0[i] = m.q0;
         tv.nQ1[i] = m.q1;
@@ -154,9 +162,9 @@ __device__ __forceinline__ bool second_arrival(uint32_t* ready, uint32_t node) {
     return __hip_atomic_fetch_add(&ready[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
 }
 
-__device__ __forceinline__ bool crosses(const TreeView& tv, uint32_t node) {
+__device__ __forceinline__ bool crosses(const TreeView& tv, uint32_t node, uint32_t chunk = kUpChunk) {
     uint2 r = tv.nrange[node];
-    return r.x / kUpChunk != r.y / kUpChunk;
+    return r.x / chunk != r.y / chunk;
 }
 
 // Pass 1: one thread per leaf (bucket moments, bh_gravity.wgsl:200-223), climbing (:224-240) through the
@@ -196,52 +204,147 @@ __global__ __launch_bounds__(kUpChunk) void k_upsweep_chunk(const float4* __rest
     lm.a = make_float4(com.x, com.y, com.z, m);
     lm.b = make_float4(lo.x, lo.y, lo.z, hmax);
     lm.c = make_float4(hi.x, hi.y, hi.z, mac_radius(lo, hi, com, inv_theta));
-    store_node(tv, li, lm, quad);
+    store_node(tv, li, lm, quad, 0u, 0u);
 
     uint32_t node = tv.parent[li];
     while (node != kNoParent && !crosses(tv, node)) {
         if (!second_arrival(tv.ready, node)) break;  // first child: the sibling's climber finishes it
         NodeMom nm = merge_children(tv, node, inv_theta, quad);
-        store_node(tv, node, nm, quad);
+        store_node(tv, node, nm, quad, tv.left[node], tv.right[node]);
         node = tv.parent[node];
     }
 }
 
-// Pass 2: one workgroup finishes the chunk-crossing internal nodes. Their non-crossing children are
-// complete (pass 1, previous launch), so each crossing node receives one arrival per such child here and
-// the climb continues to the root through crossing nodes only.
-__global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict__ cross, const uint32_t* scal,
-                                                      uint32_t L, TreeView tv, float inv_theta, int quad) {
+// Passes 2 and 3: a workgroup finishes the crossing internal nodes of its kUpChunk2-leaf chunk (pass 2,
+// nodes crossing kUpChunk but not kUpChunk2 chunks) or of the whole tree (pass 3, one workgroup, nodes
+// crossing kUpChunk2 chunks). Children finished by an earlier pass count as complete: each crossing node
+// receives one arrival per complete child from the thread that owns it, and the climb continues upward
+// through nodes of the same pass.
+__global__ __launch_bounds__(1024) void k_upsweep_cross(const uint32_t* __restrict__ cross, const uint32_t* scal,
+                                                        uint32_t L, TreeView tv, float inv_theta, int quad, int top) {
     const uint32_t count = scal[7];
+    const uint32_t lo = top ? kUpChunk2 : kUpChunk;  // children not crossing `lo` chunks are complete
     for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) {
         const uint32_t node = cross[k];
+        const bool big = crosses(tv, node, kUpChunk2);
+        if (top ? !big : (big || tv.nrange[node].x / kUpChunk2 != blockIdx.x)) continue;
         const uint32_t ch[2] = {tv.left[node], tv.right[node]};
         for (int side = 0; side < 2; ++side) {
             const uint32_t child = ch[side];
-            if (child < L - 1 && crosses(tv, child)) continue;  // a crossing child arrives by its own climb
+            if (child < L - 1 && crosses(tv, child, lo)) continue;  // arrives by its own climb in this pass
             uint32_t at = node;
-            while (at != kNoParent) {
+            while (at != kNoParent && (top || !crosses(tv, at, kUpChunk2))) {
                 if (!second_arrival(tv.ready, at)) break;
                 NodeMom nm = merge_children(tv, at, inv_theta, quad);
-                store_node(tv, at, nm, quad);
+                store_node(tv, at, nm, quad, tv.left[at], tv.right[at]);
                 at = tv.parent[at];
             }
         }
     }
 }
 
+// Walk groups: the highest nodes holding <= kGroupMax particles whose box diagonal is at most `extent`
+// times the largest smoothing length of the scene (a leaf always qualifies). Both conditions only get
+// easier down the tree, so these nodes partition the leaves into spatially compact, Morton-contiguous
+// particle runs (consecutive-index groups can straddle a Morton seam and span distant regions).
+__device__ __forceinline__ bool group_ok(uint32_t v, uint32_t n, uint32_t K, uint32_t L, const uint2* nrange,
+                                         const float4* nB, const float4* nC, float lim2, uint32_t& start,
+                                         uint32_t& count) {
+    uint32_t first, last;
+    bool leaf = v >= L - 1;
+    if (leaf) {
+        first = last = v - (L - 1);
+    } else {
+        uint2 r = nrange[v];
+        first = r.x;
+        last = r.y;
+    }
+    start = first * K;
+    uint32_t end = min((last + 1) * K, n);
+    count = end - start;
+    if (leaf) return true;
+    if (count > kGroupMax) return false;
+    float4 b = nB[v], c = nC[v];
+    float ex = c.x - b.x, ey = c.y - b.y, ez = c.z - b.z;
+    return ex * ex + ey * ey + ez * ez <= lim2;
+}
+
+// Walk groups, in two parts. First, every run of 64 consecutive sorted particles whose box diagonal is
+// within the limit is one group, stored at its chunk index (full waves, Morton order). Then the chunks that
+// straddle a Morton seam are covered by their intersections with the tree groups above (at most two per
+// tree group), appended after the chunks. The walk kernels skip empty chunk slots.
+__global__ __launch_bounds__(256) void k_chunk_groups(const float4* __restrict__ spos, uint32_t n,
+                                                      const float4* __restrict__ nB, float extent,
+                                                      uint2* __restrict__ groups, uint32_t* __restrict__ gflag) {
+    const uint32_t chunk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const uint32_t nchunks = (n + kGroupMax - 1) / kGroupMax;
+    if (chunk >= nchunks) return;  // whole wave
+    const uint32_t a = chunk * kGroupMax;
+    const uint32_t cnt = min(kGroupMax, n - a);
+    const uint32_t lane = threadIdx.x % kWave;
+    const float4 p = spos[a + min(lane, cnt - 1)];
+    float ex = wave_max(p.x) - wave_min(p.x), ey = wave_max(p.y) - wave_min(p.y), ez = wave_max(p.z) - wave_min(p.z);
+    const float lim = extent * nB[0].w;
+    const bool ok = cnt <= 1 || (extent > 0.0f && ex * ex + ey * ey + ez * ez <= lim * lim);
+    if (lane == 0) {
+        groups[chunk] = make_uint2(a, ok ? cnt : 0u);
+        gflag[chunk] = ok ? 0u : 1u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_groups(uint32_t n, uint32_t K, uint32_t L, const uint32_t* __restrict__ parent,
+                                                const uint2* __restrict__ nrange, const float4* __restrict__ nB,
+                                                const float4* __restrict__ nC, float extent, uint2* __restrict__ groups,
+                                                const uint32_t* __restrict__ gflag, uint32_t nchunks, uint32_t* scal) {
+    uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 2 * L - 1) return;
+    const float lim = extent * nB[0].w;  // the root's box carries the scene's largest h
+    const float lim2 = lim * lim;
+    uint32_t start, count, ps, pc;
+    if (!group_ok(v, n, K, L, nrange, nB, nC, lim2, start, count)) return;
+    uint32_t p = parent[v];
+    if (p != kNoParent && group_ok(p, n, K, L, nrange, nB, nC, lim2, ps, pc)) return;
+    if (nchunks == 0) {  // chunk groups disabled: the tree groups are the walk groups
+        groups[atomicAdd(&scal[8], 1u)] = make_uint2(start, count);
+        return;
+    }
+    const uint32_t end = start + count;
+    for (uint32_t c = start / kGroupMax; c * kGroupMax < end; ++c) {
+        if (!gflag[c]) continue;
+        uint32_t s = max(start, c * kGroupMax), e = min(end, (c + 1) * kGroupMax);
+        groups[nchunks + atomicAdd(&scal[8], 1u)] = make_uint2(s, e - s);
+    }
+}
+
+uint32_t walk_chunk_slots(const Dev& d, const StepParams& p) {
+    return p.chunk_extent > 0.0f ? (d.n + kGroupMax - 1) / kGroupMax : 0u;
+}
+
+void launch_groups(const Dev& d, const StepParams& p, hipStream_t s) {
+    uint32_t nchunks = walk_chunk_slots(d, p);
+    if (nchunks)
+        hipLaunchKernelGGL(k_chunk_groups, dim3((nchunks + 3) / 4), dim3(256), 0, s, d.spos, d.n, d.nB,
+                           p.chunk_extent, d.groups, d.gflag);
+    uint32_t nodes = 2 * d.L - 1;
+    hipLaunchKernelGGL(k_groups, dim3((nodes + 255) / 256), dim3(256), 0, s, d.n, d.K, d.L, d.parent, d.nrange, d.nB,
+                       d.nC, p.group_extent, d.groups, d.gflag, nchunks, d.scal);
+}
+
 void launch_karras(const Dev& d, hipStream_t s) {
     uint32_t threads = d.L > 1 ? d.L - 1 : 1;
     hipLaunchKernelGGL(k_karras, dim3((threads + 255) / 256), dim3(256), 0, s, d.lcode, d.L, d.left, d.right, d.parent,
-                       d.ready, d.nrange, d.cross, d.scal);
+                       d.ready, d.nrange, d.nLR, d.cross, d.scal);
 }
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s) {
-    TreeView tv{d.left, d.right, d.parent, d.nrange, d.ready, d.nA, d.nB, d.nC, d.nQ0, d.nQ1};
+    TreeView tv{d.left, d.right, d.parent, d.nrange, d.ready, d.nG, d.nA, d.nB, d.nC, d.nQ0, d.nQ1};
     float inv_theta = 1.0f / p.theta;
     hipLaunchKernelGGL(k_upsweep_chunk, dim3((d.L + kUpChunk - 1) / kUpChunk), dim3(kUpChunk), 0, s, d.spos, d.sh,
                        d.n, d.K, d.L, tv, inv_theta, p.quad);
     if (d.L > kUpChunk)
-        hipLaunchKernelGGL(k_upsweep_top, dim3(1), dim3(1024), 0, s, d.cross, d.scal, d.L, tv, inv_theta, p.quad);
+        hipLaunchKernelGGL(k_upsweep_cross, dim3((d.L + kUpChunk2 - 1) / kUpChunk2), dim3(1024), 0, s, d.cross, d.scal,
+                           d.L, tv, inv_theta, p.quad, 0);
+    if (d.L > kUpChunk2)
+        hipLaunchKernelGGL(k_upsweep_cross, dim3(1), dim3(1024), 0, s, d.cross, d.scal, d.L, tv, inv_theta, p.quad, 1);
 }
 
 }  // namespace isph

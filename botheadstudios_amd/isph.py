@@ -14,7 +14,7 @@ from .materials import EOS_DTYPE
 from .scene import PARTICLE_DTYPE
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libisph.so")
+LIB_PATH = os.environ.get("ISPH_LIBRARY", os.path.join(_HERE, "libisph.so"))
 
 OK = 0
 EAGAIN = -11
@@ -35,6 +35,7 @@ _SIGS = {
     "isph_set_av": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float]),
     "isph_set_gravity": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, ctypes.c_int]),
     "isph_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
+    "isph_set_walk": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float]),
     "isph_set_reuse": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_encode_kdk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "isph_step_cfl": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32]),
@@ -50,6 +51,7 @@ _SIGS = {
     "isph_phase_times": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "isph_kernel_stats": (ctypes.c_int, [_P, _P]),
     "isph_reset_stats": (ctypes.c_int, [_P]),
+    "isph_walk_stats": (ctypes.c_int, [_P, _P]),
     "isph_probe_morton": (ctypes.c_int, [_P, _P, _P, _P]),
     "isph_probe_sort": (ctypes.c_int, [_P, _P, _P]),
     "isph_probe_tree": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
@@ -135,6 +137,9 @@ class GpuSph:
     def set_gravity(self, mode=GRAVITY_TREE, theta=0.5, leaf_size=1, quadrupole=False):
         self._check(self._lib.isph_set_gravity(self._h, int(mode), float(theta), int(leaf_size), int(bool(quadrupole))))
 
+    def set_walk(self, group_extent=4.0, chunk_extent=0.0):
+        self._check(self._lib.isph_set_walk(self._h, float(group_extent), float(chunk_extent)))
+
     def set_precision(self, mode):
         self._check(self._lib.isph_set_precision(self._h, int(mode)))
 
@@ -200,6 +205,12 @@ class GpuSph:
         self._check(self._lib.isph_kernel_stats(self._h, _ptr(out)))
         keys = ("gravity_launches", "gravity_ms", "cell_interactions", "particle_interactions",
                 "quad_interactions", "density_pairs_tested", "force_pairs_tested", "gravity_flops")
+        return dict(zip(keys, out.tolist()))
+
+    def walk_stats(self):
+        out = np.zeros(9, np.float64)
+        self._check(self._lib.isph_walk_stats(self._h, _ptr(out)))
+        keys = [f"{w}_{k}" for w in ("gravity", "density", "force") for k in ("rounds", "popped", "max_stack")]
         return dict(zip(keys, out.tolist()))
 
     def reset_stats(self):

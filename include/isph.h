@@ -104,6 +104,11 @@ int isph_set_av(isph_t* h, float alpha, float beta);
 /* Gravity configuration (the reference hard-codes THETA=0.5, BUCKET_K=1: gpu_gravity.rs:55,62).
  * leaf_size in [1, 64]; quadrupole != 0 adds the node quadrupole to accepted cells. */
 int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int quadrupole);
+/* Shape of the wave-cooperative tree walks (no reference counterpart; the autotuner's knob): particles are
+ * walked in groups of <= 64 (one per lane). A run of 64 consecutive Morton-sorted particles is one group
+ * when its box diagonal is <= chunk_extent * (largest h) (0 disables such runs); everything else is
+ * covered by the largest tree nodes of <= 64 particles with diagonal <= group_extent * (largest h). */
+int isph_set_walk(isph_t* h, float group_extent, float chunk_extent);
 int isph_set_precision(isph_t* h, int mode);
 /* Reuse the tree, gravity and density of the previous evaluation when positions have not moved since
  * (eval 2 of step n and eval 1 of step n+1 see identical positions). Results are bit-identical. */
@@ -152,6 +157,9 @@ int isph_phase_times(isph_t* h, double* ms, size_t n);
  * [4] = quadrupole cell interactions, [5] = density pairs tested, [6] = force pairs tested,
  * [7] = algorithmic gravity flops (interactions times the per-interaction flop constants). */
 int isph_kernel_stats(isph_t* h, double* out8);
+/* Tree-walk statistics since the last reset: out[0..2] = gravity walk rounds, nodes popped, deepest stack;
+ * out[3..5] the same for the density walk; out[6..8] for the force walk. */
+int isph_walk_stats(isph_t* h, double* out9);
 int isph_reset_stats(isph_t* h);
 
 /* ---- stage probes (parity of the bit-exact stages against the oracle) ------------------------------- */
