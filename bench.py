@@ -57,6 +57,17 @@ def lscpu_model():
     return "unknown"
 
 
+def measured_traffic():
+    """HBM bytes per gravity launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the
+    factors measured on plain streaming kernels); None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_gravity_tree_two_body_1m_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(sc):
     """One force evaluation (density + forces_and_dudt = half a KDK step) of the oracle's restatement of
     HydroBody (hydrostatic.rs:338-446, octree gravity theta 0.5) on the same scene, 1 thread."""
@@ -150,7 +161,8 @@ def main():
                    "n": n, "precision": args.precision, "reuse": bool(args.reuse)},
         "roofline": {"bound": "mfma", "engine": "valu-fp32", "kernel": "k_gravity_tree",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": measured_traffic() if (n == 1048576 and args.precision == "fp32") else None,
                      "flops_per_launch": flops / launches,
                      "avg_launch_ms": st["gravity_ms"] / launches},
         "phases_ms_per_step": {k: v / args.steps for k, v in r["phases"].items()},
