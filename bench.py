@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--group-extent", type=float, default=4.0)
-    ap.add_argument("--chunk-extent", type=float, default=0.0)
+    ap.add_argument("--ppl", type=int, default=2, help="gravity walk particles per lane")
+    ap.add_argument("--expand", type=int, default=4, help="SPH walks take subtrees of <= this many particles whole")
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-energy", action="store_true", help="skip the O(N^2) energy diagnostics")
@@ -94,7 +95,7 @@ def run_gpu(sc, args, precision, steps, warmup):
     s.set_eos_f64(sc.eos64)
     s.set_gravity(isph.GRAVITY_TREE, args.theta, args.leaf, args.quadrupole)
     s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
-    s.set_walk(args.group_extent, args.chunk_extent)
+    s.set_walk(args.group_extent, args.ppl, args.expand)
     s.set_reuse(args.reuse)
     s.step_cfl(args.cfl, warmup)
     s.synchronize()

@@ -2,11 +2,12 @@
 //
 // Reference: bh_gravity.wgsl:337-394 (cs_gravity_bh: per-particle stack walk, robust MAC
 // dist >= |bmax - bmin|/theta + |centre - com|, leaf buckets summed directly) and :313-328 / sph_step.wgsl:
-// 235-241 (direct sum). Here one wave64 walks the tree for 64 Morton-consecutive particles at once: a
-// node is accepted only when the MAC holds for every point of the group's bounding box (unsoftened
-// distance from the node COM to the box), so each particle sees a node set at least as refined as the
-// reference's per-particle walk. Accepted cells and leaf particles are
-// appended to LDS lists with ballot/mbcnt compaction and evaluated 64 at a time with LDS broadcast reads.
+// 235-241 (direct sum). Here one wave64 walks the tree for one walk group at once: a compact LBVH subtree
+// of up to 64 x P particles (P per lane), taken from per-XCD work queues. A node is accepted only when the
+// MAC holds for every point of the group's bounding box (unsoftened distance from the node COM to the box),
+// so each particle sees a node set at least as refined as the reference's per-particle walk. Accepted
+// cells and leaf particles are appended to LDS lists with ballot/mbcnt compaction and evaluated 64 at a
+// time with LDS broadcast reads (P interactions per read).
 // The traversal stack lives in LDS; up to 64 nodes are popped per round (one per lane), which leaves at
 // most ~64 pending nodes per tree level (for compact groups the deepest stack measured at N = 1M is ~520
 // entries); the pop width narrows to depth-first near capacity, which bounds the stack by
@@ -26,15 +27,13 @@ struct Acc3 {
     float x, y, z;
 };
 
-template <bool QUAD, bool COMP>
+template <int P, bool QUAD, bool COMP>
 __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ spos, uint32_t n, uint32_t K,
-                                                     uint32_t L, const uint32_t* __restrict__ left,
-                                                     const uint32_t* __restrict__ right, const float4* __restrict__ nA,
-                                                     const float4* __restrict__ nB, const float4* __restrict__ nC,
+                                                     uint32_t L, const uint4* __restrict__ nG,
                                                      const float4* __restrict__ nQ0, const float2* __restrict__ nQ1,
                                                      float soft2, float4* __restrict__ sgrav,
                                                      unsigned long long* counters, const uint2* __restrict__ groups,
-                                                     uint32_t* scal, const uint4* __restrict__ nG, uint32_t nslots) {
+                                                     uint32_t* scal) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 cA[kBuf];
     __shared__ float4 cQ0[QUAD ? kBuf : 1];
@@ -44,215 +43,240 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
 
     const int lane = threadIdx.x;
     const uint32_t leaf0 = L - 1;
-    const uint32_t ngroups = nslots + scal[8];
     unsigned long long tot_cell = 0, tot_part = 0, rounds = 0, popped = 0;
     int max_sp = 1;
-    // Work queue over walk groups: every wave leaves the loop once the queue head passes the group count.
-    for (;;) {
-    uint32_t gi = 0;
-    if (lane == 0) gi = atomicAdd(&scal[9], 1u);
-    gi = __builtin_amdgcn_readfirstlane(gi);
-    if (gi >= ngroups) break;
-    const uint2 grp = groups[gi];
-    if (grp.y == 0) continue;  // a chunk slot covered by seam pieces
-    const uint32_t t = grp.x + lane;
-    const bool valid = (uint32_t)lane < grp.y;
-    const float4 me = spos[valid ? t : grp.x];
-    const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
-    const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
+    WalkQueue queue(scal + kQueueGrav, scal[kGroupsGrav]);
+    uint32_t gi;
+    while (queue.next(gi)) {
+        const uint2 grp = groups[gi];
+        uint32_t t[P];
+        bool valid[P];
+        float4 me[P];
+        float lx0 = INFINITY, ly0 = INFINITY, lz0 = INFINITY, lx1 = -INFINITY, ly1 = -INFINITY, lz1 = -INFINITY;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            const uint32_t slot = (uint32_t)lane + (uint32_t)k * kWave;
+            t[k] = grp.x + slot;
+            valid[k] = slot < grp.y;
+            me[k] = spos[valid[k] ? t[k] : grp.x];
+            lx0 = fminf(lx0, me[k].x); ly0 = fminf(ly0, me[k].y); lz0 = fminf(lz0, me[k].z);
+            lx1 = fmaxf(lx1, me[k].x); ly1 = fmaxf(ly1, me[k].y); lz1 = fmaxf(lz1, me[k].z);
+        }
+        const float gx0 = wave_min(lx0), gy0 = wave_min(ly0), gz0 = wave_min(lz0);
+        const float gx1 = wave_max(lx1), gy1 = wave_max(ly1), gz1 = wave_max(lz1);
 
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    KahanF kx{0.f, 0.f}, ky{0.f, 0.f}, kz{0.f, 0.f};
-    unsigned long long ncell = 0, npart = 0;
+        float ax[P], ay[P], az[P];
+        KahanF kx[P], ky[P], kz[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            ax[k] = ay[k] = az[k] = 0.f;
+            kx[k] = KahanF{0.f, 0.f};
+            ky[k] = KahanF{0.f, 0.f};
+            kz[k] = KahanF{0.f, 0.f};
+        }
+        unsigned long long ncell = 0, npart = 0;
 
-    auto flush_cells = [&](int cnt) {
-        float bx = 0.f, by = 0.f, bz = 0.f;
-        for (int j = 0; j < cnt; ++j) {
-            float4 c = cA[j];
-            float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
-            float r2 = dx * dx + dy * dy + dz * dz + soft2;
-            float inv = __builtin_amdgcn_rsqf(r2);
-            float inv2 = inv * inv;
-            float w = c.w * inv * inv2;
-            if (QUAD) {
-                float4 q0 = cQ0[j];
-                float2 q1 = cQ1[j];
-                float qx = q0.x * dx + q0.w * dy + q1.x * dz;
-                float qy = q0.w * dx + q0.y * dy + q1.y * dz;
-                float qz = q1.x * dx + q1.y * dy + q0.z * dz;
-                float dqd = qx * dx + qy * dy + qz * dz;
-                float inv5 = inv * inv2 * inv2;
-                float s = w + 2.5f * dqd * inv5 * inv2;
-                bx += dx * s - qx * inv5;
-                by += dy * s - qy * inv5;
-                bz += dz * s - qz * inv5;
+        auto add = [&](int k, float bx, float by, float bz) {
+            if (COMP) {
+                kx[k].add(bx);
+                ky[k].add(by);
+                kz[k].add(bz);
             } else {
-                bx += dx * w;
-                by += dy * w;
-                bz += dz * w;
+                ax[k] += bx;
+                ay[k] += by;
+                az[k] += bz;
             }
-        }
-        if (COMP) {
-            kx.add(bx);
-            ky.add(by);
-            kz.add(bz);
-        } else {
-            ax += bx;
-            ay += by;
-            az += bz;
-        }
-        ncell += cnt;
-    };
-    auto flush_parts = [&](int cnt) {
-        float bx = 0.f, by = 0.f, bz = 0.f;
-        for (int j = 0; j < cnt; ++j) {
-            float4 c = pA[j];
-            float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
-            float r2 = dx * dx + dy * dy + dz * dz + soft2;
-            float inv = __builtin_amdgcn_rsqf(r2);
-            float w = (pI[j] == t) ? 0.f : c.w * inv * inv * inv;  // s != t (bh_gravity.wgsl:362)
-            bx += dx * w;
-            by += dy * w;
-            bz += dz * w;
-        }
-        if (COMP) {
-            kx.add(bx);
-            ky.add(by);
-            kz.add(bz);
-        } else {
-            ax += bx;
-            ay += by;
-            az += bz;
-        }
-        npart += cnt;
-    };
-    // Drop the first 64 entries of a list after they were evaluated.
-    auto shift_cells = [&](int cnt) {
-        __syncthreads();
-        if (lane + kWave < cnt) {
-            cA[lane] = cA[lane + kWave];
-            if (QUAD) {
-                cQ0[lane] = cQ0[lane + kWave];
-                cQ1[lane] = cQ1[lane + kWave];
+        };
+        auto flush_cells = [&](int cnt) {
+            float bx[P], by[P], bz[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) bx[k] = by[k] = bz[k] = 0.f;
+            for (int j = 0; j < cnt; ++j) {
+                const float4 c = cA[j];
+                float4 q0;
+                float2 q1;
+                if (QUAD) {
+                    q0 = cQ0[j];
+                    q1 = cQ1[j];
+                }
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    float dx = c.x - me[k].x, dy = c.y - me[k].y, dz = c.z - me[k].z;
+                    float r2 = dx * dx + dy * dy + dz * dz + soft2;
+                    float inv = __builtin_amdgcn_rsqf(r2);
+                    float inv2 = inv * inv;
+                    float w = c.w * inv * inv2;
+                    if (QUAD) {
+                        float qx = q0.x * dx + q0.w * dy + q1.x * dz;
+                        float qy = q0.w * dx + q0.y * dy + q1.y * dz;
+                        float qz = q1.x * dx + q1.y * dy + q0.z * dz;
+                        float dqd = qx * dx + qy * dy + qz * dz;
+                        float inv5 = inv * inv2 * inv2;
+                        float s = w + 2.5f * dqd * inv5 * inv2;
+                        bx[k] += dx * s - qx * inv5;
+                        by[k] += dy * s - qy * inv5;
+                        bz[k] += dz * s - qz * inv5;
+                    } else {
+                        bx[k] += dx * w;
+                        by[k] += dy * w;
+                        bz[k] += dz * w;
+                    }
+                }
             }
-        }
-        __syncthreads();
-    };
-    auto shift_parts = [&](int cnt) {
-        __syncthreads();
-        if (lane + kWave < cnt) {
-            pA[lane] = pA[lane + kWave];
-            pI[lane] = pI[lane + kWave];
-        }
-        __syncthreads();
-    };
+#pragma unroll
+            for (int k = 0; k < P; ++k) add(k, bx[k], by[k], bz[k]);
+            ncell += cnt;
+        };
+        auto flush_parts = [&](int cnt) {
+            float bx[P], by[P], bz[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) bx[k] = by[k] = bz[k] = 0.f;
+            for (int j = 0; j < cnt; ++j) {
+                const float4 c = pA[j];
+                const uint32_t idx = pI[j];
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    float dx = c.x - me[k].x, dy = c.y - me[k].y, dz = c.z - me[k].z;
+                    float r2 = dx * dx + dy * dy + dz * dz + soft2;
+                    float inv = __builtin_amdgcn_rsqf(r2);
+                    float w = (idx == t[k]) ? 0.f : c.w * inv * inv * inv;  // s != t (bh_gravity.wgsl:362)
+                    bx[k] += dx * w;
+                    by[k] += dy * w;
+                    bz[k] += dz * w;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < P; ++k) add(k, bx[k], by[k], bz[k]);
+            npart += cnt;
+        };
+        // Drop the first 64 entries of a list after they were evaluated.
+        auto shift_cells = [&](int cnt) {
+            __syncthreads();
+            if (lane + kWave < cnt) {
+                cA[lane] = cA[lane + kWave];
+                if (QUAD) {
+                    cQ0[lane] = cQ0[lane + kWave];
+                    cQ1[lane] = cQ1[lane + kWave];
+                }
+            }
+            __syncthreads();
+        };
+        auto shift_parts = [&](int cnt) {
+            __syncthreads();
+            if (lane + kWave < cnt) {
+                pA[lane] = pA[lane + kWave];
+                pI[lane] = pI[lane + kWave];
+            }
+            __syncthreads();
+        };
 
-    int sp = 1, nc = 0, np = 0;
-    __syncthreads();
-    if (lane == 0) stk[0] = 0;  // root (a single leaf when L == 1)
-    __syncthreads();
-    while (sp > 0) {
-        int k = min(kWave, sp);
-        int room = kStackCap - kStackMargin - sp;
-        if (room < k) k = max(1, room);
-        sp -= k;
-        rounds += 1;
-        popped += k;
-        const bool has = lane < k;
-        const uint32_t node = has ? stk[sp + lane] : 0u;
+        int sp = 1, nc = 0, np = 0;
         __syncthreads();
-        // Leaves of one particle always go to the particle list (bh_gravity.wgsl:356-369); bucketed leaves
-        // (K > 1) are tested like internal nodes first and expanded only when the MAC fails.
-        const bool is_leaf = has && node >= leaf0;
-        bool accept = false, open = false;
-        float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
-        uint32_t lc = 0, rc = 0;
-        // Packed 32-byte record: (com, mass) then (MAC radius, left, right). One-particle leaves are never
-        // tested (bh_gravity.wgsl:356-369) and use the first half only: it is the particle itself. The MAC
-        // uses the unsoftened distance from the COM to the group box, d_min <= |x - com| for every member,
-        // so acceptance implies the reference's softened per-particle MAC for each of them, and
-        // d_min >= |bmax - bmin| / theta with theta < 1 keeps group particles outside accepted nodes.
-        if (has) {
-            const uint4 r0 = nG[2 * node];
-            A = make_float4(__uint_as_float(r0.x), __uint_as_float(r0.y), __uint_as_float(r0.z),
-                            __uint_as_float(r0.w));
-        }
-        if (has && (!is_leaf || K > 1)) {
-            const uint4 r1 = nG[2 * node + 1];
-            const float rcrit = __uint_as_float(r1.x);
-            lc = r1.y;
-            rc = r1.z;
-            float dx = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
-            float dy = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
-            float dz = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
-            float d2 = dx * dx + dy * dy + dz * dz;
-            accept = d2 > 0.f && d2 >= rcrit * rcrit;
-            open = !accept && !is_leaf;
-        }
-        const bool leaf = is_leaf && !accept;
-        const uint64_t mo = __ballot(open);
-        if (open) {
-            int pos = sp + 2 * prefix_count(mo);
-            stk[pos] = lc;
-            stk[pos + 1] = rc;
-        }
-        sp += 2 * __popcll(mo);
-        max_sp = max(max_sp, sp);
-        const uint64_t ma = __ballot(accept);
-        if (accept) {
-            int pos = nc + prefix_count(ma);
-            cA[pos] = A;
-            if (QUAD) {
-                cQ0[pos] = nQ0[node];
-                cQ1[pos] = nQ1[node];
+        if (lane == 0) stk[0] = 0;  // root (a single leaf when L == 1)
+        __syncthreads();
+        while (sp > 0) {
+            int k = min(kWave, sp);
+            int room = kStackCap - kStackMargin - sp;
+            if (room < k) k = max(1, room);
+            sp -= k;
+            rounds += 1;
+            popped += k;
+            const bool has = lane < k;
+            const uint32_t node = has ? stk[sp + lane] : 0u;
+            __syncthreads();
+            const bool is_leaf = has && node >= leaf0;
+            bool accept = false, open = false;
+            float4 A = make_float4(0.f, 0.f, 0.f, 0.f);
+            uint32_t lc = 0, rc = 0;
+            // Packed 32-byte record: (com, mass) then (MAC radius, left, right). One-particle leaves are never
+            // tested (bh_gravity.wgsl:356-369) and use the first half only: it is the particle itself. The MAC
+            // uses the unsoftened distance from the COM to the group box, d_min <= |x - com| for every member,
+            // so acceptance implies the reference's softened per-particle MAC for each of them, and
+            // d_min >= |bmax - bmin| / theta with theta < 1 keeps group particles outside accepted nodes.
+            if (has) {
+                const uint4 r0 = nG[2 * node];
+                A = make_float4(__uint_as_float(r0.x), __uint_as_float(r0.y), __uint_as_float(r0.z),
+                                __uint_as_float(r0.w));
             }
-        }
-        nc += __popcll(ma);
-        const uint64_t ml = __ballot(leaf);
-        if (ml) {
-            const uint32_t c = leaf ? node - leaf0 : 0u;
-            const uint32_t start = c * K;
-            const uint32_t end = leaf ? min(start + K, n) : 0u;
-            for (uint32_t kk = 0; kk < K; ++kk) {
-                const bool present = leaf && start + kk < end;
-                const uint64_t mp = __ballot(present);
-                if (!mp) break;
-                if (present) {
-                    int pos = np + prefix_count(mp);
-                    pA[pos] = K == 1 ? A : spos[start + kk];
-                    pI[pos] = start + kk;
-                }
-                np += __popcll(mp);
-                if (np >= kWave) {
-                    __syncthreads();
-                    flush_parts(kWave);
-                    shift_parts(np);
-                    np -= kWave;
+            if (has && (!is_leaf || K > 1)) {
+                const uint4 r1 = nG[2 * node + 1];
+                const float rcrit = __uint_as_float(r1.x);
+                lc = r1.y;
+                rc = r1.z;
+                float dx = fmaxf(fmaxf(gx0 - A.x, A.x - gx1), 0.f);
+                float dy = fmaxf(fmaxf(gy0 - A.y, A.y - gy1), 0.f);
+                float dz = fmaxf(fmaxf(gz0 - A.z, A.z - gz1), 0.f);
+                float d2 = dx * dx + dy * dy + dz * dz;
+                accept = d2 > 0.f && d2 >= rcrit * rcrit;
+                open = !accept && !is_leaf;
+            }
+            const bool leaf = is_leaf && !accept;
+            const uint64_t mo = __ballot(open);
+            if (open) {
+                int pos = sp + 2 * prefix_count(mo);
+                stk[pos] = lc;
+                stk[pos + 1] = rc;
+            }
+            sp += 2 * __popcll(mo);
+            max_sp = max(max_sp, sp);
+            const uint64_t ma = __ballot(accept);
+            if (accept) {
+                int pos = nc + prefix_count(ma);
+                cA[pos] = A;
+                if (QUAD) {
+                    cQ0[pos] = nQ0[node];
+                    cQ1[pos] = nQ1[node];
                 }
             }
+            nc += __popcll(ma);
+            const uint64_t ml = __ballot(leaf);
+            if (ml) {
+                const uint32_t c = leaf ? node - leaf0 : 0u;
+                const uint32_t start = c * K;
+                const uint32_t end = leaf ? min(start + K, n) : 0u;
+                for (uint32_t kk = 0; kk < K; ++kk) {
+                    const bool present = leaf && start + kk < end;
+                    const uint64_t mp = __ballot(present);
+                    if (!mp) break;
+                    if (present) {
+                        int pos = np + prefix_count(mp);
+                        pA[pos] = K == 1 ? A : spos[start + kk];
+                        pI[pos] = start + kk;
+                    }
+                    np += __popcll(mp);
+                    if (np >= kWave) {
+                        __syncthreads();
+                        flush_parts(kWave);
+                        shift_parts(np);
+                        np -= kWave;
+                    }
+                }
+            }
+            __syncthreads();
+            if (nc >= kWave) {
+                flush_cells(kWave);
+                shift_cells(nc);
+                nc -= kWave;
+            }
+            if (np >= kWave) {
+                flush_parts(kWave);
+                shift_parts(np);
+                np -= kWave;
+            }
         }
-        __syncthreads();
-        if (nc >= kWave) {
-            flush_cells(kWave);
-            shift_cells(nc);
-            nc -= kWave;
+        if (nc > 0) flush_cells(nc);
+        if (np > 0) flush_parts(np);
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (COMP) {
+                ax[k] = kx[k].s;
+                ay[k] = ky[k].s;
+                az[k] = kz[k].s;
+            }
+            if (valid[k]) sgrav[t[k]] = make_float4(kG * ax[k], kG * ay[k], kG * az[k], 0.f);
         }
-        if (np >= kWave) {
-            flush_parts(kWave);
-            shift_parts(np);
-            np -= kWave;
-        }
-    }
-    if (nc > 0) flush_cells(nc);
-    if (np > 0) flush_parts(np);
-    if (COMP) {
-        ax = kx.s;
-        ay = ky.s;
-        az = kz.s;
-    }
-    if (valid) sgrav[t] = make_float4(kG * ax, kG * ay, kG * az, 0.f);
-    tot_cell += ncell * grp.y;
-    tot_part += npart * grp.y;
+        tot_cell += ncell * grp.y;
+        tot_part += npart * grp.y;
     }
     if (lane == 0 && counters) {
         atomicAdd(&counters[QUAD ? 2 : 0], tot_cell);
@@ -320,17 +344,20 @@ __global__ void k_scatter_gravity(uint32_t n, const uint32_t* __restrict__ order
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t waves = std::min<uint32_t>(kWalkWaves, d.n);
-    auto args = [&](auto kern) {
-        hipLaunchKernelGGL(kern, dim3(waves), dim3(kWave), 0, s, d.spos, d.n, d.K, d.L, d.left, d.right, d.nA, d.nB,
-                           d.nC, d.nQ0, d.nQ1, p.soft2, d.sgrav, d.counters, d.groups, d.scal, d.nG,
-                           walk_chunk_slots(d, p));
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(waves), dim3(kWave), 0, s, d.spos, d.n, d.K, d.L, d.nG, d.nQ0, d.nQ1, p.soft2,
+                           d.sgrav, d.counters, d.ggroups, d.scal);
     };
-    if (p.quad) {
-        if (p.mixed) args(k_gravity_tree<true, true>);
-        else args(k_gravity_tree<true, false>);
-    } else {
-        if (p.mixed) args(k_gravity_tree<false, true>);
-        else args(k_gravity_tree<false, false>);
+    const int sel = (p.grav_ppl == 2 ? 4 : 0) | (p.quad ? 2 : 0) | (p.mixed ? 1 : 0);
+    switch (sel) {
+        case 0: go(k_gravity_tree<1, false, false>); break;
+        case 1: go(k_gravity_tree<1, false, true>); break;
+        case 2: go(k_gravity_tree<1, true, false>); break;
+        case 3: go(k_gravity_tree<1, true, true>); break;
+        case 4: go(k_gravity_tree<2, false, false>); break;
+        case 5: go(k_gravity_tree<2, false, true>); break;
+        case 6: go(k_gravity_tree<2, true, false>); break;
+        default: go(k_gravity_tree<2, true, true>); break;
     }
 }
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s) {

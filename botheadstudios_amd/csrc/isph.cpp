@@ -83,7 +83,7 @@ void free_all(isph_ctx* c) {
     Dev& d = c->d;
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
-                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.gflag, d.srho64, d.spr64, d.scs64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.ggroups, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -126,8 +126,11 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.ready, cap);
     dalloc(&d.nrange, cap);
     dalloc(&d.cross, cap);
-    dalloc(&d.groups, (size_t)cap + cap / kGroupMax + 16);
-    dalloc(&d.gflag, cap / kGroupMax + 16);
+    dalloc(&d.groups, cap);
+    dalloc(&d.ggroups, cap);
+    dalloc(&d.gflag, cap);
+    dalloc(&d.gscan, cap / kWave + 16);
+    HIP_OK(hipMemset(d.gflag, 0, (size_t)cap * sizeof(uint32_t)));
     dalloc(&d.nG, 2 * nodes);
     dalloc(&d.nLR, cap);
     dalloc(&d.nA, nodes);
@@ -135,7 +138,7 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.nC, nodes);
     dalloc(&d.nQ0, nodes);
     dalloc(&d.nQ1, nodes);
-    dalloc(&d.scal, 16);
+    dalloc(&d.scal, kScalars);
     dalloc(&d.dt, 1);
     dalloc(&d.time, 1);
     dalloc(&d.counters, 16);
@@ -238,7 +241,8 @@ void build_tree(isph_ctx* c, const Dev& d) {
     }
     Phase ph(c, ISPH_PHASE_UPSWEEP);
     launch_upsweep(d, c->p, c->stream);
-    launch_groups(d, c->p, c->stream);
+    launch_groups(d, c->p, kGroupMax, d.groups, d.scal + kGroupsSph, c->stream);
+    launch_groups(d, c->p, kGroupMax * (uint32_t)c->p.grav_ppl, d.ggroups, d.scal + kGroupsGrav, c->stream);
 }
 
 void run_gravity(isph_ctx* c, const Dev& d, int mode) {
@@ -445,11 +449,14 @@ int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int q
     });
 }
 
-int isph_set_walk(isph_t* h, float group_extent, float chunk_extent) {
+int isph_set_walk(isph_t* h, float group_extent, int grav_particles_per_lane, int sph_expand) {
     return guarded(h, [&]() -> int {
-        if (!(group_extent > 0.0f) || chunk_extent < 0.0f) return fail(h, ISPH_EINVAL, "bad walk extents");
+        if (!(group_extent > 0.0f) || (grav_particles_per_lane != 1 && grav_particles_per_lane != 2) ||
+            sph_expand < 1 || sph_expand > 64)
+            return fail(h, ISPH_EINVAL, "bad walk shape");
         h->p.group_extent = group_extent;
-        h->p.chunk_extent = chunk_extent;
+        h->p.grav_ppl = grav_particles_per_lane;
+        h->p.sph_expand = sph_expand;
         h->reuse_valid = false;
         return ISPH_OK;
     });

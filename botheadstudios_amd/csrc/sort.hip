@@ -12,18 +12,19 @@ constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per block per pass
 
 // mode 0: everything before a tree build; mode 1: only what a force evaluation on the existing tree uses
-// (Courant minimum, force-walk queue head).
-__global__ void k_reset_scalars(uint32_t* scal, int mode) {
-    int i = threadIdx.x;
-    if (mode == 1) {
-        if (i == 6) scal[i] = 0x7f800000u;
-        if (i == 11) scal[i] = 0u;
-        return;
+// (Courant minimum, force-walk queue heads).
+__global__ __launch_bounds__(1024) void k_reset_scalars(uint32_t* scal, int mode) {
+    for (int i = threadIdx.x; i < kScalars; i += blockDim.x) {
+        if (mode == 1) {
+            if (i == 6) scal[i] = 0x7f800000u;
+            if (i >= kQueueForce && i < kQueueForce + kQueueArea) scal[i] = 0u;
+            continue;
+        }
+        if (i < 3) scal[i] = 0xffffffffu;        // min slots start at +inf key (bh_gravity.wgsl:64-65)
+        else if (i < 6) scal[i] = 0u;            // max slots at -inf key
+        else if (i == 6) scal[i] = 0x7f800000u;  // Courant signal min: +inf bits
+        else scal[i] = 0u;                       // crossing nodes, group counts, work-queue areas
     }
-    if (i < 3) scal[i] = 0xffffffffu;              // min slots start at +inf key (bh_gravity.wgsl:64-65)
-    else if (i < 6) scal[i] = 0u;                  // max slots at -inf key
-    else if (i == 6) scal[i] = 0x7f800000u;        // Courant signal min: +inf bits
-    else if (i < 12) scal[i] = 0u;                 // crossing nodes, walk groups, work-queue heads
 }
 
 // Block-reduced float-radix min/max (bh_gravity.wgsl:67-78 does one atomic per particle per axis).
@@ -223,7 +224,7 @@ __global__ __launch_bounds__(256) void k_gather_vel(uint32_t n, const uint32_t* 
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
 void launch_reset_scalars(const Dev& d, hipStream_t s, int mode) {
-    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, d.scal, mode);
+    hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1024), 0, s, d.scal, mode);
 }
 void launch_bbox(const Dev& d, const float4* pos, hipStream_t s) {
     uint32_t blocks = std::min<uint32_t>(cdiv(d.n, 256), 2048);
@@ -246,6 +247,9 @@ void launch_radix_sort(const Dev& d, hipStream_t s) {
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
+}
+void launch_exclusive_scan(uint32_t* data, uint32_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, data, count);
 }
 void launch_gather(const Dev& d, const float4* pos, const float* mass, hipStream_t s) {
     hipLaunchKernelGGL(k_gather, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.K, d.vals, d.keys, pos, mass, d.velu,

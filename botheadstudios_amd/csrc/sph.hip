@@ -33,7 +33,8 @@ static_assert(sizeof(ParticleRec) == 48, "SphParticle layout");
 // indices (cnt <= 64); it must evaluate them for the calling lane.
 template <class Flush>
 __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, uint32_t n, uint32_t K, uint32_t L,
-                                               const uint2* __restrict__ nlr,
+                                               const uint2* __restrict__ nlr, const uint2* __restrict__ nrange,
+                                               uint32_t expand,
                                                const float4* __restrict__ nB, const float4* __restrict__ nC,
                                                float gx0, float gy0, float gz0, float gx1, float gy1, float gz1,
                                                float hgmax, unsigned long long* stats, Flush&& flush) {
@@ -75,8 +76,24 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
             float rr = 0.5f * (hgmax + B.w);
             keep = dx * dx + dy * dy + dz * dz < rr * rr;
         }
-        const bool leaf = keep && node >= leaf0;
-        const bool open = keep && node < leaf0;
+        // A kept subtree of <= `expand` particles is taken whole (its leaf range is contiguous in sorted
+        // order) instead of visiting its leaves one round at a time; the exact radius test follows anyway.
+        uint32_t first = 0, cnt = 0;
+        if (keep) {
+            if (node >= leaf0) {
+                first = (node - leaf0) * K;
+                cnt = min(first + K, n) - first;
+            } else {
+                const uint2 r = nrange[node];
+                const uint32_t pf = r.x * K, pe = min((r.y + 1) * K, n);
+                if (pe - pf <= expand) {
+                    first = pf;
+                    cnt = pe - pf;
+                }
+            }
+        }
+        const bool leaf = keep && cnt > 0;
+        const bool open = keep && cnt == 0;
         const uint64_t mo = __ballot(open);
         if (open) {
             int pos = sp + 2 * prefix_count(mo);
@@ -87,14 +104,12 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
         max_sp = max(max_sp, sp);
         const uint64_t ml = __ballot(leaf);
         if (ml) {
-            const uint32_t c = leaf ? node - leaf0 : 0u;
-            const uint32_t start = c * K;
-            const uint32_t end = leaf ? min(start + K, n) : 0u;
-            for (uint32_t kk = 0; kk < K; ++kk) {
-                const bool present = leaf && start + kk < end;
+            const uint32_t width = max(K, expand);
+            for (uint32_t kk = 0; kk < width; ++kk) {
+                const bool present = leaf && kk < cnt;
                 const uint64_t mp = __ballot(present);
                 if (!mp) break;
-                if (present) cand[nq + prefix_count(mp)] = start + kk;
+                if (present) cand[nq + prefix_count(mp)] = first + kk;
                 nq += __popcll(mp);
                 if (nq >= kWave) {
                     __syncthreads();
@@ -155,21 +170,17 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                                                 float* __restrict__ rho_orig, double* __restrict__ srho64,
                                                 double* __restrict__ spr64, double* __restrict__ scs64, uint32_t* scal,
                                                 unsigned long long* counters, const uint2* __restrict__ groups,
-                                                uint32_t nslots) {
+                                                const uint2* __restrict__ nrange, uint32_t expand) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ uint32_t cand[kBuf];
     __shared__ float4 cP[kWave];
     __shared__ float cH[kWave];
     const int lane = threadIdx.x;
-    const uint32_t ngroups = nslots + scal[8];
     unsigned long long tested_all = 0;
-    for (;;) {
-    uint32_t gi = 0;
-    if (lane == 0) gi = atomicAdd(&scal[10], 1u);
-    gi = __builtin_amdgcn_readfirstlane(gi);
-    if (gi >= ngroups) break;
+    WalkQueue queue(scal + kQueueDensity, scal[kGroupsSph]);
+    uint32_t gi;
+    while (queue.next(gi)) {
     const uint2 grp = groups[gi];
-    if (grp.y == 0) continue;  // a chunk slot covered by seam pieces
     const uint32_t t = grp.x + lane;
     const bool valid = (uint32_t)lane < grp.y;
     const uint32_t tc = valid ? t : grp.x;
@@ -186,7 +197,7 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
     float rho = 0.f;
     double rho64 = 0.0;
     unsigned long long tested = 0;
-    neighbour_walk(stk, cand, n, K, L, nlr, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
+    neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                    counters ? counters + 8 : nullptr, [&](int cnt) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
@@ -262,7 +273,8 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
                                               const float4* __restrict__ sgrav, float4* __restrict__ acc_orig,
                                               const double* __restrict__ srho64, const double* __restrict__ spr64,
                                               const double* __restrict__ scs64, float av_alpha, float av_beta, unsigned long long* counters,
-                                              const uint2* __restrict__ groups, uint32_t* scal, uint32_t nslots) {
+                                              const uint2* __restrict__ groups, uint32_t* scal,
+                                              const uint2* __restrict__ nrange, uint32_t expand) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ uint32_t cand[kBuf];
     __shared__ float4 cP[kWave];  // x, y, z, mass
@@ -272,15 +284,11 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
     __shared__ double cPr64[F64 ? kWave : 1];
     __shared__ double cC64[F64 ? kWave : 1];
     const int lane = threadIdx.x;
-    const uint32_t ngroups = nslots + scal[8];
     unsigned long long tested_all = 0;
-    for (;;) {
-    uint32_t gi = 0;
-    if (lane == 0) gi = atomicAdd(&scal[11], 1u);
-    gi = __builtin_amdgcn_readfirstlane(gi);
-    if (gi >= ngroups) break;
+    WalkQueue queue(scal + kQueueForce, scal[kGroupsSph]);
+    uint32_t gi;
+    while (queue.next(gi)) {
     const uint2 grp = groups[gi];
-    if (grp.y == 0) continue;  // a chunk slot covered by seam pieces
     const uint32_t t = grp.x + lane;
     const bool valid = (uint32_t)lane < grp.y;
     const uint32_t tc = valid ? t : grp.x;
@@ -300,7 +308,7 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
     double ax64 = 0.0, ay64 = 0.0, az64 = 0.0, de64 = 0.0;
     const double rhoi64 = F64 ? srho64[tc] : 0.0, pri64 = F64 ? spr64[tc] : 0.0, ci64 = F64 ? scs64[tc] : 0.0;
     unsigned long long tested = 0;
-    neighbour_walk(stk, cand, n, K, L, nlr, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
+    neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                    counters ? counters + 11 : nullptr, [&](int cnt) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
@@ -626,12 +634,12 @@ void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     if (p.mixed)
         hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
                            d.L, d.nLR, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.srho64,
-                           d.spr64, d.scs64, d.scal, d.counters, d.groups, walk_chunk_slots(d, p));
+                           d.spr64, d.scs64, d.scal, d.counters, d.groups, d.nrange, (uint32_t)p.sph_expand);
     else
         hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
                            d.L, d.nLR, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho,
                            (double*)nullptr, (double*)nullptr, (double*)nullptr, d.scal, d.counters, d.groups,
-                           walk_chunk_slots(d, p));
+                           d.nrange, (uint32_t)p.sph_expand);
 }
 void launch_eos(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_eos, dim3(cdiv(d.n, kWave)), dim3(kWave), 0, s, d.sh, d.svel, d.smat, d.n, d.eos, d.srho,
@@ -642,13 +650,13 @@ void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
     if (p.mixed)
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
                            d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
-                           d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, walk_chunk_slots(d, p));
+                           d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
+                           (uint32_t)p.sph_expand);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
                            d.scs, d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
-                           p.av_beta, d.counters,
-                           d.groups, d.scal, walk_chunk_slots(d, p));
+                           p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand);
 }
 void launch_kick_drift(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,

@@ -55,8 +55,10 @@ struct Dev {
     uint32_t* ready = nullptr;
     uint2* nrange = nullptr;    // [first, last] leaf range of each internal node
     uint32_t* cross = nullptr;  // internal nodes whose range crosses an upsweep chunk boundary
-    uint2* groups = nullptr;    // walk groups: (first sorted particle, count); chunk slots then seam pieces
-    uint32_t* gflag = nullptr;  // 64-particle chunks that straddle a Morton seam
+    uint2* groups = nullptr;    // SPH walk groups (first sorted particle, count), Morton order
+    uint2* ggroups = nullptr;   // gravity walk groups (up to 64 x particles-per-lane), Morton order
+    uint32_t* gflag = nullptr;  // group marks at their first particle (all zero between builds)
+    uint32_t* gscan = nullptr;  // per-64-particle mark counts / offsets
     uint4* nG = nullptr;   // gravity walk record, 2 x 16 B: (com.xyz, mass), (MAC radius bits, left, right, 0)
     uint2* nLR = nullptr;  // children of internal nodes, for the SPH walks
     float4* nA = nullptr;  // com.xyz, mass
@@ -66,8 +68,9 @@ struct Dev {
     float2* nQ1 = nullptr; // quadrupole xz, yz
 
     // scalars
-    // [0..5] bbox keys, [6] Courant signal min (float bits), [7] crossing-node count, [8] walk groups,
-    // [9..11] work-queue heads of the gravity, density and force walks
+    // [0..5] bbox keys, [6] Courant signal min (float bits), [7] crossing-node count, [8] SPH walk groups,
+    // [12] gravity walk groups, [512..) / [1024..) / [1536..) work-queue areas of the gravity, density and
+    // force walks
     uint32_t* scal = nullptr;
     float* dt = nullptr;       // current dt
     double* time = nullptr;    // accumulated simulated time
@@ -87,12 +90,71 @@ struct StepParams {
     float av_beta = 2.0f;
     int quad = 0;
     int mixed = 0;
-    float group_extent = 4.0f;  // subtree walk groups: box diagonal <= this many times the largest h
-    float chunk_extent = 0.0f;  // 64-particle chunk groups: box diagonal limit in the same unit (0 = off)
+    float group_extent = 4.0f;  // walk groups: box diagonal <= this many times the largest h
+    int grav_ppl = 2;           // gravity walk: particles per lane (1 or 2)
+    int sph_expand = 4;         // SPH walks take subtrees of <= this many particles whole as candidates
 };
 
-constexpr uint32_t kGroupMax = 64;  // particles per walk group (one per lane)
+constexpr uint32_t kGroupMax = 64;    // particles per SPH walk group (one per lane)
 constexpr uint32_t kWalkWaves = 8192; // resident waves of the work-queue walks
+#ifndef ISPH_QUEUES
+#define ISPH_QUEUES 8
+#endif
+#ifndef ISPH_QUEUE_STRIDE
+#define ISPH_QUEUE_STRIDE 32
+#endif
+#ifndef ISPH_QUEUE_MASK
+#define ISPH_QUEUE_MASK 1
+#endif
+constexpr uint32_t kQueues = ISPH_QUEUES;             // work queues per walk (1 or 8)
+constexpr uint32_t kQueueStride = ISPH_QUEUE_STRIDE;  // uint32 slots between queue heads (32 = 128 B line)
+static_assert(kQueues == 1 || kQueues == 8, "one queue or one per XCD");
+static_assert(kQueueStride >= 1 && kQueueStride * 8 < 512, "queue area overflow");
+constexpr int kQueueArea = 512;  // per walk: kQueues heads, then the exhausted-range mask
+constexpr int kGroupsSph = 8, kGroupsGrav = 12;
+constexpr int kQueueGrav = 512, kQueueDensity = 1024, kQueueForce = 1536;
+constexpr int kScalars = 2048;
+
+// Work queues over a Morton-ordered group list. With one queue per XCD the list is cut into 8 contiguous
+// ranges; a wave starts on the range of the XCD it was dispatched to (workgroup id mod 8) and moves on to
+// the next range when its own runs dry, so concurrently walked groups on one XCD are spatial neighbours and
+// share its L2. The wave that drains a range sets its bit in the area's mask, and later waves skip that
+// range with one load instead of a failing atomic. next() returns false once every range is drained, an
+// exit every wave reaches.
+struct WalkQueue {
+    uint32_t* area;
+    uint32_t total;
+    uint32_t q, tries;
+    __device__ WalkQueue(uint32_t* a, uint32_t total_groups) : area(a), total(total_groups) {
+        q = kQueues == 1 ? 0u : (blockIdx.x & 7u);
+        tries = 0;
+    }
+    __device__ __forceinline__ bool next(uint32_t& gi) {
+        uint32_t* mask = area + kQueueStride * kQueues;
+        while (tries < kQueues) {
+            const uint32_t lo = (uint32_t)(((uint64_t)q * total) / kQueues);
+            const uint32_t hi = (uint32_t)(((uint64_t)(q + 1) * total) / kQueues);
+            uint32_t g = hi;
+            if (__lane_id() == 0) {
+                bool drained = false;
+                if (ISPH_QUEUE_MASK && kQueues > 1)
+                    drained = (__hip_atomic_load(mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> q) & 1u;
+                if (!drained) {
+                    g = lo + atomicAdd(&area[q * kQueueStride], 1u);
+                    if (ISPH_QUEUE_MASK && kQueues > 1 && g == hi) atomicOr(mask, 1u << q);
+                }
+            }
+            g = __builtin_amdgcn_readfirstlane(g);
+            if (g < hi) {
+                gi = g;
+                return true;
+            }
+            q = (q + 1) % kQueues;
+            ++tries;
+        }
+        return false;
+    }
+};
 
 // ---- launchers (each .hip file implements its own) ----
 void launch_reset_scalars(const Dev& d, hipStream_t s, int mode = 0);
@@ -104,8 +166,8 @@ void launch_gather_vel(const Dev& d, hipStream_t s);
 
 void launch_karras(const Dev& d, hipStream_t s);
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s);
-void launch_groups(const Dev& d, const StepParams& p, hipStream_t s);
-uint32_t walk_chunk_slots(const Dev& d, const StepParams& p);  // leading chunk slots of the group list
+void launch_groups(const Dev& d, const StepParams& p, uint32_t maxc, uint2* out, uint32_t* total, hipStream_t s);
+void launch_exclusive_scan(uint32_t* data, uint32_t count, hipStream_t s);
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s);
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s);

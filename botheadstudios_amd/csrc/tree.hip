@@ -243,13 +243,13 @@ __global__ __launch_bounds__(1024) void k_upsweep_cross(const uint32_t* __restri
     }
 }
 
-// Walk groups: the highest nodes holding <= kGroupMax particles whose box diagonal is at most `extent`
-// times the largest smoothing length of the scene (a leaf always qualifies). Both conditions only get
-// easier down the tree, so these nodes partition the leaves into spatially compact, Morton-contiguous
-// particle runs (consecutive-index groups can straddle a Morton seam and span distant regions).
+// Walk groups: the highest nodes holding <= `maxc` particles whose box diagonal is at most `extent` times
+// the largest smoothing length of the scene (a leaf always qualifies). Both conditions only get easier down
+// the tree, so these nodes partition the leaves into spatially compact, Morton-contiguous particle runs
+// (runs of consecutive indices can straddle a Morton seam and span distant regions).
 __device__ __forceinline__ bool group_ok(uint32_t v, uint32_t n, uint32_t K, uint32_t L, const uint2* nrange,
-                                         const float4* nB, const float4* nC, float lim2, uint32_t& start,
-                                         uint32_t& count) {
+                                         const float4* nB, const float4* nC, float lim2, uint32_t maxc,
+                                         uint32_t& start, uint32_t& count) {
     uint32_t first, last;
     bool leaf = v >= L - 1;
     if (leaf) {
@@ -263,71 +263,60 @@ __device__ __forceinline__ bool group_ok(uint32_t v, uint32_t n, uint32_t K, uin
     uint32_t end = min((last + 1) * K, n);
     count = end - start;
     if (leaf) return true;
-    if (count > kGroupMax) return false;
+    if (count > maxc) return false;
     float4 b = nB[v], c = nC[v];
     float ex = c.x - b.x, ey = c.y - b.y, ez = c.z - b.z;
     return ex * ex + ey * ey + ez * ez <= lim2;
 }
 
-// Walk groups, in two parts. First, every run of 64 consecutive sorted particles whose box diagonal is
-// within the limit is one group, stored at its chunk index (full waves, Morton order). Then the chunks that
-// straddle a Morton seam are covered by their intersections with the tree groups above (at most two per
-// tree group), appended after the chunks. The walk kernels skip empty chunk slots.
-__global__ __launch_bounds__(256) void k_chunk_groups(const float4* __restrict__ spos, uint32_t n,
-                                                      const float4* __restrict__ nB, float extent,
-                                                      uint2* __restrict__ groups, uint32_t* __restrict__ gflag) {
-    const uint32_t chunk = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const uint32_t nchunks = (n + kGroupMax - 1) / kGroupMax;
-    if (chunk >= nchunks) return;  // whole wave
-    const uint32_t a = chunk * kGroupMax;
-    const uint32_t cnt = min(kGroupMax, n - a);
-    const uint32_t lane = threadIdx.x % kWave;
-    const float4 p = spos[a + min(lane, cnt - 1)];
-    float ex = wave_max(p.x) - wave_min(p.x), ey = wave_max(p.y) - wave_min(p.y), ez = wave_max(p.z) - wave_min(p.z);
-    const float lim = extent * nB[0].w;
-    const bool ok = cnt <= 1 || (extent > 0.0f && ex * ex + ey * ey + ez * ez <= lim * lim);
-    if (lane == 0) {
-        groups[chunk] = make_uint2(a, ok ? cnt : 0u);
-        gflag[chunk] = ok ? 0u : 1u;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_groups(uint32_t n, uint32_t K, uint32_t L, const uint32_t* __restrict__ parent,
-                                                const uint2* __restrict__ nrange, const float4* __restrict__ nB,
-                                                const float4* __restrict__ nC, float extent, uint2* __restrict__ groups,
-                                                const uint32_t* __restrict__ gflag, uint32_t nchunks, uint32_t* scal) {
+// Mark each group at its first particle (gmark[start] = count); gmark is all-zero on entry.
+__global__ __launch_bounds__(256) void k_group_mark(uint32_t n, uint32_t K, uint32_t L, const uint32_t* __restrict__ parent,
+                                                    const uint2* __restrict__ nrange, const float4* __restrict__ nB,
+                                                    const float4* __restrict__ nC, float extent, uint32_t maxc,
+                                                    uint32_t* __restrict__ gmark) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 2 * L - 1) return;
     const float lim = extent * nB[0].w;  // the root's box carries the scene's largest h
     const float lim2 = lim * lim;
     uint32_t start, count, ps, pc;
-    if (!group_ok(v, n, K, L, nrange, nB, nC, lim2, start, count)) return;
+    if (!group_ok(v, n, K, L, nrange, nB, nC, lim2, maxc, start, count)) return;
     uint32_t p = parent[v];
-    if (p != kNoParent && group_ok(p, n, K, L, nrange, nB, nC, lim2, ps, pc)) return;
-    if (nchunks == 0) {  // chunk groups disabled: the tree groups are the walk groups
-        groups[atomicAdd(&scal[8], 1u)] = make_uint2(start, count);
-        return;
-    }
-    const uint32_t end = start + count;
-    for (uint32_t c = start / kGroupMax; c * kGroupMax < end; ++c) {
-        if (!gflag[c]) continue;
-        uint32_t s = max(start, c * kGroupMax), e = min(end, (c + 1) * kGroupMax);
-        groups[nchunks + atomicAdd(&scal[8], 1u)] = make_uint2(s, e - s);
-    }
+    if (p != kNoParent && group_ok(p, n, K, L, nrange, nB, nC, lim2, maxc, ps, pc)) return;
+    gmark[start] = count;
 }
 
-uint32_t walk_chunk_slots(const Dev& d, const StepParams& p) {
-    return p.chunk_extent > 0.0f ? (d.n + kGroupMax - 1) / kGroupMax : 0u;
+// Compaction of the marks in particle (Morton) order: per 64-particle block count, exclusive scan of the
+// counts, emit. The emit pass clears the marks for the next build and stores the group total.
+__global__ __launch_bounds__(256) void k_group_count(uint32_t n, const uint32_t* __restrict__ gmark,
+                                                     uint32_t* __restrict__ bcnt) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t b = __ballot(i < n && gmark[i] != 0u);
+    if (lane_id() == 0 && i / kWave < (n + kWave - 1) / kWave) bcnt[i / kWave] = (uint32_t)__popcll(b);
+}
+__global__ __launch_bounds__(256) void k_group_emit(uint32_t n, uint32_t* __restrict__ gmark,
+                                                    const uint32_t* __restrict__ boff, uint2* __restrict__ groups,
+                                                    uint32_t* __restrict__ total) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c = i < n ? gmark[i] : 0u;
+    uint64_t b = __ballot(c != 0u);
+    uint32_t nb = (n + kWave - 1) / kWave;
+    if (i / kWave >= nb) return;  // whole wave
+    uint32_t off = boff[i / kWave];
+    if (c) {
+        groups[off + prefix_count(b)] = make_uint2(i, c);
+        gmark[i] = 0u;
+    }
+    if (i / kWave == nb - 1 && lane_id() == 0) *total = off + (uint32_t)__popcll(b);
 }
 
-void launch_groups(const Dev& d, const StepParams& p, hipStream_t s) {
-    uint32_t nchunks = walk_chunk_slots(d, p);
-    if (nchunks)
-        hipLaunchKernelGGL(k_chunk_groups, dim3((nchunks + 3) / 4), dim3(256), 0, s, d.spos, d.n, d.nB,
-                           p.chunk_extent, d.groups, d.gflag);
+void launch_groups(const Dev& d, const StepParams& p, uint32_t maxc, uint2* out, uint32_t* total, hipStream_t s) {
     uint32_t nodes = 2 * d.L - 1;
-    hipLaunchKernelGGL(k_groups, dim3((nodes + 255) / 256), dim3(256), 0, s, d.n, d.K, d.L, d.parent, d.nrange, d.nB,
-                       d.nC, p.group_extent, d.groups, d.gflag, nchunks, d.scal);
+    uint32_t nb = (d.n + kWave - 1) / kWave;
+    hipLaunchKernelGGL(k_group_mark, dim3((nodes + 255) / 256), dim3(256), 0, s, d.n, d.K, d.L, d.parent, d.nrange, d.nB,
+                       d.nC, p.group_extent, maxc, d.gflag);
+    hipLaunchKernelGGL(k_group_count, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.gflag, d.gscan);
+    launch_exclusive_scan(d.gscan, nb, s);
+    hipLaunchKernelGGL(k_group_emit, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.gflag, d.gscan, out, total);
 }
 
 void launch_karras(const Dev& d, hipStream_t s) {

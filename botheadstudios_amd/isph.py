@@ -35,7 +35,7 @@ _SIGS = {
     "isph_set_av": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float]),
     "isph_set_gravity": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, ctypes.c_int]),
     "isph_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
-    "isph_set_walk": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float]),
+    "isph_set_walk": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int, ctypes.c_int]),
     "isph_set_reuse": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_encode_kdk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "isph_step_cfl": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32]),
@@ -137,8 +137,9 @@ class GpuSph:
     def set_gravity(self, mode=GRAVITY_TREE, theta=0.5, leaf_size=1, quadrupole=False):
         self._check(self._lib.isph_set_gravity(self._h, int(mode), float(theta), int(leaf_size), int(bool(quadrupole))))
 
-    def set_walk(self, group_extent=4.0, chunk_extent=0.0):
-        self._check(self._lib.isph_set_walk(self._h, float(group_extent), float(chunk_extent)))
+    def set_walk(self, group_extent=4.0, grav_particles_per_lane=2, sph_expand=4):
+        self._check(self._lib.isph_set_walk(self._h, float(group_extent), int(grav_particles_per_lane),
+                                            int(sph_expand)))
 
     def set_precision(self, mode):
         self._check(self._lib.isph_set_precision(self._h, int(mode)))

@@ -104,11 +104,11 @@ int isph_set_av(isph_t* h, float alpha, float beta);
 /* Gravity configuration (the reference hard-codes THETA=0.5, BUCKET_K=1: gpu_gravity.rs:55,62).
  * leaf_size in [1, 64]; quadrupole != 0 adds the node quadrupole to accepted cells. */
 int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int quadrupole);
-/* Shape of the wave-cooperative tree walks (no reference counterpart; the autotuner's knob): particles are
- * walked in groups of <= 64 (one per lane). A run of 64 consecutive Morton-sorted particles is one group
- * when its box diagonal is <= chunk_extent * (largest h) (0 disables such runs); everything else is
- * covered by the largest tree nodes of <= 64 particles with diagonal <= group_extent * (largest h). */
-int isph_set_walk(isph_t* h, float group_extent, float chunk_extent);
+/* Shape of the wave-cooperative tree walks (no reference counterpart; the autotuner's knobs). Particles are
+ * walked in groups: the largest tree nodes whose box diagonal is <= group_extent * (largest h) and that
+ * hold <= 64 particles (SPH walks, one per lane) or <= 64 * grav_particles_per_lane (gravity walk, 1 or 2
+ * per lane). The SPH walks take a kept subtree of <= sph_expand particles whole as candidates. */
+int isph_set_walk(isph_t* h, float group_extent, int grav_particles_per_lane, int sph_expand);
 int isph_set_precision(isph_t* h, int mode);
 /* Reuse the tree, gravity and density of the previous evaluation when positions have not moved since
  * (eval 2 of step n and eval 1 of step n+1 see identical positions). Results are bit-identical. */
