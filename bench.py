@@ -34,16 +34,20 @@ def parse():
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--precision", choices=["fp32", "mixed"], default="fp32")
     ap.add_argument("--reuse", action="store_true", help="reuse tree/gravity/density between evals at equal positions")
+    ap.add_argument("--gravity", choices=["tree", "direct"], default="tree")
     ap.add_argument("--quadrupole", action="store_true")
     ap.add_argument("--leaf", type=int, default=1)
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--group-extent", type=float, default=4.0)
-    ap.add_argument("--ppl", type=int, default=2, help="gravity walk particles per lane")
+    ap.add_argument("--ppl", type=int, default=1, help="gravity walk particles per lane")
     ap.add_argument("--expand", type=int, default=4, help="SPH walks take subtrees of <= this many particles whole")
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-energy", action="store_true", help="skip the O(N^2) energy diagnostics")
-    ap.add_argument("--compare-mixed", action="store_true", help="also run the mixed-precision form")
+    ap.add_argument("--variants", default="mixed,reuse",
+                    help="comma list of extra timed runs of the same workload: mixed (precision form), reuse "
+                         "(tree/gravity/density carried from the step's second evaluation to the next step's "
+                         "first, bit-identical results); empty for none")
     return ap.parse_args()
 
 
@@ -74,6 +78,7 @@ def cpu_baseline(sc):
     HydroBody (hydrostatic.rs:338-446, octree gravity theta 0.5) on the same scene, 1 thread."""
     from botheadstudios_amd import scene
     from oracle import oracle
+    oracle.set_threads(1)
     pos, vel, mass, u, h, mat = scene.f64_state(sc.particles)
     hb = oracle.Hydro(pos, vel, mass, u, h, mat, sc.eos64, sc.softening)
     t0 = time.perf_counter()
@@ -86,17 +91,18 @@ def cpu_baseline(sc):
                       f"{dt:.1f} s on 1 thread of {lscpu_model()}"}
 
 
-def run_gpu(sc, args, precision, steps, warmup):
+def run_gpu(sc, args, precision, steps, warmup, reuse):
     from botheadstudios_amd import isph
     rec = sc.particles
     n = len(rec)
     s = isph.GpuSph(n)
     s.upload(rec, sc.eos, sc.softening)
     s.set_eos_f64(sc.eos64)
-    s.set_gravity(isph.GRAVITY_TREE, args.theta, args.leaf, args.quadrupole)
+    s.set_gravity(isph.GRAVITY_TREE if args.gravity == "tree" else isph.GRAVITY_DIRECT, args.theta, args.leaf,
+                  args.quadrupole)
     s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
     s.set_walk(args.group_extent, args.ppl, args.expand)
-    s.set_reuse(args.reuse)
+    s.set_reuse(reuse)
     s.step_cfl(args.cfl, warmup)
     s.synchronize()
     d0 = None if args.no_energy else s.diagnostics(with_pe=True)
@@ -135,7 +141,7 @@ def main():
 
     sc = scene.two_body_1m() if args.n == 1048576 else scene.two_body(args.n)
     n = len(sc.particles)
-    r = run_gpu(sc, args, args.precision, args.steps, args.warmup)
+    r = run_gpu(sc, args, args.precision, args.steps, args.warmup, args.reuse)
     st = r["stats"]
     launches = max(st["gravity_launches"], 1.0)
     flops = st["gravity_flops"]
@@ -156,9 +162,10 @@ def main():
         "dtype": "f32" if args.precision == "fp32" else "f32-pairs/f64-state",
         "data": "synthetic",
         "config": {"workload": f"two-body giant impact, N={n} ({sc.n_target} target + {sc.n_impactor} impactor), "
-                               f"LBVH Barnes-Hut theta={args.theta}, leaf={args.leaf}, "
-                               f"{'quadrupole' if args.quadrupole else 'monopole'}, SPH 48 neighbours, "
-                               f"Tillotson granite/iron, KDK at CFL {args.cfl}",
+                               + (f"LBVH Barnes-Hut theta={args.theta}, leaf={args.leaf}, "
+                                  f"{'quadrupole' if args.quadrupole else 'monopole'}, "
+                                  if args.gravity == "tree" else "direct-sum gravity, ")
+                               + f"SPH 48 neighbours, Tillotson granite/iron, KDK at CFL {args.cfl}",
                    "n": n, "precision": args.precision, "reuse": bool(args.reuse)},
         "roofline": {"bound": "mfma", "engine": "valu-fp32", "kernel": "k_gravity_tree",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -175,10 +182,15 @@ def main():
     if "dE_rel" in r:
         line["dE_rel"] = r["dE_rel"]
         line["dL_rel"] = r["dL_rel"]
-    if args.compare_mixed and args.precision == "fp32":
-        rm = run_gpu(sc, args, "mixed", args.steps, args.warmup)
-        line["mixed"] = {"value": n * args.steps / rm["elapsed"], "ms_per_step": rm["elapsed"] * 1e3 / args.steps,
-                         "dE_rel": rm.get("dE_rel"), "dL_rel": rm.get("dL_rel")}
+    for v in [x for x in args.variants.split(",") if x]:
+        if v not in ("mixed", "reuse"):
+            sys.exit(f"unknown variant {v!r}")
+        prec = "mixed" if v == "mixed" else args.precision
+        rv = run_gpu(sc, args, prec, args.steps, args.warmup, args.reuse or v == "reuse")
+        line[v] = {"value": n * args.steps / rv["elapsed"], "ms_per_step": rv["elapsed"] * 1e3 / args.steps,
+                   "ratio_to_value": r["elapsed"] / rv["elapsed"],
+                   "phases_ms_per_step": {k: x / args.steps for k, x in rv["phases"].items()},
+                   "dE_rel": rv.get("dE_rel"), "dL_rel": rv.get("dL_rel")}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)

@@ -15,7 +15,7 @@ constexpr int kWave = 64;
 
 // Algorithmic flop constants per counted interaction (the roofline numerator).
 constexpr double kFlopsMonopole = 20.0;  // d(3) r2(6) rsqrt(1) r^-3(3) m*r^-3(1) accumulate(6)
-constexpr double kFlopsQuadrupole = 48.0;// monopole + Q.d(15) d.Q.d(5) quadrupole terms(8)
+constexpr double kFlopsQuadrupole = 51.0;// d(3) r2(6) rsqrt(1) inv2(1) w(2) Q.d(15) tq(7) s(2) terms(8) acc(6)
 constexpr double kFlopsDensityPair = 20.0;
 constexpr double kFlopsForcePair = 75.0;
 

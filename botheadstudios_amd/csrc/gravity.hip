@@ -9,23 +9,19 @@
 // cells and leaf particles are appended to LDS lists with ballot/mbcnt compaction and evaluated 64 at a
 // time with LDS broadcast reads (P interactions per read).
 // The traversal stack lives in LDS; up to 64 nodes are popped per round (one per lane), which leaves at
-// most ~64 pending nodes per tree level (for compact groups the deepest stack measured at N = 1M is ~520
-// entries); the pop width narrows to depth-first near capacity, which bounds the stack by
-// capacity - margin + tree depth.
+// most ~64 pending nodes per tree level (unrestricted, the deepest stack at N = 1M reaches ~540 entries).
+// The pop width narrows towards depth-first near capacity, which bounds the stack by capacity - margin +
+// tree depth; a 512-entry stack keeps the workgroup's LDS small enough for 6 resident waves per SIMD.
 #include "state.h"
 
 namespace isph {
 
 #ifndef ISPH_GRAVITY_STACK
-#define ISPH_GRAVITY_STACK 1024
+#define ISPH_GRAVITY_STACK 512
 #endif
 constexpr int kStackCap = ISPH_GRAVITY_STACK;
 constexpr int kStackMargin = 128;  // > maximum Karras depth (30 code bits + 32 index tie-break bits)
 constexpr int kBuf = 128;
-
-struct Acc3 {
-    float x, y, z;
-};
 
 template <int P, bool QUAD, bool COMP>
 __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ spos, uint32_t n, uint32_t K,
@@ -33,7 +29,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                                                      const float4* __restrict__ nQ0, const float2* __restrict__ nQ1,
                                                      float soft2, float4* __restrict__ sgrav,
                                                      unsigned long long* counters, const uint2* __restrict__ groups,
-                                                     uint32_t* scal) {
+                                                     uint32_t* scal, uint32_t count_slot) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 cA[kBuf];
     __shared__ float4 cQ0[QUAD ? kBuf : 1];
@@ -45,7 +41,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
     const uint32_t leaf0 = L - 1;
     unsigned long long tot_cell = 0, tot_part = 0, rounds = 0, popped = 0;
     int max_sp = 1;
-    WalkQueue queue(scal + kQueueGrav, scal[kGroupsGrav]);
+    WalkQueue queue(scal + kQueueGrav, scal[count_slot]);
     uint32_t gi;
     while (queue.next(gi)) {
         const uint2 grp = groups[gi];
@@ -107,15 +103,16 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                     float inv2 = inv * inv;
                     float w = c.w * inv * inv2;
                     if (QUAD) {
+                        // g = G M / r^3 [d (1 + 2.5 (d.Qhat.d) / r^4) - Qhat.d / r^2], Qhat = Q / M; every
+                        // intermediate stays within f32 range in SI units.
                         float qx = q0.x * dx + q0.w * dy + q1.x * dz;
                         float qy = q0.w * dx + q0.y * dy + q1.y * dz;
                         float qz = q1.x * dx + q1.y * dy + q0.z * dz;
-                        float dqd = qx * dx + qy * dy + qz * dz;
-                        float inv5 = inv * inv2 * inv2;
-                        float s = w + 2.5f * dqd * inv5 * inv2;
-                        bx[k] += dx * s - qx * inv5;
-                        by[k] += dy * s - qy * inv5;
-                        bz[k] += dz * s - qz * inv5;
+                        float tq = (qx * dx + qy * dy + qz * dz) * inv2 * inv2;
+                        float s = 1.0f + 2.5f * tq;
+                        bx[k] += w * (dx * s - qx * inv2);
+                        by[k] += w * (dy * s - qy * inv2);
+                        bz[k] += w * (dz * s - qz * inv2);
                     } else {
                         bx[k] += dx * w;
                         by[k] += dy * w;
@@ -239,8 +236,11 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                     const uint64_t mp = __ballot(present);
                     if (!mp) break;
                     if (present) {
+                        // Two explicit stores: a conditional on the float4 lvalues would select between a
+                        // register value and a global address, and forces A into private memory.
                         int pos = np + prefix_count(mp);
-                        pA[pos] = K == 1 ? A : spos[start + kk];
+                        if (K == 1) pA[pos] = A;
+                        else pA[pos] = spos[start + kk];
                         pI[pos] = start + kk;
                     }
                     np += __popcll(mp);
@@ -344,9 +344,12 @@ __global__ void k_scatter_gravity(uint32_t n, const uint32_t* __restrict__ order
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t waves = std::min<uint32_t>(kWalkWaves, d.n);
+    // One particle per lane walks the SPH group list (groups of <= 64); two per lane use the gravity list.
+    const bool own = p.grav_ppl == 2;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(waves), dim3(kWave), 0, s, d.spos, d.n, d.K, d.L, d.nG, d.nQ0, d.nQ1, p.soft2,
-                           d.sgrav, d.counters, d.ggroups, d.scal);
+                           d.sgrav, d.counters, own ? d.ggroups : d.groups, d.scal,
+                           (uint32_t)(own ? kGroupsGrav : kGroupsSph));
     };
     const int sel = (p.grav_ppl == 2 ? 4 : 0) | (p.quad ? 2 : 0) | (p.mixed ? 1 : 0);
     switch (sel) {

@@ -242,7 +242,8 @@ void build_tree(isph_ctx* c, const Dev& d) {
     Phase ph(c, ISPH_PHASE_UPSWEEP);
     launch_upsweep(d, c->p, c->stream);
     launch_groups(d, c->p, kGroupMax, d.groups, d.scal + kGroupsSph, c->stream);
-    launch_groups(d, c->p, kGroupMax * (uint32_t)c->p.grav_ppl, d.ggroups, d.scal + kGroupsGrav, c->stream);
+    if (c->p.grav_ppl == 2)
+        launch_groups(d, c->p, 2 * kGroupMax, d.ggroups, d.scal + kGroupsGrav, c->stream);
 }
 
 void run_gravity(isph_ctx* c, const Dev& d, int mode) {
@@ -636,6 +637,7 @@ int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass
         Dev g = h->d;
         g.n = (uint32_t)n;
         g.L = (g.n + g.K - 1) / g.K;
+        g.sh = nullptr;  // no smoothing lengths: the upsweep stores hmax = 0
         StepParams saved = h->p;
         h->p.soft2 = (float)(softening * softening);
         try {

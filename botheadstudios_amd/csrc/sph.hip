@@ -14,8 +14,11 @@
 
 namespace isph {
 
-constexpr int kStackCap = 1024;
-constexpr int kStackMargin = 128;
+#ifndef ISPH_SPH_STACK
+#define ISPH_SPH_STACK 512
+#endif
+constexpr int kStackCap = ISPH_SPH_STACK;  // LDS traversal stack (entries); the pop width narrows near it
+constexpr int kStackMargin = 128;          // > maximum Karras depth
 constexpr int kBuf = 128;
 
 // AoS record, identical layout to isph_particle (gpu_sph.rs:27-37).
@@ -133,42 +136,32 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
     }
 }
 
-// Per-particle EOS + Courant signal epilogue shared by the density and the EOS-only kernels.
-__device__ __forceinline__ void eos_epilogue(bool valid, uint32_t t, double rho, float h, float4 v, uint32_t m,
-                                             const Till* __restrict__ eos, float* srho, float* spr, float* scs,
-                                             double* srho64, double* spr64, double* scs64, uint32_t* scal) {
-    float sig = INFINITY;
-    if (valid) {
-        const Till& T = eos[m];
-        double p, c2;
-        till_p_c2(T, rho, (double)v.w, p, c2);
-        double r = rho;
-        double c = sqrt(c2);
-        srho[t] = (float)rho;
-        spr[t] = (float)(p / (r * r));
-        scs[t] = (float)c;
-        if (srho64) {
-            srho64[t] = rho;
-            spr64[t] = p / (r * r);
-            scs64[t] = c;
-        }
-        float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-        sig = h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
+// Per-particle EOS: pressure / rho^2 and sound speed from the f64 Tillotson (eos.rs:88-96); returns the
+// particle's Courant signal h / max(c + |v|, 1).
+__device__ __forceinline__ float eos_point(uint32_t t, double rho, float h, float4 v, uint32_t m,
+                                           const Till* __restrict__ eos, float* spr, float* scs, double* spr64,
+                                           double* scs64) {
+    const Till& T = eos[m];
+    double p, c2;
+    till_p_c2(T, rho, (double)v.w, p, c2);
+    double c = sqrt(c2);
+    spr[t] = (float)(p / (rho * rho));
+    scs[t] = (float)c;
+    if (spr64) {
+        spr64[t] = p / (rho * rho);
+        scs64[t] = c;
     }
-    sig = wave_min(sig);
-    if (threadIdx.x == 0 && sig < INFINITY) atomicMin(&scal[6], __float_as_uint(sig));
+    float speed = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    return h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
 }
 
 template <bool F64>
 __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
-                                                const float4* __restrict__ svel, const uint32_t* __restrict__ smat,
                                                 uint32_t n, uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
                                                 const float4* __restrict__ nB,
                                                 const float4* __restrict__ nC, const uint32_t* __restrict__ order,
-                                                const Till* __restrict__ eos, float* __restrict__ srho,
-                                                float* __restrict__ spr, float* __restrict__ scs,
-                                                float* __restrict__ rho_orig, double* __restrict__ srho64,
-                                                double* __restrict__ spr64, double* __restrict__ scs64, uint32_t* scal,
+                                                float* __restrict__ srho, float* __restrict__ rho_orig,
+                                                double* __restrict__ srho64, uint32_t* scal,
                                                 unsigned long long* counters, const uint2* __restrict__ groups,
                                                 const uint2* __restrict__ nrange, uint32_t expand) {
     __shared__ uint32_t stk[kStackCap];
@@ -244,23 +237,41 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
     } else {
         rho_i = valid ? (double)rho_orig[order[t]] : 1.0;
     }
-    eos_epilogue(valid, t, rho_i, hi, svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scs64, scal);
+    // The per-particle EOS runs in its own pass (k_eos): its f64 Tillotson evaluations would otherwise set
+    // this walk's register budget and halve the resident waves.
+    if (valid) {
+        srho[t] = (float)rho_i;
+        if (F64) srho64[t] = rho_i;
+    }
     tested_all += tested * grp.y;
     }
     if (lane == 0 && counters) atomicAdd(&counters[3], tested_all);
 }
 
-// EOS + signal only (the density of the previous evaluation is reused: positions have not moved).
-__global__ __launch_bounds__(64) void k_eos(const float* __restrict__ sh, const float4* __restrict__ svel,
-                                            const uint32_t* __restrict__ smat, uint32_t n, const Till* __restrict__ eos,
-                                            float* __restrict__ srho, float* __restrict__ spr, float* __restrict__ scs,
-                                            double* __restrict__ srho64, double* __restrict__ spr64,
-                                            double* __restrict__ scs64, uint32_t* scal) {
-    const uint32_t t = blockIdx.x * kWave + threadIdx.x;
-    const bool valid = t < n;
-    const uint32_t tc = valid ? t : n - 1;
-    const double rho = srho64 ? srho64[tc] : (double)srho[tc];
-    eos_epilogue(valid, t, rho, sh[tc], svel[tc], smat[tc], eos, srho, spr, scs, srho64, spr64, scs64, scal);
+// Per-particle EOS and the Courant signal minimum, after the density walk (or on the reused density when
+// positions have not moved). Grid-stride with a block-level minimum: one atomic per block, since
+// same-address atomics from every wave would serialise.
+constexpr int kEosThreads = 256;
+constexpr uint32_t kEosBlocks = 1024;
+__global__ __launch_bounds__(kEosThreads) void k_eos(const float* __restrict__ sh, const float4* __restrict__ svel,
+                                                     const uint32_t* __restrict__ smat, uint32_t n,
+                                                     const Till* __restrict__ eos, const float* __restrict__ srho,
+                                                     float* __restrict__ spr, float* __restrict__ scs,
+                                                     const double* __restrict__ srho64, double* __restrict__ spr64,
+                                                     double* __restrict__ scs64, uint32_t* scal) {
+    __shared__ float red[kEosThreads / kWave];
+    float sig = INFINITY;
+    for (uint32_t t = blockIdx.x * kEosThreads + threadIdx.x; t < n; t += gridDim.x * kEosThreads) {
+        const double rho = srho64 ? srho64[t] : (double)srho[t];
+        sig = fminf(sig, eos_point(t, rho, sh[t], svel[t], smat[t], eos, spr, scs, spr64, scs64));
+    }
+    sig = wave_min(sig);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = sig;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kEosThreads / kWave; ++w) sig = fminf(sig, red[w]);
+        if (sig < INFINITY) atomicMin(&scal[6], __float_as_uint(sig));
+    }
 }
 
 template <bool F64>
@@ -632,18 +643,19 @@ static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
     if (p.mixed)
-        hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
-                           d.L, d.nLR, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho, d.srho64,
-                           d.spr64, d.scs64, d.scal, d.counters, d.groups, d.nrange, (uint32_t)p.sph_expand);
+        hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR, d.nB,
+                           d.nC, d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
+                           (uint32_t)p.sph_expand);
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.n, d.K,
-                           d.L, d.nLR, d.nB, d.nC, d.vals, d.eos, d.srho, d.spr, d.scs, d.rho,
-                           (double*)nullptr, (double*)nullptr, (double*)nullptr, d.scal, d.counters, d.groups,
+        hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR,
+                           d.nB, d.nC, d.vals, d.srho, d.rho, (double*)nullptr, d.scal, d.counters, d.groups,
                            d.nrange, (uint32_t)p.sph_expand);
+    launch_eos(d, s);
 }
 void launch_eos(const Dev& d, hipStream_t s) {
-    hipLaunchKernelGGL(k_eos, dim3(cdiv(d.n, kWave)), dim3(kWave), 0, s, d.sh, d.svel, d.smat, d.n, d.eos, d.srho,
-                       d.spr, d.scs, d.srho64, d.spr64, d.scs64, d.scal);
+    const uint32_t blocks = std::min<uint32_t>(kEosBlocks, cdiv(d.n, kEosThreads));
+    hipLaunchKernelGGL(k_eos, dim3(blocks), dim3(kEosThreads), 0, s, d.sh, d.svel, d.smat, d.n, d.eos, d.srho, d.spr,
+                       d.scs, (const double*)d.srho64, d.spr64, d.scs64, d.scal);
 }
 void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);

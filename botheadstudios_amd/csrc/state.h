@@ -91,7 +91,7 @@ struct StepParams {
     int quad = 0;
     int mixed = 0;
     float group_extent = 4.0f;  // walk groups: box diagonal <= this many times the largest h
-    int grav_ppl = 2;           // gravity walk: particles per lane (1 or 2)
+    int grav_ppl = 1;           // gravity walk: particles per lane (1 or 2)
     int sph_expand = 4;         // SPH walks take subtrees of <= this many particles whole as candidates
 };
 

@@ -81,24 +81,27 @@ __device__ __forceinline__ float mac_radius(float3 lo, float3 hi, float3 com, fl
     return size * inv_theta + delta;
 }
 
-__device__ __forceinline__ void add_shifted_quad(float4& q0, float2& q1, float4 cq0, float2 cq1, float m, float dx,
+// Quadrupoles are stored per unit mass, Qhat = Q / M with Q_ij = sum m (3 d_i d_j - |d|^2 delta_ij) about the
+// node COM: in SI units Q itself (~M R^2 ~ 1e38 for a planet) would overflow f32, while Qhat ~ R^2 does not.
+// Adds f * (Qhat_child + 3 d d - |d|^2 I) for a child (or particle, Qhat = 0) of mass fraction f at offset
+// d from the parent COM.
+__device__ __forceinline__ void add_shifted_quad(float4& q0, float2& q1, float4 cq0, float2 cq1, float f, float dx,
                                                  float dy, float dz) {
-    // Q_ij about the parent COM = Q_child + m (3 d_i d_j - |d|^2 delta_ij), d = child com - parent com.
     float r2 = dx * dx + dy * dy + dz * dz;
-    q0.x += cq0.x + m * (3.0f * dx * dx - r2);
-    q0.y += cq0.y + m * (3.0f * dy * dy - r2);
-    q0.z += cq0.z + m * (3.0f * dz * dz - r2);
-    q0.w += cq0.w + m * (3.0f * dx * dy);
-    q1.x += cq1.x + m * (3.0f * dx * dz);
-    q1.y += cq1.y + m * (3.0f * dy * dz);
+    q0.x += f * (cq0.x + 3.0f * dx * dx - r2);
+    q0.y += f * (cq0.y + 3.0f * dy * dy - r2);
+    q0.z += f * (cq0.z + 3.0f * dz * dz - r2);
+    q0.w += f * (cq0.w + 3.0f * dx * dy);
+    q1.x += f * (cq1.x + 3.0f * dx * dz);
+    q1.y += f * (cq1.y + 3.0f * dy * dz);
 }
 
 struct NodeMom {
     float4 a;   // com.xyz, mass
     float4 b;   // bmin.xyz, max h
     float4 c;   // bmax.xyz, MAC radius
-    float4 q0;  // quadrupole xx, yy, zz, xy
-    float2 q1;  // quadrupole xz, yz
+    float4 q0;  // quadrupole per unit mass: xx, yy, zz, xy
+    float2 q1;  // quadrupole per unit mass: xz, yz
 };
 
 struct TreeView {
@@ -148,9 +151,10 @@ __device__ __forceinline__ NodeMom merge_children(const TreeView& tv, uint32_t n
     m.c = make_float4(nhi.x, nhi.y, nhi.z, mac_radius(nlo, nhi, cm, inv_theta));
     m.q0 = make_float4(0, 0, 0, 0);
     m.q1 = make_float2(0, 0);
-    if (quad) {
-        add_shifted_quad(m.q0, m.q1, tv.nQ0[l], tv.nQ1[l], al.w, al.x - cm.x, al.y - cm.y, al.z - cm.z);
-        add_shifted_quad(m.q0, m.q1, tv.nQ0[r], tv.nQ1[r], ar.w, ar.x - cm.x, ar.y - cm.y, ar.z - cm.z);
+    if (quad && mm > 0.0f) {
+        const float fl = al.w / mm, fr = ar.w / mm;
+        add_shifted_quad(m.q0, m.q1, tv.nQ0[l], tv.nQ1[l], fl, al.x - cm.x, al.y - cm.y, al.z - cm.z);
+        add_shifted_quad(m.q0, m.q1, tv.nQ0[r], tv.nQ1[r], fr, ar.x - cm.x, ar.y - cm.y, ar.z - cm.z);
     }
     return m;
 }
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(kUpChunk) void k_upsweep_chunk(const float4* __rest
         mz += b.z * b.w;
         lo = make_float3(fminf(lo.x, b.x), fminf(lo.y, b.y), fminf(lo.z, b.z));
         hi = make_float3(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z));
-        hmax = fmaxf(hmax, sh[s]);
+        if (sh) hmax = fmaxf(hmax, sh[s]);  // gravity-only builds carry no smoothing lengths
     }
     float3 com;
     if (end - start == 1) com = make_float3(spos[start].x, spos[start].y, spos[start].z);
@@ -194,10 +198,10 @@ __global__ __launch_bounds__(kUpChunk) void k_upsweep_chunk(const float4* __rest
     NodeMom lm;
     lm.q0 = make_float4(0, 0, 0, 0);
     lm.q1 = make_float2(0, 0);
-    if (quad && end - start > 1) {
+    if (quad && end - start > 1 && m > 0.0f) {
         for (uint32_t s = start; s < end; ++s) {
             float4 b = spos[s];
-            add_shifted_quad(lm.q0, lm.q1, make_float4(0, 0, 0, 0), make_float2(0, 0), b.w, b.x - com.x,
+            add_shifted_quad(lm.q0, lm.q1, make_float4(0, 0, 0, 0), make_float2(0, 0), b.w / m, b.x - com.x,
                              b.y - com.y, b.z - com.z);
         }
     }
@@ -276,7 +280,12 @@ __global__ __launch_bounds__(256) void k_group_mark(uint32_t n, uint32_t K, uint
                                                     uint32_t* __restrict__ gmark) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 2 * L - 1) return;
-    const float lim = extent * nB[0].w;  // the root's box carries the scene's largest h
+    // Length unit of the limit: the scene's largest h (carried by the root box), or, for gravity-only
+    // builds without smoothing lengths, the root diagonal / cbrt(n) (about twice the mean spacing).
+    const float4 b0 = nB[0], c0 = nC[0];
+    const float ex0 = c0.x - b0.x, ey0 = c0.y - b0.y, ez0 = c0.z - b0.z;
+    const float spacing = sqrtf(ex0 * ex0 + ey0 * ey0 + ez0 * ez0) / cbrtf((float)n);
+    const float lim = extent * fmaxf(b0.w, spacing);
     const float lim2 = lim * lim;
     uint32_t start, count, ps, pc;
     if (!group_ok(v, n, K, L, nrange, nB, nC, lim2, maxc, start, count)) return;

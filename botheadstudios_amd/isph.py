@@ -137,7 +137,7 @@ class GpuSph:
     def set_gravity(self, mode=GRAVITY_TREE, theta=0.5, leaf_size=1, quadrupole=False):
         self._check(self._lib.isph_set_gravity(self._h, int(mode), float(theta), int(leaf_size), int(bool(quadrupole))))
 
-    def set_walk(self, group_extent=4.0, grav_particles_per_lane=2, sph_expand=4):
+    def set_walk(self, group_extent=4.0, grav_particles_per_lane=1, sph_expand=4):
         self._check(self._lib.isph_set_walk(self._h, float(group_extent), int(grav_particles_per_lane),
                                             int(sph_expand)))
 

@@ -107,7 +107,9 @@ int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int q
 /* Shape of the wave-cooperative tree walks (no reference counterpart; the autotuner's knobs). Particles are
  * walked in groups: the largest tree nodes whose box diagonal is <= group_extent * (largest h) and that
  * hold <= 64 particles (SPH walks, one per lane) or <= 64 * grav_particles_per_lane (gravity walk, 1 or 2
- * per lane). The SPH walks take a kept subtree of <= sph_expand particles whole as candidates. */
+ * per lane). The SPH walks take a kept subtree of <= sph_expand particles whole as candidates.
+ * Defaults: 4.0, 1, 4. Returns ISPH_EINVAL for group_extent <= 0, grav_particles_per_lane not 1 or 2, or
+ * sph_expand outside [1, 64]. */
 int isph_set_walk(isph_t* h, float group_extent, int grav_particles_per_lane, int sph_expand);
 int isph_set_precision(isph_t* h, int mode);
 /* Reuse the tree, gravity and density of the previous evaluation when positions have not moved since

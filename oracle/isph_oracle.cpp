@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <omp.h>
 #include <unordered_map>
 #include <vector>
 
@@ -245,8 +246,11 @@ struct Octree {
         return phi;
     }
 
+    // The per-target loops below run in parallel (OpenMP); each target's sum keeps its serial order, so
+    // the results do not depend on the thread count.
     void accelerations(const V3* pos, const double* mass, V3* acc) const {  // bhtree.rs:129-148
         if (nodes.empty()) {
+#pragma omp parallel for schedule(dynamic, 64)
             for (size_t i = 0; i < n; ++i) {
                 V3 a{0, 0, 0};
                 for (size_t j = 0; j < n; ++j) {
@@ -259,11 +263,13 @@ struct Octree {
             }
             return;
         }
+#pragma omp parallel for schedule(dynamic, 64)
         for (size_t i = 0; i < n; ++i) acc[i] = accel_on(i, 0, pos, mass);
     }
 
     void potentials(const V3* pos, const double* mass, double* phi) const {  // bhtree.rs:192-209
         if (nodes.empty()) {
+#pragma omp parallel for schedule(dynamic, 64)
             for (size_t i = 0; i < n; ++i) {
                 double p = 0.0;
                 for (size_t j = 0; j < n; ++j) {
@@ -275,6 +281,7 @@ struct Octree {
             }
             return;
         }
+#pragma omp parallel for schedule(dynamic, 64)
         for (size_t i = 0; i < n; ++i) phi[i] = potential_on(i, 0, pos, mass);
     }
 };
@@ -347,6 +354,9 @@ struct Grid {
 // ---------------------------------------------------------------------------------------------------
 // HydroBody: hydrostatic.rs:42-56 (state), :338-355 (density), :401-446 (forces + du/dt),
 // :451-465 (KDK step), :472-480 (Courant dt).
+extern "C" void orc_lbvh_accel_f64(const double* pos3, const double* mass, uint64_t n, double soft, double theta,
+                                   double* acc3);
+
 struct Hydro {
     std::vector<V3> pos, vel;
     std::vector<double> mass, u, h, rho;
@@ -356,7 +366,9 @@ struct Hydro {
     double theta = 0.5;    // hydrostatic.rs:405 (BarnesHut::build(.., 0.5, ..))
     double av_alpha = 1.0; // hydrostatic.rs:414-415
     double av_beta = 2.0;
-    int gravity_mode = 0;  // 0 = octree (the reference), 1 = f64 direct sum, 2 = none (SPH part only)
+    // 0 = octree (the reference CPU path), 1 = f64 direct sum, 2 = none (SPH part only), 3 = the reference
+    // GPU tree path (f32 LBVH walk, gpu_gravity.rs accelerations_tree)
+    int gravity_mode = 0;
 
     size_t n() const { return pos.size(); }
     double max_h() const {
@@ -389,6 +401,7 @@ struct Hydro {
         if (gravity_mode == 2) return;
         if (gravity_mode == 1) {
             double s2 = softening * softening;
+#pragma omp parallel for schedule(dynamic, 64)
             for (size_t i = 0; i < N; ++i) {
                 V3 a{0, 0, 0};
                 for (size_t j = 0; j < N; ++j) {
@@ -399,6 +412,10 @@ struct Hydro {
                 }
                 acc[i] = a;
             }
+            return;
+        }
+        if (gravity_mode == 3) {
+            orc_lbvh_accel_f64(&pos[0].x, mass.data(), N, softening, theta, &acc[0].x);
             return;
         }
         Octree bh;
@@ -665,6 +682,7 @@ void orc_lbvh_gravity_f32(const float* sbodies, uint64_t n, uint32_t K, uint32_t
                           const uint32_t* right, const float* com4, const float* bmin3, const float* bmax3,
                           float theta, float soft2, const uint32_t* order, float* acc3) {
     uint32_t leaf0 = L - 1;
+#pragma omp parallel for schedule(dynamic, 64)
     for (uint64_t t = 0; t < n; ++t) {
         const float* pi = sbodies + 4 * t;
         float a[3] = {0, 0, 0};
@@ -712,9 +730,45 @@ void orc_lbvh_gravity_f32(const float* sbodies, uint64_t n, uint32_t K, uint32_t
 }
 
 // Exact softened direct sum, f64: bhtree.rs:133-143 / gpu-bh-verify cpu_direct main.rs:138-158.
+// The reference GPU tree leg end to end (gpu_gravity.rs:330-388 + bh_gravity.wgsl): bodies to f32, bbox,
+// Morton, (code, index) sort, Karras tree (leaf size 1), moments, per-particle walk; f64 out.
+void orc_lbvh_accel_f64(const double* pos3, const double* mass, uint64_t n, double soft, double theta,
+                        double* acc3) {
+    if (n == 0) return;
+    std::vector<float> p(3 * n), sb(4 * n), acc(3 * n);
+    for (uint64_t i = 0; i < 3 * n; ++i) p[i] = (float)pos3[i];
+    float lo[3], hi[3];
+    orc_bbox_f32(p.data(), n, lo, hi);
+    std::vector<uint32_t> codes(n), sc(n), order(n);
+    orc_morton_f32(p.data(), n, lo, hi, codes.data());
+    orc_sort_pairs(codes.data(), n, sc.data(), order.data());
+    const uint32_t L = (uint32_t)n;
+    std::vector<uint32_t> left(std::max<uint32_t>(L - 1, 1)), right(std::max<uint32_t>(L - 1, 1)),
+        parent(2 * L - 1);
+    orc_karras(sc.data(), L, left.data(), right.data(), parent.data());
+    for (uint64_t t = 0; t < n; ++t) {
+        uint32_t o = order[t];
+        sb[4 * t] = p[3 * o];
+        sb[4 * t + 1] = p[3 * o + 1];
+        sb[4 * t + 2] = p[3 * o + 2];
+        sb[4 * t + 3] = (float)mass[o];
+    }
+    std::vector<float> com(4 * (2 * L - 1)), bmin(3 * (2 * L - 1)), bmax(3 * (2 * L - 1));
+    orc_upsweep_f32(sb.data(), n, 1, L, left.data(), right.data(), com.data(), bmin.data(), bmax.data());
+    float s32 = (float)soft;
+    orc_lbvh_gravity_f32(sb.data(), n, 1, L, left.data(), right.data(), com.data(), bmin.data(), bmax.data(),
+                         (float)theta, s32 * s32, order.data(), acc.data());
+    for (uint64_t i = 0; i < 3 * n; ++i) acc3[i] = acc[i];
+}
+
+// Threads for the parallel gravity loops (the CPU baseline is timed on 1).
+void orc_set_threads(int t) { omp_set_num_threads(t > 0 ? t : 1); }
+int orc_get_threads() { return omp_get_max_threads(); }
+
 void orc_direct_gravity_f64(const double* pos3, const double* mass, uint64_t n, double soft, double* acc3) {
     double s2 = soft * soft;
     const V3* pos = reinterpret_cast<const V3*>(pos3);
+#pragma omp parallel for schedule(dynamic, 64)
     for (uint64_t i = 0; i < n; ++i) {
         V3 a{0, 0, 0};
         for (uint64_t j = 0; j < n; ++j) {

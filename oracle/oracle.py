@@ -28,7 +28,10 @@ _SIGS = {
     "orc_upsweep_f32": (None, [_P, _U64, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P]),
     "orc_lbvh_gravity_f32": (None, [_P, _U64, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
                                     ctypes.c_float, ctypes.c_float, _P, _P]),
+    "orc_set_threads": (None, [ctypes.c_int]),
+    "orc_get_threads": (ctypes.c_int, []),
     "orc_direct_gravity_f64": (None, [_P, _P, _U64, _D, _P]),
+    "orc_lbvh_accel_f64": (None, [_P, _P, _U64, _D, _D, _P]),
     "orc_octree_accel": (None, [_P, _P, _U64, _D, _D, _P]),
     "orc_octree_potential": (None, [_P, _P, _U64, _D, _D, _P]),
     "orc_grid_pairs": (_U64, [_P, _U64, _D, _P, _U64]),
@@ -70,6 +73,11 @@ def lib():
             f.argtypes = a
         _lib = l
     return _lib
+
+
+def set_threads(t):
+    """Threads of the per-target gravity loops (results do not depend on it)."""
+    lib().orc_set_threads(int(t))
 
 
 def _p(a):
@@ -159,6 +167,14 @@ def lbvh_gravity_f32(pos, mass, soft, theta=0.5, K=1):
     return acc
 
 
+def lbvh_accel(pos, mass, soft, theta=0.5):
+    """lbvh_gravity_f32 as one C call (f64 in and out), the gravity of the CPU step's "lbvh" mode."""
+    p = c64(pos).reshape(-1, 3)
+    acc = np.zeros_like(p)
+    lib().orc_lbvh_accel_f64(_p(p), _p(c64(mass)), p.shape[0], float(soft), float(theta), _p(acc))
+    return acc
+
+
 # ---- f64 physics ----
 def direct_gravity(pos, mass, soft):
     p = c64(pos).reshape(-1, 3)
@@ -228,7 +244,7 @@ class Hydro:
         self.close()
 
     def configure(self, theta=0.5, av_alpha=1.0, av_beta=2.0, gravity="octree"):
-        mode = {"octree": 0, "direct": 1, "none": 2}[gravity]
+        mode = {"octree": 0, "direct": 1, "none": 2, "lbvh": 3}[gravity]
         lib().orc_hydro_set(self.h, theta, av_alpha, av_beta, mode)
 
     def compute_density(self):

@@ -169,6 +169,56 @@ def test_gravity_tree_at_least_as_accurate_as_reference_walk(theta):
     assert rms_rel(a, exact)[0] <= rms_rel(ref, exact)[0] * 1.05 + 2e-6
 
 
+def _pair_field(pos, mass, quadrupole):
+    with isph.GpuSph(8) as s:
+        s.set_walk(1e-6, 1, 4)  # one-particle walk groups, so the far particle sees the pair as one node
+        s.set_gravity(isph.GRAVITY_TREE, 0.5, 1, quadrupole)
+        return s.gravity_accelerations(pos, mass, 1.0, isph.GRAVITY_TREE)
+
+
+@pytest.mark.parametrize("layout", ["on_axis", "broadside"])
+def test_quadrupole_closed_form_pair(layout):
+    """Two equal masses at distance a either side of the origin, seen from r = 10 a on the x axis. The exact
+    pull is G m [1/(r-a)^2 + 1/(r+a)^2] on axis and 2 G m r / (r^2 + a^2)^1.5 broadside; the monopole
+    misses the (a/r)^2 term (+3 and -1.5 (a/r)^2), the quadrupole leaves O((a/r)^4) (~7e-4, ~2e-4).
+    Masses ~1e24 kg at 1e5 m also check that no f32 intermediate overflows in SI units."""
+    a, r, m, G = 1.0e5, 1.0e6, 3.0e24, 6.674e-11
+    axis = 0 if layout == "on_axis" else 1
+    pos = np.zeros((3, 3))
+    pos[0, axis], pos[1, axis] = -a, a
+    pos[2, 0] = r
+    mass = np.array([m, m, 1.0])
+    exact = G * m * (1 / (r - a) ** 2 + 1 / (r + a) ** 2) if axis == 0 else 2 * G * m * r / (r * r + a * a) ** 1.5
+    mono = -_pair_field(pos, mass, False)[2, 0]
+    quad = -_pair_field(pos, mass, True)[2, 0]
+    e_mono, e_quad = abs(mono - exact) / exact, abs(quad - exact) / exact
+    print(f"{layout}: monopole {e_mono:.2e} quadrupole {e_quad:.2e}")
+    assert e_mono > 1e-2                       # the pair really was taken as one node
+    assert e_quad < (8e-4 if axis == 0 else 3e-4)
+
+
+def test_quadrupole_improves_accuracy_and_opens_to_exact():
+    """On the gpu-bh-verify cloud (N=1500) and on the SI-scale C1 scene, quadrupole cells cut the theta=0.5
+    RMS error against the f64 direct sum at least in half; theta -> 0 is the direct sum."""
+    c1 = scene.two_body_8k()
+    p1 = c1.particles["pos"].astype(np.float64)
+    m1 = c1.particles["mass"].astype(np.float64)
+    for pos, mass, soft in [splitmix_cloud(1500, 0xABCD1234) + (5e3,), (p1, m1, float(c1.softening))]:
+        exact = oracle.direct_gravity(pos, mass, soft)
+        with isph.GpuSph(len(mass)) as s:
+            s.set_gravity(isph.GRAVITY_TREE, 0.5, 1, False)
+            mono = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_TREE)
+            s.set_gravity(isph.GRAVITY_TREE, 0.5, 1, True)
+            quad = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_TREE)
+            s.set_gravity(isph.GRAVITY_TREE, 1e-4, 1, True)
+            opened = s.gravity_accelerations(pos, mass, soft, isph.GRAVITY_TREE)
+        assert np.isfinite(quad).all()
+        e_mono, e_quad = rms_rel(mono, exact)[0], rms_rel(quad, exact)[0]
+        print(f"N={len(mass)}: RMS monopole {e_mono:.2e} quadrupole {e_quad:.2e}")
+        assert e_quad <= 0.5 * e_mono
+        assert rms_rel(opened, exact)[0] < 1e-4
+
+
 # Stated bounds per precision form (max over particles of the floored relative error):
 #   plain FP32 (the reference shader's arithmetic): rho 1e-5, acceleration 1e-4 (RMS 1e-5), du/dt 1e-3 —
 #     the pressure P(rho) = A (rho/rho0 - 1) + ... cancels near rho0, so the f32 density error (~5e-7)
@@ -315,3 +365,89 @@ def test_retired_particles_are_inert(c1):
         out = s.readback_sync()
     assert np.array_equal(out["pos"][:50], rec["pos"][:50])
     assert np.isfinite(out["pos"]).all() and np.isfinite(out["vel"]).all()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_reuse_is_bit_identical(c1, precision):
+    """Carrying the tree, gravity and density from a step's second evaluation into the next step's first
+    (positions are equal there) changes no bit of the trajectory or of the Courant time."""
+    rec = with_random_velocities(c1)
+    outs = []
+    for reuse in (False, True):
+        with _sim(rec, c1.softening) as s:
+            s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
+            s.set_reuse(reuse)
+            s.step_cfl(0.3, 6)
+            outs.append((s.readback_sync(), s.sim_time(), s.diagnostics(with_pe=False)))
+    (a, ta, da), (b, tb, db) = outs
+    for f in ("pos", "vel", "u", "rho", "h"):
+        assert np.array_equal(a[f], b[f]), f
+    assert ta == tb
+    assert da["ke"] == db["ke"] and da["ie"] == db["ie"]
+
+
+_C1_STEPS = 100
+
+
+@pytest.fixture(scope="module")
+def c1_cpu_run(c1):
+    """The f64 CPU step (hydrostatic.rs:451-480) over the C1 scene: 100 KDK steps at its own Courant dt
+    (CFL 0.3, rounded to f32 so that the device integrates with the same dt), with exact (direct-sum)
+    gravity and with the reference GPU tree walk (f32 LBVH, theta 0.5). Returns the dt sequence and the
+    relative energy drift and angular-momentum change of each."""
+    runs = {}
+    pos, vel, mass, u, h, mat = scene.f64_state(c1.particles)
+    ke, ie, pe = oracle.total_energy(pos, vel, mass, u, c1.softening)
+    e0 = ke + ie + pe
+    l0 = oracle.angular_momentum(pos, vel, mass)
+    for grav in ("direct", "lbvh"):
+        hb = hydro_from(c1.particles, EOS64, c1.softening, gravity=grav)
+        hb.compute_density()
+        dts = []
+        for _ in range(_C1_STEPS):
+            dt = float(np.float32(hb.courant_dt(0.3)))
+            dts.append(dt)
+            hb.step(dt)
+        p1, v1, u1, _ = hb.state()
+        ke, ie, pe = oracle.total_energy(p1, v1, mass, u1, c1.softening)
+        l1 = oracle.angular_momentum(p1, v1, mass)
+        runs[grav] = (np.array(dts), (ke + ie + pe - e0) / abs(e0), (l1 - l0) / np.linalg.norm(l0))
+    return runs
+
+
+@pytest.mark.timeout(400)
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("gravity", ["direct", "tree_quadrupole", "tree"])
+def test_c1_energy_and_momentum_drift_match_cpu_run(c1, c1_cpu_run, precision, gravity):
+    """Run-level parity over the 100-step C1 run (7168 + 1024 particles, CFL 0.3, the CPU path's dt sequence
+    fed to the device). Direct gravity and LBVH + quadrupole: |dE_gpu - dE_cpu| / |E0| and
+    |dL_gpu - dL_cpu| / |L0| <= 1e-4 against the f64 CPU step with exact gravity. LBVH monopole (theta 0.5,
+    the reference's GPU tree): energy within 1e-4 of the CPU step driven by the reference's own LBVH walk,
+    and an angular-momentum change no larger than that run's (the monopole walk does not conserve L to
+    1e-4 in the reference either: ~1.1e-3 over this run)."""
+    dts, de_cpu, dl_cpu = c1_cpu_run["direct"]
+    _, de_lbvh, dl_lbvh = c1_cpu_run["lbvh"]
+    with _sim(c1.particles, c1.softening) as s:
+        s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
+        if gravity == "direct":
+            s.set_gravity(isph.GRAVITY_DIRECT, 0.5, 1)
+        else:
+            s.set_gravity(isph.GRAVITY_TREE, 0.5, 1, gravity == "tree_quadrupole")
+        d0 = s.diagnostics(with_pe=True)
+        for dt in dts:
+            s.set_dt(float(dt), 1.0)
+            s.encode_kdk(1)
+        d1 = s.diagnostics(with_pe=True)
+    e0 = d0["ke"] + d0["ie"] + d0["pe"]
+    de_gpu = (d1["ke"] + d1["ie"] + d1["pe"] - e0) / abs(e0)
+    dl_gpu = (d1["L"] - d0["L"]) / np.linalg.norm(d0["L"])
+    e_err = abs(de_gpu - de_cpu)
+    l_err = float(np.linalg.norm(dl_gpu - dl_cpu))
+    print(f"{gravity} {precision}: dE cpu {de_cpu:.4e} gpu {de_gpu:.4e} |diff| {e_err:.2e}; |dL| cpu "
+          f"{np.linalg.norm(dl_cpu):.2e} gpu {np.linalg.norm(dl_gpu):.2e} |diff| {l_err:.2e}; reference LBVH "
+          f"walk on the CPU: dE {de_lbvh:.4e} |dL| {np.linalg.norm(dl_lbvh):.2e}")
+    if gravity == "tree":
+        assert abs(de_gpu - de_lbvh) <= 1e-4
+        assert np.linalg.norm(dl_gpu) <= np.linalg.norm(dl_lbvh)
+    else:
+        assert e_err <= 1e-4 and l_err <= 1e-4

@@ -160,3 +160,20 @@ def test_lbvh_walk_reproduces_published_accuracy(theta, rms, mx):
     assert got_rms == pytest.approx(rms, rel=0.1)
     if mx is not None:
         assert got_max == pytest.approx(mx, rel=0.1)
+
+
+def test_lbvh_accel_is_the_same_leg_and_thread_count_free():
+    """The one-call LBVH leg used by the CPU step equals the staged Python chain, and the parallel
+    per-target loops give the same bits on 1 and 4 threads (direct sum and octree too)."""
+    pos, mass = splitmix_cloud(1500, 0xABCD1234)
+    staged = oracle.lbvh_gravity_f32(pos, mass, 5e3, 0.5)
+    outs = []
+    default = oracle.lib().orc_get_threads()
+    for t in (1, 4):
+        oracle.set_threads(t)
+        outs.append((oracle.lbvh_accel(pos, mass, 5e3, 0.5), oracle.direct_gravity(pos, mass, 5e3),
+                     oracle.octree_accel(pos, mass, 0.5, 5e3)))
+    oracle.set_threads(default)
+    assert np.array_equal(outs[0][0].astype(np.float32), staged)
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
