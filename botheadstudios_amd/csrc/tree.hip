@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_group_mark(uint32_t n, uint32_t K, uint
     const float4 b0 = nB[0], c0 = nC[0];
     const float ex0 = c0.x - b0.x, ey0 = c0.y - b0.y, ez0 = c0.z - b0.z;
     const float spacing = sqrtf(ex0 * ex0 + ey0 * ey0 + ez0 * ez0) / cbrtf((float)n);
-    const float lim = extent * fmaxf(b0.w, spacing);
+    const float lim = extent * (b0.w > 0.0f ? b0.w : spacing);
     const float lim2 = lim * lim;
     uint32_t start, count, ps, pc;
     if (!group_ok(v, n, K, L, nrange, nB, nC, lim2, maxc, start, count)) return;
