@@ -208,9 +208,9 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                 if (F64) {
                     double ddx = (double)me.x - (double)pj.x, ddy = (double)me.y - (double)pj.y,
                            ddz = (double)me.z - (double)pj.z;
-                    double h2 = 0.5 * ((double)hi + (double)cH[jj]);
-                    double q = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / h2;
-                    double sig = 8.0 / (3.14159265358979323846 * h2 * h2 * h2);
+                    double inv_h = 1.0 / (0.5 * ((double)hi + (double)cH[jj]));
+                    double q = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) * inv_h;
+                    double sig = (8.0 / 3.14159265358979323846) * (inv_h * inv_h * inv_h);
                     double w = q < 0.5 ? sig * (6.0 * (q * q * q - q * q) + 1.0)
                                        : (q < 1.0 ? sig * 2.0 * (1.0 - q) * (1.0 - q) * (1.0 - q) : 0.0);
                     rho64 += (double)pj.w * w;
@@ -274,8 +274,9 @@ __global__ __launch_bounds__(kEosThreads) void k_eos(const float* __restrict__ s
     }
 }
 
+// Five resident waves per SIMD: the f64 form needs 96 VGPRs at most then, without spilling.
 template <bool F64>
-__global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
+__global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
                                               const float4* __restrict__ svel, const float* __restrict__ srho,
                                               const float* __restrict__ spr, const float* __restrict__ scs, uint32_t n,
                                               uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
@@ -338,13 +339,20 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
             if (F64) {
                 float4 pj = cP[jj];
                 float4 sj = cS[jj];
+                // f32 pre-filter with a 1e-4 margin (a superset of the exact f64 radius test), so the f64
+                // arithmetic runs only on the ~1 in 4 candidates that are true neighbours.
+                float fx = me.x - pj.x, fy = me.y - pj.y, fz = me.z - pj.z;
+                float fh = 0.5f * (hi + sj.x);
+                if (!(fx * fx + fy * fy + fz * fz < 1.0001f * fh * fh) || cand[jj] == t) continue;
                 double dx = (double)me.x - (double)pj.x, dy = (double)me.y - (double)pj.y,
                        dz = (double)me.z - (double)pj.z;
                 double r2 = dx * dx + dy * dy + dz * dz;
                 double hij = 0.5 * ((double)hi + (double)sj.x);
-                if (r2 < hij * hij && r2 > 1.0e-18 && cand[jj] != t) {
+                if (r2 < hij * hij && r2 > 1.0e-18) {
                     float4 vj = cV[jj];
-                    double r = sqrt(r2);
+                    double inv_r = 1.0 / sqrt(r2);
+                    double inv_h = 1.0 / hij;
+                    double q = r2 * inv_r * inv_h;
                     double dvx = (double)vi.x - (double)vj.x, dvy = (double)vi.y - (double)vj.y,
                            dvz = (double)vi.z - (double)vj.z;
                     double vr = dvx * dx + dvy * dy + dvz * dz;
@@ -352,19 +360,18 @@ __global__ __launch_bounds__(64) void k_force(const float4* __restrict__ spos, c
                     if (vr < 0.0) {
                         double mu = hij * vr / (r2 + 0.01 * hij * hij);
                         double cbar = 0.5 * (ci64 + cC64[jj]);
-                        double rbar = 0.5 * (rhoi64 + cR64[jj]);
-                        pij = (-(double)av_alpha * cbar * mu + (double)av_beta * mu * mu) / rbar;
+                        pij = 2.0 * (-(double)av_alpha * cbar * mu + (double)av_beta * mu * mu) / (rhoi64 + cR64[jj]);
                     }
                     double coeff = pri64 + cPr64[jj] + pij;
-                    double q = r / hij;
-                    double h2 = hij * hij;
-                    double sig = 8.0 / (3.14159265358979323846 * (h2 * h2));
-                    double dw = q < 0.5 ? sig * (18.0 * q * q - 12.0 * q) : (q < 1.0 ? sig * -6.0 * (1.0 - q) * (1.0 - q) : 0.0);
-                    double gx = dx / r * dw, gy = dy / r * dw, gz = dz / r * dw;
+                    double ih2 = inv_h * inv_h;
+                    double sig = (8.0 / 3.14159265358979323846) * (ih2 * ih2);
+                    double dw = q < 0.5 ? sig * (18.0 * q * q - 12.0 * q) : sig * -6.0 * (1.0 - q) * (1.0 - q);
+                    double gs = dw * inv_r;
+                    double gx = dx * gs, gy = dy * gs, gz = dz * gs;
                     double mj = (double)pj.w;
-                    ax64 += gx * (-coeff * mj);
-                    ay64 += gy * (-coeff * mj);
-                    az64 += gz * (-coeff * mj);
+                    ax64 -= gx * (coeff * mj);
+                    ay64 -= gy * (coeff * mj);
+                    az64 -= gz * (coeff * mj);
                     de64 += 0.5 * mj * coeff * (dvx * gx + dvy * gy + dvz * gz);
                 }
                 continue;

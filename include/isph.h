@@ -65,7 +65,7 @@ typedef struct isph_ctx isph_t;
 
 /* Precision modes (isph_set_precision). */
 #define ISPH_PRECISION_FP32 0  /* plain: f32 state, f32 pair sums (the reference's GPU arithmetic) */
-#define ISPH_PRECISION_MIXED 1 /* f64 position/velocity state, f32 pairs with compensated sums */
+#define ISPH_PRECISION_MIXED 1 /* f64 position/velocity state; f64 SPH pair sums and per-particle rho, P, c; compensated f32 gravity sums */
 
 /* Phase indices of isph_phase_times. */
 #define ISPH_PHASE_SORT 0      /* bbox + Morton + radix sort + gather */
