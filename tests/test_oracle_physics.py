@@ -243,3 +243,49 @@ def test_energy_and_angular_momentum_diagnostics(small_scene):
     L = oracle.angular_momentum(pos, vel, mass)
     c = (mass[:, None] * pos).sum(0) / mass.sum()
     assert np.allclose(L, (mass[:, None] * np.cross(pos - c, vel)).sum(0), rtol=1e-10)
+
+
+# ---- f64 direct sum and the Kahan sum (bhtree.rs:133-143, impact-run main.rs:195-205) ----
+def test_direct_gravity_closed_forms():
+    """Two bodies: |a1| = G m2 d / (d^2 + eps^2)^1.5 towards body 2; a numpy O(N^2) sum on a random cloud;
+    total momentum sum m_i a_i = 0; a particle at the centre of a symmetric 6-point star feels nothing."""
+    soft = 10.0
+    pos = np.array([[0.0, 0.0, 0.0], [3.0e3, 4.0e3, 0.0]])
+    mass = np.array([2.0e20, 5.0e20])
+    a = oracle.direct_gravity(pos, mass, soft)
+    d = 5.0e3
+    want = G * mass[1] * d / (d * d + soft * soft) ** 1.5
+    assert np.linalg.norm(a[0]) == pytest.approx(want, rel=1e-14)
+    assert np.allclose(a[0] / np.linalg.norm(a[0]), [0.6, 0.8, 0.0], rtol=0, atol=1e-15)
+    rng = np.random.default_rng(5)
+    p = rng.normal(size=(300, 3)) * 1e6
+    m = rng.uniform(1e18, 2e18, 300)
+    acc = oracle.direct_gravity(p, m, 1e3)
+    dd = p[None, :, :] - p[:, None, :]
+    r2 = (dd ** 2).sum(-1) + 1e6
+    np.fill_diagonal(r2, np.inf)
+    ref = (G * m[None, :, None] * dd / r2[..., None] ** 1.5).sum(1)
+    assert np.allclose(acc, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    assert np.linalg.norm((m[:, None] * acc).sum(0)) < 1e-12 * (m[:, None] * np.abs(acc)).sum()
+    star = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], float) * 1e6
+    a_star = oracle.direct_gravity(star, np.full(7, 1e20), 1.0)
+    assert np.abs(a_star[0]).max() < 1e-12 * np.abs(a_star[1]).max()
+
+
+def test_kahan_sum_matches_exact_sum():
+    """sum_oi (impact-run main.rs:195-205): sorted by magnitude, then compensated. Its result does not depend
+    on the input order and stays within a few ulp of the exact sum (math.fsum) where a plain running sum
+    in the given order drifts by many."""
+    import math
+    rng = np.random.default_rng(9)
+    terms = rng.uniform(0.0, 1.0, 200000) * 10.0 ** rng.integers(-8, 8, 200000)
+    exact = math.fsum(terms)
+    got = oracle.lib().orc_sum_oi(oracle._p(np.ascontiguousarray(terms.copy())), terms.shape[0])
+    shuffled = rng.permutation(terms)
+    got2 = oracle.lib().orc_sum_oi(oracle._p(np.ascontiguousarray(shuffled)), terms.shape[0])
+    naive = 0.0
+    for t in terms:
+        naive += t
+    assert got == got2
+    assert abs(got - exact) <= 4 * math.ulp(exact)
+    assert abs(naive - exact) > 4 * math.ulp(exact)
