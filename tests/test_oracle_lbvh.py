@@ -177,3 +177,39 @@ def test_lbvh_accel_is_the_same_leg_and_thread_count_free():
     assert np.array_equal(outs[0][0].astype(np.float32), staged)
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+def test_cluster_codes_and_every_node_moment_by_brute_force():
+    """Leaf cluster code = the bucket's first sorted code (gpu_gravity.rs:382-388); every node's mass, COM
+    and AABB equal the brute-force moments of the sorted particles under it (bh_gravity.wgsl:200-241)."""
+    pos, mass = splitmix_cloud(700, 0x5A5A5A5A)
+    p = pos.astype(np.float32)
+    m = mass.astype(np.float32)
+    lo, hi = oracle.bbox_f32(p)
+    sc, order = oracle.sort_pairs(oracle.morton_f32(p, lo, hi))
+    for K in (1, 8):
+        lc = oracle.cluster_codes(sc, K)
+        assert np.array_equal(lc, sc[::K])
+        l, r, _ = oracle.karras(lc)
+        L = len(lc)
+        sb = np.concatenate([p[order], m[order, None]], axis=1)
+        com, bmin, bmax = oracle.upsweep_f32(sb, K, l, r)
+        first = np.zeros(2 * L - 1, np.int64)
+        last = np.zeros(2 * L - 1, np.int64)
+        for c in range(L):
+            first[L - 1 + c] = c * K
+            last[L - 1 + c] = min(c * K + K, len(m)) - 1
+        def span(v):
+            if v >= L - 1:
+                return first[v], last[v]
+            a, _ = span(l[v])
+            _, b = span(r[v])
+            return a, b
+        for v in range(2 * L - 1):
+            a, b = span(v)
+            q = sb[a:b + 1].astype(np.float64)
+            mt = q[:, 3].sum()
+            c = (q[:, :3] * q[:, 3:4]).sum(0) / mt
+            assert abs(com[v, 3] - mt) <= 1e-5 * mt
+            assert np.allclose(com[v, :3], c, rtol=0, atol=1e-5 * np.abs(q[:, :3]).max())
+            assert np.array_equal(bmin[v], sb[a:b + 1, :3].min(0)) and np.array_equal(bmax[v], sb[a:b + 1, :3].max(0))
