@@ -61,28 +61,127 @@ __device__ __forceinline__ float bbox_decode(uint32_t k) {
     return __uint_as_float((k >> 31) ? (k ^ 0x80000000u) : ~k);
 }
 
-// ---- Tillotson, f64 (eos.rs:52-96; the GPU shader's f32 form is sph_step.wgsl:87-109) ----------------
-__device__ __forceinline__ double till_pressure(const Till& t, double rho_in, double u) {
-    double rho = fmax(rho_in, 1.0e-9);
-    double eta = rho / t.rho0;
-    double mu = eta - 1.0;
-    double omega = u / (t.e0 * eta * eta) + 1.0;
-    double pc = (t.a + t.b / omega) * rho * u + t.cap_a * mu + t.cap_b * mu * mu;
+// ---- forward-mode dual numbers (value + N partials), for the adjoint's per-particle EOS Jacobian --------
+template <int N>
+struct Dual {
+    double v;
+    double d[N];
+};
+template <int N>
+__device__ __forceinline__ Dual<N> dual_const(double x) {
+    Dual<N> r;
+    r.v = x;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = 0.0;
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> dual_var(double x, int k) {
+    Dual<N> r = dual_const<N>(x);
+    r.d[k] = 1.0;
+    return r;
+}
+#define ISPH_DUAL_BINOP(OP, VEXPR, DEXPR)                                                                 \
+    template <int N>                                                                                      \
+    __device__ __forceinline__ Dual<N> operator OP(const Dual<N>& a, const Dual<N>& b) {                  \
+        Dual<N> r;                                                                                        \
+        r.v = VEXPR;                                                                                      \
+        _Pragma("unroll") for (int k = 0; k < N; ++k) r.d[k] = DEXPR;                                     \
+        return r;                                                                                         \
+    }                                                                                                     \
+    template <int N>                                                                                      \
+    __device__ __forceinline__ Dual<N> operator OP(const Dual<N>& a, double bv) {                         \
+        return a OP dual_const<N>(bv);                                                                    \
+    }                                                                                                     \
+    template <int N>                                                                                      \
+    __device__ __forceinline__ Dual<N> operator OP(double av, const Dual<N>& b) {                         \
+        return dual_const<N>(av) OP b;                                                                    \
+    }
+ISPH_DUAL_BINOP(+, a.v + b.v, a.d[k] + b.d[k])
+ISPH_DUAL_BINOP(-, a.v - b.v, a.d[k] - b.d[k])
+ISPH_DUAL_BINOP(*, a.v * b.v, a.d[k] * b.v + a.v * b.d[k])
+ISPH_DUAL_BINOP(/, a.v / b.v, (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v))
+#undef ISPH_DUAL_BINOP
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a) {
+    return dual_const<N>(0.0) - a;
+}
+template <int N>
+__device__ __forceinline__ bool operator>=(const Dual<N>& a, double b) { return a.v >= b; }
+template <int N>
+__device__ __forceinline__ bool operator<=(const Dual<N>& a, const Dual<N>& b) { return a.v <= b.v; }
+template <int N>
+__device__ __forceinline__ bool operator>=(const Dual<N>& a, const Dual<N>& b) { return a.v >= b.v; }
+
+// Scalar functions spelled once for double and for duals (branch decisions follow the value).
+__device__ __forceinline__ double gexp(double x) { return exp(x); }
+__device__ __forceinline__ double gsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ double gfabs(double x) { return fabs(x); }
+__device__ __forceinline__ double gfmax(double x, double y) { return fmax(x, y); }
+__device__ __forceinline__ double gval(double x) { return x; }
+template <int N>
+__device__ __forceinline__ Dual<N> gexp(const Dual<N>& a) {
+    Dual<N> r;
+    r.v = exp(a.v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = r.v * a.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> gsqrt(const Dual<N>& a) {
+    Dual<N> r;
+    r.v = sqrt(a.v);
+    const double s = r.v > 0.0 ? 0.5 / r.v : 0.0;
+#pragma unroll
+    for (int k = 0; k < N; ++k) r.d[k] = s * a.d[k];
+    return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> gfabs(const Dual<N>& a) {
+    return a.v < 0.0 ? -a : a;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> gfmax(const Dual<N>& a, double y) {
+    return a.v >= y ? a : dual_const<N>(y);
+}
+template <int N>
+__device__ __forceinline__ double gval(const Dual<N>& a) { return a.v; }
+
+// ---- Tillotson (eos.rs:52-96; the GPU shader's f32 form is sph_step.wgsl:87-109) ----------------------
+// Written once for a scalar type T: double for the step, Dual<N> for the adjoint's Jacobian.
+template <class T>
+struct TillT {
+    T rho0, a, b, cap_a, cap_b, e0, e_iv, e_cv, alpha, beta;
+};
+template <class T>
+__device__ __forceinline__ T till_pressure_t(const TillT<T>& t, T rho_in, T u) {
+    T rho = gfmax(rho_in, 1.0e-9);
+    T eta = rho / t.rho0;
+    T mu = eta - 1.0;
+    T omega = u / (t.e0 * eta * eta) + 1.0;
+    T pc = (t.a + t.b / omega) * rho * u + t.cap_a * mu + t.cap_b * mu * mu;
     if (eta >= 1.0 || u <= t.e_iv) return pc;
-    double z = t.rho0 / rho - 1.0;
-    double pe = t.a * rho * u + (t.b * rho * u / omega + t.cap_a * mu * exp(-t.beta * z)) * exp(-t.alpha * z * z);
+    T z = t.rho0 / rho - 1.0;
+    T pe = t.a * rho * u + (t.b * rho * u / omega + t.cap_a * mu * gexp(-t.beta * z)) * gexp(-t.alpha * z * z);
     if (u >= t.e_cv) return pe;
     return ((u - t.e_iv) * pe + (t.e_cv - u) * pc) / (t.e_cv - t.e_iv);
 }
 // Pressure and squared sound speed (central differences, relative step 1e-4 as eos.rs:88-96).
+template <class T>
+__device__ __forceinline__ void till_p_c2_t(const TillT<T>& t, T rho_in, T u, T& p, T& c2) {
+    T rho = gfmax(rho_in, 1.0e-9);
+    T dr = rho * 1.0e-4;
+    T du = gfabs(u) * 1.0e-4 + 1.0;
+    T dp_drho = (till_pressure_t(t, rho + dr, u) - till_pressure_t(t, rho - dr, u)) / (2.0 * dr);
+    T dp_du = (till_pressure_t(t, rho, u + du) - till_pressure_t(t, rho, u - du)) / (2.0 * du);
+    p = till_pressure_t(t, rho, u);
+    c2 = gfmax(dp_drho + p / (rho * rho) * dp_du, 0.0);
+}
+__device__ __forceinline__ double till_pressure(const Till& t, double rho_in, double u) {
+    return till_pressure_t<double>(reinterpret_cast<const TillT<double>&>(t), rho_in, u);
+}
 __device__ __forceinline__ void till_p_c2(const Till& t, double rho_in, double u, double& p, double& c2) {
-    double rho = fmax(rho_in, 1.0e-9);
-    double dr = rho * 1.0e-4;
-    double du = fabs(u) * 1.0e-4 + 1.0;
-    double dp_drho = (till_pressure(t, rho + dr, u) - till_pressure(t, rho - dr, u)) / (2.0 * dr);
-    double dp_du = (till_pressure(t, rho, u + du) - till_pressure(t, rho, u - du)) / (2.0 * du);
-    p = till_pressure(t, rho, u);
-    c2 = fmax(dp_drho + p / (rho * rho) * dp_du, 0.0);
+    till_p_c2_t<double>(reinterpret_cast<const TillT<double>&>(t), rho_in, u, p, c2);
 }
 
 // ---- cubic spline with support h (sph_step.wgsl:73-86, atmosphere.rs:52-75) ---------------------------

@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.h"
 #include "state.h"
 
 using namespace isph;
@@ -70,6 +71,9 @@ struct isph_ctx {
     // gravity-API scratch
     float4* gpos = nullptr;
     float* gmass = nullptr;
+
+    // adjoint arrays (allocated on first use)
+    Adj adj;
 };
 
 namespace {
@@ -83,9 +87,13 @@ void free_all(isph_ctx* c) {
     Dev& d = c->d;
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
-                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.ggroups, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.ggroups, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    const Adj& a = c->adj;
+    void* aptrs[] = {a.xb, a.vb, a.ub, a.mb, a.ab, a.sa, a.sxb, a.svb, a.srb, a.sqb, a.scb, a.nbar, a.pbar, a.red};
+    for (void* q : aptrs)
         if (q) (void)hipFree(q);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->rb_event) (void)hipEventDestroy(c->rb_event);
@@ -160,6 +168,8 @@ void ensure64(isph_ctx* c) {
         dalloc(&c->d.srho64, c->d.cap);
         dalloc(&c->d.spr64, c->d.cap);
         dalloc(&c->d.scs64, c->d.cap);
+        dalloc(&c->d.acc64, 3 * (size_t)c->d.cap);
+        HIP_OK(hipMemset(c->d.acc64, 0, 3 * (size_t)c->d.cap * sizeof(double)));
     }
 }
 
@@ -322,6 +332,132 @@ void widen_eos(isph_ctx* c, const isph_eos* eos, size_t n) {
                     eos[i].alpha, eos[i].beta};
     }
     HIP_OK(hipMemcpy(c->d.eos, t, sizeof(t), hipMemcpyHostToDevice));
+}
+
+// ---- adjoint driver --------------------------------------------------------------------------------
+void ensure_adj(isph_ctx* c) {
+    Adj& a = c->adj;
+    if (a.xb) return;
+    const size_t cap = c->d.cap;
+    dalloc(&a.xb, 3 * cap);
+    dalloc(&a.vb, 3 * cap);
+    dalloc(&a.ub, cap);
+    dalloc(&a.mb, cap);
+    dalloc(&a.ab, cap);
+    dalloc(&a.sa, cap);
+    dalloc(&a.sxb, cap);
+    dalloc(&a.svb, cap);
+    dalloc(&a.srb, cap);
+    dalloc(&a.sqb, cap);
+    dalloc(&a.scb, cap);
+    dalloc(&a.nbar, 2 * cap);
+    dalloc(&a.pbar, kMaxMaterials * 4);
+    dalloc(&a.red, 16);
+}
+
+// The per-evaluation arrays of Dev: one set per stored evaluation of a checkpoint segment.
+struct EvalBufs {
+    uint32_t *keys = nullptr, *vals = nullptr, *keys_alt = nullptr, *vals_alt = nullptr, *hist = nullptr;
+    float4 *spos = nullptr, *svel = nullptr, *sgrav = nullptr, *acc = nullptr;
+    float *sh = nullptr, *srho = nullptr, *spr = nullptr, *scs = nullptr;
+    uint32_t* smat = nullptr;
+    double *srho64 = nullptr, *spr64 = nullptr, *scs64 = nullptr;
+    uint32_t *lcode = nullptr, *left = nullptr, *right = nullptr, *parent = nullptr, *ready = nullptr, *cross = nullptr;
+    uint32_t *gscan = nullptr, *scal = nullptr;
+    uint2 *nrange = nullptr, *groups = nullptr, *nLR = nullptr;
+    uint4* nG = nullptr;
+    float4 *nA = nullptr, *nB = nullptr, *nC = nullptr;
+
+    void alloc(uint32_t cap, uint32_t sort_blocks) {
+        const size_t nodes = 2 * (size_t)cap;
+        dalloc(&keys, cap); dalloc(&vals, cap); dalloc(&keys_alt, cap); dalloc(&vals_alt, cap);
+        dalloc(&hist, 256 * (size_t)sort_blocks);
+        dalloc(&spos, cap); dalloc(&svel, cap); dalloc(&sgrav, cap); dalloc(&acc, cap);
+        dalloc(&sh, cap); dalloc(&srho, cap); dalloc(&spr, cap); dalloc(&scs, cap); dalloc(&smat, cap);
+        dalloc(&srho64, cap); dalloc(&spr64, cap); dalloc(&scs64, cap);
+        dalloc(&lcode, cap); dalloc(&left, cap); dalloc(&right, cap); dalloc(&parent, nodes); dalloc(&ready, cap);
+        dalloc(&cross, cap); dalloc(&gscan, cap / kWave + 16); dalloc(&scal, kScalars);
+        dalloc(&nrange, cap); dalloc(&groups, cap); dalloc(&nLR, cap);
+        dalloc(&nG, 2 * nodes); dalloc(&nA, nodes); dalloc(&nB, nodes); dalloc(&nC, nodes);
+    }
+    void release() {
+        void* ptrs[] = {keys, vals, keys_alt, vals_alt, hist, spos, svel, sgrav, acc, sh, srho, spr, scs, smat,
+                        srho64, spr64, scs64, lcode, left, right, parent, ready, cross, gscan, scal, nrange, groups,
+                        nLR, nG, nA, nB, nC};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+        *this = EvalBufs{};
+    }
+    // Exchange these arrays with the ones the pipeline writes.
+    void swap_with(Dev& d) {
+        std::swap(keys, d.keys); std::swap(vals, d.vals); std::swap(keys_alt, d.keys_alt);
+        std::swap(vals_alt, d.vals_alt); std::swap(hist, d.hist);
+        std::swap(spos, d.spos); std::swap(svel, d.svel); std::swap(sgrav, d.sgrav); std::swap(acc, d.acc);
+        std::swap(sh, d.sh); std::swap(srho, d.srho); std::swap(spr, d.spr); std::swap(scs, d.scs);
+        std::swap(smat, d.smat); std::swap(srho64, d.srho64); std::swap(spr64, d.spr64); std::swap(scs64, d.scs64);
+        std::swap(lcode, d.lcode); std::swap(left, d.left); std::swap(right, d.right); std::swap(parent, d.parent);
+        std::swap(ready, d.ready); std::swap(cross, d.cross); std::swap(gscan, d.gscan); std::swap(scal, d.scal);
+        std::swap(nrange, d.nrange); std::swap(groups, d.groups); std::swap(nLR, d.nLR); std::swap(nG, d.nG);
+        std::swap(nA, d.nA); std::swap(nB, d.nB); std::swap(nC, d.nC);
+    }
+};
+
+// Snapshot of the integrated state: f64 position and velocity, f32 internal energy.
+struct StateBuf {
+    double *x = nullptr, *v = nullptr;
+    float* u = nullptr;
+    void alloc(uint32_t cap) {
+        dalloc(&x, 3 * (size_t)cap);
+        dalloc(&v, 3 * (size_t)cap);
+        dalloc(&u, cap);
+    }
+    void release() {
+        if (x) (void)hipFree(x);
+        if (v) (void)hipFree(v);
+        if (u) (void)hipFree(u);
+        *this = StateBuf{};
+    }
+};
+
+// Owns every buffer of one reverse pass; frees them on any exit.
+struct VjpScratch {
+    std::vector<StateBuf> ck, start, mid;
+    std::vector<EvalBufs> ev;
+    float* dts = nullptr;
+    StateBuf final_state;
+    ~VjpScratch() {
+        for (auto& b : ck) b.release();
+        for (auto& b : start) b.release();
+        for (auto& b : mid) b.release();
+        for (auto& e : ev) e.release();
+        final_state.release();
+        if (dts) (void)hipFree(dts);
+    }
+};
+
+// One force evaluation into the arrays of `e` (they are swapped in for the evaluation and back out).
+void eval_into(isph_ctx* c, EvalBufs& e) {
+    e.swap_with(c->d);
+    try {
+        eval(c, true);
+    } catch (...) {
+        e.swap_with(c->d);
+        throw;
+    }
+    e.swap_with(c->d);
+    // the integrators read the evaluation's output from the handle's own array
+    HIP_OK(hipMemcpyAsync(c->d.acc, e.acc, (size_t)c->d.n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+}
+
+void vjp_of(isph_ctx* c, EvalBufs& e) {
+    e.swap_with(c->d);
+    try {
+        launch_vjp_eval(c->d, c->p, c->gravity_mode, c->adj, c->stream);
+    } catch (...) {
+        e.swap_with(c->d);
+        throw;
+    }
+    e.swap_with(c->d);
 }
 
 }  // namespace
@@ -497,6 +633,7 @@ static Dev integ_view(isph_ctx* c) {
         d.srho64 = nullptr;
         d.spr64 = nullptr;
         d.scs64 = nullptr;
+        d.acc64 = nullptr;
     }
     return d;
 }
@@ -813,6 +950,181 @@ int isph_probe_tree(isph_t* h, uint32_t* left, uint32_t* right, uint32_t* parent
         if (bmax4) HIP_OK(hipMemcpy(bmax4, h->d.nC, nodes * 16, hipMemcpyDeviceToHost));
         h->reuse_valid = false;
         return (int)L;
+    });
+}
+
+static LossParams to_loss(const isph_loss* l) {
+    return LossParams{l->radius, l->r_cut, l->k_r, l->k_j, l->axis[0], l->axis[1], l->axis[2]};
+}
+
+int isph_set_state_f64(isph_t* h, const double* pos3, const double* vel3) {
+    return guarded(h, [&]() -> int {
+        if (h->precision != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "the f64 state needs the mixed form");
+        if (h->d.n == 0) return ISPH_OK;
+        if (!pos3 || !vel3) return fail(h, ISPH_EINVAL, "NULL array");
+        HIP_OK(hipSetDevice(h->device));
+        const size_t n = h->d.n;
+        StateBuf cur, b;
+        try {
+            cur.alloc((uint32_t)n);
+            b.alloc((uint32_t)n);
+            launch_save_state(h->d, cur.x, cur.v, cur.u, h->stream);  // keeps u
+            HIP_OK(hipMemcpyAsync(b.x, pos3, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+            HIP_OK(hipMemcpyAsync(b.v, vel3, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+            launch_set_state(h->d, b.x, b.v, cur.u, h->stream);
+            HIP_OK(hipStreamSynchronize(h->stream));
+            HIP_OK(hipGetLastError());
+        } catch (...) {
+            cur.release();
+            b.release();
+            throw;
+        }
+        cur.release();
+        b.release();
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_loss_value(isph_t* h, const isph_loss* loss, double* value) {
+    return guarded(h, [&]() -> int {
+        if (!loss || !value) return fail(h, ISPH_EINVAL, "NULL argument");
+        if (h->precision != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "the loss needs the mixed form");
+        if (h->d.n == 0) {
+            *value = 0.0;
+            return ISPH_OK;
+        }
+        HIP_OK(hipSetDevice(h->device));
+        ensure_adj(h);
+        launch_adj_loss(h->d, h->adj, to_loss(loss), 1.0, h->stream);
+        HIP_OK(hipMemcpyAsync(value, h->adj.red + 7, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        return ISPH_OK;
+    });
+}
+
+int isph_vjp_run(isph_t* h, float cfl, const float* dts_in, uint32_t steps, uint32_t checkpoint_every,
+                 const isph_loss* loss, double* loss_out, double* dpos3, double* dvel3, double* du, double* dmass,
+                 double* deos4, float* dts_out, double* ms2) {
+    return guarded(h, [&]() -> int {
+        if (!loss) return fail(h, ISPH_EINVAL, "NULL loss");
+        if (h->precision != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "the adjoint needs the mixed form");
+        if (h->gravity_mode == ISPH_GRAVITY_TREE && (h->p.quad || h->p.grav_ppl != 1))
+            return fail(h, ISPH_EINVAL, "the adjoint's tree gravity is the monopole walk with one particle per lane");
+        if (steps == 0 || checkpoint_every == 0) return fail(h, ISPH_EINVAL, "steps and checkpoint_every must be > 0");
+        if (!dts_in && !(cfl > 0.0f)) return fail(h, ISPH_EINVAL, "cfl must be > 0 without a dt sequence");
+        if (h->d.n == 0) return fail(h, ISPH_EINVAL, "no particles");
+        HIP_OK(hipSetDevice(h->device));
+        ensure_adj(h);
+        const uint32_t n = h->d.n, T = steps, K = std::min(checkpoint_every, steps);
+        const uint32_t nseg = (T + K - 1) / K;
+        const LossParams lp = to_loss(loss);
+        VjpScratch w;
+        w.ck.resize(nseg);
+        for (auto& b : w.ck) b.alloc(n);
+        w.start.resize(K);
+        w.mid.resize(K);
+        for (auto& b : w.start) b.alloc(n);
+        for (auto& b : w.mid) b.alloc(n);
+        w.ev.resize(2 * K);
+        for (auto& e : w.ev) e.alloc(h->d.cap, h->d.nsort_blocks);
+        w.final_state.alloc(n);
+        dalloc(&w.dts, T);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        const bool saved_reuse = h->reuse_enabled;
+        h->reuse_enabled = false;
+        h->reuse_valid = false;
+        hipEvent_t e0, e1, e2;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        HIP_OK(hipEventCreate(&e2));
+        struct EvGuard {
+            hipEvent_t* e;
+            ~EvGuard() {
+                for (int k = 0; k < 3; ++k) (void)hipEventDestroy(e[k]);
+            }
+        };
+        hipEvent_t evs[3] = {e0, e1, e2};
+        EvGuard eg{evs};
+        struct ReuseGuard {
+            isph_ctx* c;
+            bool v;
+            ~ReuseGuard() { c->reuse_enabled = v; c->reuse_valid = false; }
+        } rg{h, saved_reuse};
+        const Dev& d = h->d;
+        const hipStream_t st = h->stream;
+
+        // forward, checkpointing every K steps; the device records each step's dt
+        HIP_OK(hipEventRecord(e0, st));
+        for (uint32_t s = 0; s < T; ++s) {
+            if (s % K == 0) launch_save_state(d, w.ck[s / K].x, w.ck[s / K].v, w.ck[s / K].u, st);
+            if (dts_in) {
+                launch_set_dt_value(d, dts_in[s], st);
+                kdk(h, false, 0.0f);
+            } else {
+                kdk(h, true, cfl);
+            }
+            HIP_OK(hipMemcpyAsync(w.dts + s, d.dt, sizeof(float), hipMemcpyDeviceToDevice, st));
+        }
+        HIP_OK(hipEventRecord(e1, st));
+        launch_save_state(d, w.final_state.x, w.final_state.v, w.final_state.u, st);
+        const double scale = 1.0;
+        launch_adj_loss(d, h->adj, lp, scale, st);
+        HIP_OK(hipMemsetAsync(h->adj.pbar, 0, kMaxMaterials * 4 * sizeof(double), st));
+        std::vector<float> dts(T);
+        HIP_OK(hipMemcpyAsync(dts.data(), w.dts, T * sizeof(float), hipMemcpyDeviceToHost, st));
+        double lval = 0.0;
+        HIP_OK(hipMemcpyAsync(&lval, h->adj.red + 7, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+
+        // reverse: recompute each segment from its checkpoint (storing every evaluation), then walk back
+        for (int seg = (int)nseg - 1; seg >= 0; --seg) {
+            const uint32_t s0 = (uint32_t)seg * K, s1 = std::min(T, s0 + K);
+            launch_set_state(d, w.ck[seg].x, w.ck[seg].v, w.ck[seg].u, st);
+            for (uint32_t s = s0; s < s1; ++s) {
+                const uint32_t i = s - s0;
+                launch_save_state(d, w.start[i].x, w.start[i].v, w.start[i].u, st);
+                launch_set_dt_value(d, dts[s], st);
+                eval_into(h, w.ev[2 * i]);
+                launch_kick_drift(d, st);
+                launch_save_state(d, w.mid[i].x, w.mid[i].v, w.mid[i].u, st);
+                eval_into(h, w.ev[2 * i + 1]);
+                launch_kick(d, st);
+            }
+            for (int s = (int)s1 - 1; s >= (int)s0; --s) {
+                const uint32_t i = (uint32_t)s - s0;
+                launch_vjp_kick(d, h->adj, w.mid[i].u, w.ev[2 * i + 1].acc, dts[s], st);
+                vjp_of(h, w.ev[2 * i + 1]);
+                launch_vjp_kick_drift(d, h->adj, w.start[i].u, w.ev[2 * i].acc, dts[s], st);
+                vjp_of(h, w.ev[2 * i]);
+            }
+        }
+        HIP_OK(hipEventRecord(e2, st));
+        // leave the handle at the forward run's final state
+        launch_set_state(d, w.final_state.x, w.final_state.v, w.final_state.u, st);
+        HIP_OK(hipStreamSynchronize(st));
+        HIP_OK(hipGetLastError());
+        if (ms2) {
+            float a = 0.f, b = 0.f;
+            HIP_OK(hipEventElapsedTime(&a, e0, e1));
+            HIP_OK(hipEventElapsedTime(&b, e1, e2));
+            ms2[0] = a;
+            ms2[1] = b;
+        }
+        if (loss_out) *loss_out = lval;
+        auto fetch = [&](double* dst, const double* src, size_t count) {
+            if (!dst) return;
+            HIP_OK(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < count; ++k) dst[k] /= scale;
+        };
+        fetch(dpos3, h->adj.xb, 3 * (size_t)n);
+        fetch(dvel3, h->adj.vb, 3 * (size_t)n);
+        fetch(du, h->adj.ub, n);
+        fetch(dmass, h->adj.mb, n);
+        fetch(deos4, h->adj.pbar, 4 * (size_t)kMaxMaterials);
+        if (dts_out) std::copy(dts.begin(), dts.end(), dts_out);
+        return ISPH_OK;
     });
 }
 

@@ -10,16 +10,10 @@
 // cell-membership-guarded 27-cell scan does. Waves take walk groups (compact subtrees of <= 64 particles)
 // from a work queue. Pressure and sound speed are computed once per particle (the
 // reference shader re-evaluates six Tillotson calls per neighbour pair, sph_step.wgsl:273-275).
-#include "state.h"
+#include "walk.h"
 
 namespace isph {
 
-#ifndef ISPH_SPH_STACK
-#define ISPH_SPH_STACK 512
-#endif
-constexpr int kStackCap = ISPH_SPH_STACK;  // LDS traversal stack (entries); the pop width narrows near it
-constexpr int kStackMargin = 128;          // > maximum Karras depth
-constexpr int kBuf = 128;
 
 // AoS record, identical layout to isph_particle (gpu_sph.rs:27-37).
 struct ParticleRec {
@@ -31,110 +25,6 @@ struct ParticleRec {
     uint32_t prov;
 };
 static_assert(sizeof(ParticleRec) == 48, "SphParticle layout");
-
-// Wave-cooperative neighbour traversal. `flush(cnt)` is called with cand[0..cnt) holding candidate sorted
-// indices (cnt <= 64); it must evaluate them for the calling lane.
-template <class Flush>
-__device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, uint32_t n, uint32_t K, uint32_t L,
-                                               const uint2* __restrict__ nlr, const uint2* __restrict__ nrange,
-                                               uint32_t expand,
-                                               const float4* __restrict__ nB, const float4* __restrict__ nC,
-                                               float gx0, float gy0, float gz0, float gx1, float gy1, float gz1,
-                                               float hgmax, unsigned long long* stats, Flush&& flush) {
-    const int lane = threadIdx.x;
-    const uint32_t leaf0 = L - 1;
-    int sp = 1, nq = 0;
-    unsigned long long rounds = 0, popped = 0;
-    int max_sp = 1;
-    if (lane == 0) stk[0] = 0;
-    __syncthreads();
-    auto shift = [&](int cnt) {
-        __syncthreads();
-        if (lane + kWave < cnt) cand[lane] = cand[lane + kWave];
-        __syncthreads();
-    };
-    while (sp > 0) {
-        int k = min(kWave, sp);
-        int room = kStackCap - kStackMargin - sp;
-        if (room < k) k = max(1, room);
-        sp -= k;
-        rounds += 1;
-        popped += k;
-        const bool has = lane < k;
-        const uint32_t node = has ? stk[sp + lane] : 0u;
-        __syncthreads();
-        bool keep = false;
-        uint32_t lc = 0, rc = 0;
-        if (has) {
-            const float4 B = nB[node];
-            const float4 C = nC[node];
-            if (node < leaf0) {  // child indices travel with the box: one memory round trip per round
-                const uint2 ch = nlr[node];
-                lc = ch.x;
-                rc = ch.y;
-            }
-            float dx = fmaxf(fmaxf(B.x - gx1, gx0 - C.x), 0.f);
-            float dy = fmaxf(fmaxf(B.y - gy1, gy0 - C.y), 0.f);
-            float dz = fmaxf(fmaxf(B.z - gz1, gz0 - C.z), 0.f);
-            float rr = 0.5f * (hgmax + B.w);
-            keep = dx * dx + dy * dy + dz * dz < rr * rr;
-        }
-        // A kept subtree of <= `expand` particles is taken whole (its leaf range is contiguous in sorted
-        // order) instead of visiting its leaves one round at a time; the exact radius test follows anyway.
-        uint32_t first = 0, cnt = 0;
-        if (keep) {
-            if (node >= leaf0) {
-                first = (node - leaf0) * K;
-                cnt = min(first + K, n) - first;
-            } else {
-                const uint2 r = nrange[node];
-                const uint32_t pf = r.x * K, pe = min((r.y + 1) * K, n);
-                if (pe - pf <= expand) {
-                    first = pf;
-                    cnt = pe - pf;
-                }
-            }
-        }
-        const bool leaf = keep && cnt > 0;
-        const bool open = keep && cnt == 0;
-        const uint64_t mo = __ballot(open);
-        if (open) {
-            int pos = sp + 2 * prefix_count(mo);
-            stk[pos] = lc;
-            stk[pos + 1] = rc;
-        }
-        sp += 2 * __popcll(mo);
-        max_sp = max(max_sp, sp);
-        const uint64_t ml = __ballot(leaf);
-        if (ml) {
-            const uint32_t width = max(K, expand);
-            for (uint32_t kk = 0; kk < width; ++kk) {
-                const bool present = leaf && kk < cnt;
-                const uint64_t mp = __ballot(present);
-                if (!mp) break;
-                if (present) cand[nq + prefix_count(mp)] = first + kk;
-                nq += __popcll(mp);
-                if (nq >= kWave) {
-                    __syncthreads();
-                    flush(kWave);
-                    shift(nq);
-                    nq -= kWave;
-                }
-            }
-        }
-        __syncthreads();
-    }
-    if (nq > 0) {
-        __syncthreads();
-        flush(nq);
-        __syncthreads();
-    }
-    if (lane == 0 && stats) {
-        atomicAdd(&stats[0], rounds);
-        atomicAdd(&stats[1], popped);
-        atomicMax(&stats[2], (unsigned long long)max_sp);
-    }
-}
 
 // Per-particle EOS: pressure / rho^2 and sound speed from the f64 Tillotson (eos.rs:88-96); returns the
 // particle's Courant signal h / max(c + |v|, 1).
@@ -164,8 +54,8 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                                                 double* __restrict__ srho64, uint32_t* scal,
                                                 unsigned long long* counters, const uint2* __restrict__ groups,
                                                 const uint2* __restrict__ nrange, uint32_t expand) {
-    __shared__ uint32_t stk[kStackCap];
-    __shared__ uint32_t cand[kBuf];
+    __shared__ uint32_t stk[kSphStack];
+    __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];
     __shared__ float cH[kWave];
     const int lane = threadIdx.x;
@@ -286,9 +176,10 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
                                               const double* __restrict__ srho64, const double* __restrict__ spr64,
                                               const double* __restrict__ scs64, float av_alpha, float av_beta, unsigned long long* counters,
                                               const uint2* __restrict__ groups, uint32_t* scal,
-                                              const uint2* __restrict__ nrange, uint32_t expand) {
-    __shared__ uint32_t stk[kStackCap];
-    __shared__ uint32_t cand[kBuf];
+                                              const uint2* __restrict__ nrange, uint32_t expand,
+                                              double* __restrict__ acc64) {
+    __shared__ uint32_t stk[kSphStack];
+    __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, mass
     __shared__ float4 cV[kWave];  // vx, vy, vz, rho
     __shared__ float4 cS[kWave];  // h, P/rho^2, c
@@ -415,7 +306,14 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
     if (valid) {
         float4 gr = sgrav[t];
         float4 out = me.w > 0.f ? make_float4(gr.x + ax, gr.y + ay, gr.z + az, de) : make_float4(0.f, 0.f, 0.f, 0.f);
-        acc_orig[order[t]] = out;
+        const uint32_t o = order[t];
+        acc_orig[o] = out;
+        if (F64 && acc64) {  // the f64 integrators take the sum unrounded
+            const bool live = me.w > 0.f;
+            acc64[3 * o] = live ? (double)gr.x + ax64 : 0.0;
+            acc64[3 * o + 1] = live ? (double)gr.y + ay64 : 0.0;
+            acc64[3 * o + 2] = live ? (double)gr.z + az64 : 0.0;
+        }
     }
     tested_all += tested * grp.y;
     }
@@ -425,7 +323,8 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
 // ---- KDK (sph_step.wgsl:304-333; hydrostatic.rs:451-465), retired particles are not integrated ----------
 __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, float4* velu,
                                                     const float* __restrict__ mass, const float4* __restrict__ acc,
-                                                    const float* __restrict__ dtp, double* pos64, double* vel64) {
+                                                    const float* __restrict__ dtp, double* pos64, double* vel64,
+                                                    const double* __restrict__ acc64) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !(mass[i] > 0.f)) return;
     const float dt = *dtp;
@@ -435,8 +334,8 @@ __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, fl
     v.w = fmaxf(v.w + a.w * (0.5f * dt), 0.0f);
     if (pos64) {
         double hd = 0.5 * (double)dt;
-        double vx = vel64[3 * i] + (double)a.x * hd, vy = vel64[3 * i + 1] + (double)a.y * hd,
-               vz = vel64[3 * i + 2] + (double)a.z * hd;
+        double vx = vel64[3 * i] + acc64[3 * i] * hd, vy = vel64[3 * i + 1] + acc64[3 * i + 1] * hd,
+               vz = vel64[3 * i + 2] + acc64[3 * i + 2] * hd;
         double x = pos64[3 * i] + vx * (double)dt, y = pos64[3 * i + 1] + vy * (double)dt,
                z = pos64[3 * i + 2] + vz * (double)dt;
         vel64[3 * i] = vx;
@@ -465,7 +364,7 @@ __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, fl
 
 __global__ __launch_bounds__(256) void k_kick(uint32_t n, float4* velu, const float* __restrict__ mass,
                                               const float4* __restrict__ acc, const float* __restrict__ dtp,
-                                              double* vel64) {
+                                              double* vel64, const double* __restrict__ acc64) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !(mass[i] > 0.f)) return;
     const float dt = *dtp;
@@ -474,8 +373,8 @@ __global__ __launch_bounds__(256) void k_kick(uint32_t n, float4* velu, const fl
     v.w = fmaxf(v.w + a.w * (0.5f * dt), 0.0f);
     if (vel64) {
         double hd = 0.5 * (double)dt;
-        double vx = vel64[3 * i] + (double)a.x * hd, vy = vel64[3 * i + 1] + (double)a.y * hd,
-               vz = vel64[3 * i + 2] + (double)a.z * hd;
+        double vx = vel64[3 * i] + acc64[3 * i] * hd, vy = vel64[3 * i + 1] + acc64[3 * i + 1] * hd,
+               vz = vel64[3 * i + 2] + acc64[3 * i + 2] * hd;
         vel64[3 * i] = vx;
         vel64[3 * i + 1] = vy;
         vel64[3 * i + 2] = vz;
@@ -670,19 +569,21 @@ void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
                            d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
                            d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
-                           (uint32_t)p.sph_expand);
+                           (uint32_t)p.sph_expand, d.acc64);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
                            d.scs, d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
-                           p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand);
+                           p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand,
+                           (double*)nullptr);
 }
 void launch_kick_drift(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,
-                       d.pos64, d.vel64);
+                       d.pos64, d.vel64, d.acc64);
 }
 void launch_kick(const Dev& d, hipStream_t s) {
-    hipLaunchKernelGGL(k_kick, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.velu, d.mass, d.acc, d.dt, d.vel64);
+    hipLaunchKernelGGL(k_kick, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.velu, d.mass, d.acc, d.dt, d.vel64,
+                       d.acc64);
 }
 void launch_set_dt_cfl(const Dev& d, float cfl, hipStream_t s) {
     hipLaunchKernelGGL(k_set_dt_cfl, dim3(1), dim3(1), 0, s, d.scal, cfl, d.dt);

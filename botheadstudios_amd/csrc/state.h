@@ -25,6 +25,7 @@ struct Dev {
     float4* acc = nullptr;    // ax, ay, az, du/dt of the last evaluation
     double* pos64 = nullptr;  // mixed precision: x, y, z per particle (3n)
     double* vel64 = nullptr;  // mixed precision: vx, vy, vz per particle (3n)
+    double* acc64 = nullptr;  // mixed precision: ax, ay, az of the last evaluation, unrounded (3n)
 
     // Morton + sort
     uint32_t* keys = nullptr;  // codes

@@ -1,0 +1,119 @@
+// Wave-cooperative neighbour search over the LBVH, shared by the SPH walks and their adjoints.
+#pragma once
+
+#include "state.h"
+
+namespace isph {
+
+#ifndef ISPH_SPH_STACK
+#define ISPH_SPH_STACK 512
+#endif
+constexpr int kSphStack = ISPH_SPH_STACK;  // LDS traversal stack (entries); the pop width narrows near it
+constexpr int kSphStackMargin = 128;       // > maximum Karras depth
+constexpr int kSphBuf = 128;               // candidate list (two flushes of 64)
+
+// Wave-cooperative neighbour traversal. `flush(cnt)` is called with cand[0..cnt) holding candidate sorted
+// indices (cnt <= 64); it must evaluate them for the calling lane.
+template <class Flush>
+__device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, uint32_t n, uint32_t K, uint32_t L,
+                                               const uint2* __restrict__ nlr, const uint2* __restrict__ nrange,
+                                               uint32_t expand,
+                                               const float4* __restrict__ nB, const float4* __restrict__ nC,
+                                               float gx0, float gy0, float gz0, float gx1, float gy1, float gz1,
+                                               float hgmax, unsigned long long* stats, Flush&& flush) {
+    const int lane = threadIdx.x;
+    const uint32_t leaf0 = L - 1;
+    int sp = 1, nq = 0;
+    unsigned long long rounds = 0, popped = 0;
+    int max_sp = 1;
+    if (lane == 0) stk[0] = 0;
+    __syncthreads();
+    auto shift = [&](int cnt) {
+        __syncthreads();
+        if (lane + kWave < cnt) cand[lane] = cand[lane + kWave];
+        __syncthreads();
+    };
+    while (sp > 0) {
+        int k = min(kWave, sp);
+        int room = kSphStack - kSphStackMargin - sp;
+        if (room < k) k = max(1, room);
+        sp -= k;
+        rounds += 1;
+        popped += k;
+        const bool has = lane < k;
+        const uint32_t node = has ? stk[sp + lane] : 0u;
+        __syncthreads();
+        bool keep = false;
+        uint32_t lc = 0, rc = 0;
+        if (has) {
+            const float4 B = nB[node];
+            const float4 C = nC[node];
+            if (node < leaf0) {  // child indices travel with the box: one memory round trip per round
+                const uint2 ch = nlr[node];
+                lc = ch.x;
+                rc = ch.y;
+            }
+            float dx = fmaxf(fmaxf(B.x - gx1, gx0 - C.x), 0.f);
+            float dy = fmaxf(fmaxf(B.y - gy1, gy0 - C.y), 0.f);
+            float dz = fmaxf(fmaxf(B.z - gz1, gz0 - C.z), 0.f);
+            float rr = 0.5f * (hgmax + B.w);
+            keep = dx * dx + dy * dy + dz * dz < rr * rr;
+        }
+        // A kept subtree of <= `expand` particles is taken whole (its leaf range is contiguous in sorted
+        // order) instead of visiting its leaves one round at a time; the exact radius test follows anyway.
+        uint32_t first = 0, cnt = 0;
+        if (keep) {
+            if (node >= leaf0) {
+                first = (node - leaf0) * K;
+                cnt = min(first + K, n) - first;
+            } else {
+                const uint2 r = nrange[node];
+                const uint32_t pf = r.x * K, pe = min((r.y + 1) * K, n);
+                if (pe - pf <= expand) {
+                    first = pf;
+                    cnt = pe - pf;
+                }
+            }
+        }
+        const bool leaf = keep && cnt > 0;
+        const bool open = keep && cnt == 0;
+        const uint64_t mo = __ballot(open);
+        if (open) {
+            int pos = sp + 2 * prefix_count(mo);
+            stk[pos] = lc;
+            stk[pos + 1] = rc;
+        }
+        sp += 2 * __popcll(mo);
+        max_sp = max(max_sp, sp);
+        const uint64_t ml = __ballot(leaf);
+        if (ml) {
+            const uint32_t width = max(K, expand);
+            for (uint32_t kk = 0; kk < width; ++kk) {
+                const bool present = leaf && kk < cnt;
+                const uint64_t mp = __ballot(present);
+                if (!mp) break;
+                if (present) cand[nq + prefix_count(mp)] = first + kk;
+                nq += __popcll(mp);
+                if (nq >= kWave) {
+                    __syncthreads();
+                    flush(kWave);
+                    shift(nq);
+                    nq -= kWave;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (nq > 0) {
+        __syncthreads();
+        flush(nq);
+        __syncthreads();
+    }
+    if (lane == 0 && stats) {
+        atomicAdd(&stats[0], rounds);
+        atomicAdd(&stats[1], popped);
+        atomicMax(&stats[2], (unsigned long long)max_sp);
+    }
+}
+
+}  // namespace isph

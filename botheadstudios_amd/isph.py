@@ -55,7 +55,22 @@ _SIGS = {
     "isph_probe_morton": (ctypes.c_int, [_P, _P, _P, _P]),
     "isph_probe_sort": (ctypes.c_int, [_P, _P, _P]),
     "isph_probe_tree": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "isph_loss_value": (ctypes.c_int, [_P, _P, _P]),
+    "isph_set_state_f64": (ctypes.c_int, [_P, _P, _P]),
+    "isph_vjp_run": (ctypes.c_int, [_P, ctypes.c_float, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
+                                    _P, _P, _P, _P]),
 }
+
+
+class Loss(ctypes.Structure):
+    """`isph_loss`: smoothed mass outside r_cut radii whose specific angular momentum exceeds the circular
+    value (SI units)."""
+    _fields_ = [("radius", ctypes.c_double), ("r_cut", ctypes.c_double), ("k_r", ctypes.c_double),
+                ("k_j", ctypes.c_double), ("axis", ctypes.c_double * 3)]
+
+    @classmethod
+    def make(cls, radius, r_cut=2.9, k_r=20.0, k_j=10.0, axis=(0.0, 0.0, 1.0)):
+        return cls(float(radius), float(r_cut), float(k_r), float(k_j), (ctypes.c_double * 3)(*axis))
 
 _lib = None
 
@@ -187,6 +202,34 @@ class GpuSph:
         rho = np.zeros(n, np.float32)
         self._check(self._lib.isph_force_eval(self._h, _ptr(acc), _ptr(rho)))
         return acc[:, :3].copy(), acc[:, 3].copy(), rho
+
+    # ---- adjoint ----
+    def loss_value(self, loss):
+        out = ctypes.c_double(0.0)
+        self._check(self._lib.isph_loss_value(self._h, ctypes.byref(loss), ctypes.byref(out)))
+        return out.value
+
+    def set_state_f64(self, pos, vel):
+        p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        self._check(self._lib.isph_set_state_f64(self._h, _ptr(p), _ptr(v)))
+
+    def vjp_run(self, loss, steps, checkpoint_every=20, cfl=0.3, dts=None):
+        """Reverse-mode gradient of `loss` after `steps` KDK steps from the current state. Returns a dict:
+        loss, dpos (n,3), dvel (n,3), du (n,), dmass (n,), deos (16 materials x (a, b, A, B)), dts, ms
+        (forward, reverse)."""
+        n = self.count()
+        out = {"dpos": np.zeros((n, 3)), "dvel": np.zeros((n, 3)), "du": np.zeros(n), "dmass": np.zeros(n),
+               "deos": np.zeros((16, 4)), "dts": np.zeros(steps, np.float32), "ms": np.zeros(2)}
+        lv = ctypes.c_double(0.0)
+        d_in = None if dts is None else np.ascontiguousarray(dts, np.float32)
+        self._check(self._lib.isph_vjp_run(self._h, float(cfl), None if d_in is None else _ptr(d_in), int(steps),
+                                           int(checkpoint_every), ctypes.byref(loss), ctypes.byref(lv),
+                                           _ptr(out["dpos"]), _ptr(out["dvel"]), _ptr(out["du"]),
+                                           _ptr(out["dmass"]), _ptr(out["deos"]), _ptr(out["dts"]),
+                                           _ptr(out["ms"])))
+        out["loss"] = lv.value
+        return out
 
     def diagnostics(self, with_pe=True):
         out = np.zeros(7, np.float64)

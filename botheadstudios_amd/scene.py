@@ -185,3 +185,70 @@ def f64_state(rec):
     """Widen `SphParticle` records to the f64 arrays the CPU path takes (bit-identical start state)."""
     return (rec["pos"].astype(np.float64), rec["vel"].astype(np.float64), rec["mass"].astype(np.float64),
             rec["u"].astype(np.float64), rec["h"].astype(np.float64), rec["mat"].astype(np.uint32))
+
+
+class ImpactFamily:
+    """The two-body scene as a smooth function of the impact parameters, for the differentiable step.
+
+    Bodies are generated once (as in `impact_scene`); `state(angle_deg, v_over_vesc, mass_ratio)` places the
+    impactor and sets its velocity and particle masses, and `tangents(...)` gives the derivative of the f64
+    SI state with respect to each parameter. Smoothing lengths stay those of the nominal impactor (mass
+    ratio 0.11), so that the state is smooth in the mass ratio. At the nominal parameters the state equals
+    `impact_scene`'s.
+    """
+
+    def __init__(self, n_target, n_impactor, seed, r_t=1.0, m_t=1.0, r_i=0.5, m_i=0.11, core=0.3, u0=1e-3,
+                 softening=0.01):
+        self.pt, self.mt, self.ht, self.matt, _ = make_body(n_target, r_t, m_t, core, seed ^ 0x7A1)
+        self.pi, _, self.hi, self.mati, _ = make_body(n_impactor, r_i, m_i, core, seed ^ 0x1A2)
+        self.r_t, self.m_t, self.r_i = r_t, m_t, r_i
+        self.n_target, self.n_impactor = n_target, n_impactor
+        self.u0, self.softening = u0, softening
+
+    def _v(self, speed, ratio):
+        return speed * np.sqrt(2.0 * (self.m_t + ratio * self.m_t) / (self.r_t + self.r_i))
+
+    def arrays(self, angle_deg=45.0, v_over_vesc=1.2, mass_ratio=0.11):
+        """The f64 SI state: pos (n,3), vel (n,3), mass, h, u, mat."""
+        contact = self.r_t + self.r_i
+        b = contact * np.sin(np.deg2rad(angle_deg))
+        pos = np.concatenate([self.pt, self.pi + np.array([1.6 * contact, b, 0.0])]) * L_U
+        vel = np.zeros_like(pos)
+        vel[self.n_target:, 0] = -self._v(v_over_vesc, mass_ratio) * V_U
+        mass = np.concatenate([self.mt, np.full(self.n_impactor, mass_ratio * self.m_t / self.n_impactor)]) * M_U
+        h = np.concatenate([self.ht, self.hi]) * L_U
+        mat = np.concatenate([self.matt, self.mati])
+        u = np.full(len(mass), self.u0 * E_U)
+        return pos, vel, mass, h, u, mat
+
+    def state(self, angle_deg=45.0, v_over_vesc=1.2, mass_ratio=0.11):
+        """(records, pos, vel): the 48-byte records (f32) and the f64 position / velocity state."""
+        pos, vel, mass, h, u, mat = self.arrays(angle_deg, v_over_vesc, mass_ratio)
+        n = len(mass)
+        rec = np.zeros(n, dtype=PARTICLE_DTYPE)
+        rec["pos"] = pos.astype(np.float32)
+        rec["vel"] = vel.astype(np.float32)
+        rec["h"] = h.astype(np.float32)
+        rec["u"] = np.float32(self.u0 * E_U)
+        rec["mass"] = mass.astype(np.float32)
+        rec["mat"] = mat
+        rec["prov"] = np.concatenate([np.zeros(self.n_target, np.uint32), np.ones(self.n_impactor, np.uint32)])
+        return rec, pos, vel
+
+    def tangents(self, angle_deg=45.0, v_over_vesc=1.2, mass_ratio=0.11):
+        """d(pos, vel, mass)/d(angle in degrees, v_over_vesc, mass_ratio), each (n,3), (n,3), (n,)."""
+        n, nt = self.n_target + self.n_impactor, self.n_target
+        contact = self.r_t + self.r_i
+        out = []
+        dpos = np.zeros((n, 3))
+        dpos[nt:, 1] = contact * np.cos(np.deg2rad(angle_deg)) * np.pi / 180.0 * L_U
+        out.append((dpos, np.zeros((n, 3)), np.zeros(n)))
+        dvel = np.zeros((n, 3))
+        dvel[nt:, 0] = -self._v(1.0, mass_ratio) * V_U
+        out.append((np.zeros((n, 3)), dvel, np.zeros(n)))
+        dvel = np.zeros((n, 3))
+        dvel[nt:, 0] = -v_over_vesc * (self.m_t / contact) / np.sqrt(2.0 * (self.m_t + mass_ratio * self.m_t) / contact) * V_U
+        dm = np.zeros(n)
+        dm[nt:] = self.m_t / self.n_impactor * M_U
+        out.append((np.zeros((n, 3)), dvel, dm))
+        return out

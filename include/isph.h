@@ -164,6 +164,35 @@ int isph_kernel_stats(isph_t* h, double* out8);
 int isph_walk_stats(isph_t* h, double* out9);
 int isph_reset_stats(isph_t* h);
 
+/* ---- adjoint (no reference counterpart: the differentiable form of the step) ------------------------ */
+/* Smoothed disk mass: loss = sum_i m_i sig(k_r (r_i / radius - r_cut)) sig(k_j (j_i / sqrt(G M r_i) - 1)),
+ * r_i = |x_i - X|, j_i = axis . ((x_i - X) x (v_i - V)), X, V, M the centre of mass, its velocity and the
+ * total mass; sig(z) = 1 / (1 + e^-z). SI units, f64. */
+typedef struct isph_loss {
+    double radius; /* length unit of the cut (the target radius), m */
+    double r_cut;  /* cut radius in units of `radius` */
+    double k_r;    /* sharpness of the radial sigmoid */
+    double k_j;    /* sharpness of the angular-momentum sigmoid */
+    double axis[3];
+} isph_loss;
+
+/* Loss of the current state (mixed form). */
+int isph_loss_value(isph_t* h, const isph_loss* loss, double* value);
+/* Replace the f64 position / velocity state exactly (mixed form; u, h, mass unchanged). */
+int isph_set_state_f64(isph_t* h, const double* pos3, const double* vel3);
+/* Reverse-mode gradient of `loss` after `steps` KDK steps from the current state (mixed form; gravity direct,
+ * none, or the monopole tree walk with one particle per lane). Time steps: dts_in[steps] if not NULL, else
+ * the device Courant dt at `cfl`; they are held fixed in the derivative. The forward run stores a state
+ * every `checkpoint_every` steps; the reverse pass recomputes each segment from its checkpoint and walks it
+ * back one evaluation at a time with the tree topology and walk decisions of the forward evaluation.
+ * Outputs (any may be NULL): the loss; d loss / d (position[3n], velocity[3n], u[n], mass[n]) of the start
+ * state in upload order; d loss / d (a, b, A, B) of each material's Tillotson parameters [16][4]; the dt
+ * sequence [steps]; ms2 = {forward ms, reverse ms} from HIP events on the handle's stream. The handle is
+ * left at the forward run's final state. */
+int isph_vjp_run(isph_t* h, float cfl, const float* dts_in, uint32_t steps, uint32_t checkpoint_every,
+                 const isph_loss* loss, double* loss_out, double* dpos3, double* dvel3, double* du, double* dmass,
+                 double* deos4, float* dts_out, double* ms2);
+
 /* ---- stage probes (parity of the bit-exact stages against the oracle) ------------------------------- */
 /* bbox + Morton of the current positions: codes in upload order, lo3/hi3 the f32 bbox (cs_bbox/cs_morton,
  * bh_gravity.wgsl:67-108). */

@@ -502,6 +502,288 @@ struct Karras {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------------
+// Reference gradient of the CPU step by forward-mode dual numbers (value + N partials): the step of
+// hydrostatic.rs:338-465 (direct-sum gravity, grid neighbours, FD Tillotson sound speed, KDK with the
+// u >= 0 clamp) written once for a generic scalar, run with the partials seeded by the tangents of the
+// initial state and by unit seeds on a material's Tillotson a, b, A, B. Branch decisions and neighbour
+// sets follow the values, so the result is the derivative of the f64 step itself.
+constexpr int kDualN = 7;
+struct Dl {
+    double v;
+    double d[kDualN];
+    Dl() : v(0.0) { std::fill(d, d + kDualN, 0.0); }
+    Dl(double x) : v(x) { std::fill(d, d + kDualN, 0.0); }  // NOLINT: implicit on purpose
+};
+inline double val(double x) { return x; }
+inline double val(const Dl& a) { return a.v; }
+inline Dl operator+(const Dl& a, const Dl& b) {
+    Dl r(a.v + b.v);
+    for (int k = 0; k < kDualN; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+inline Dl operator-(const Dl& a, const Dl& b) {
+    Dl r(a.v - b.v);
+    for (int k = 0; k < kDualN; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+inline Dl operator-(const Dl& a) { return Dl(0.0) - a; }
+inline Dl operator*(const Dl& a, const Dl& b) {
+    Dl r(a.v * b.v);
+    for (int k = 0; k < kDualN; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+inline Dl operator/(const Dl& a, const Dl& b) {
+    Dl r(a.v / b.v);
+    for (int k = 0; k < kDualN; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+    return r;
+}
+inline Dl operator+(const Dl& a, double b) { return a + Dl(b); }
+inline Dl operator+(double a, const Dl& b) { return Dl(a) + b; }
+inline Dl operator-(const Dl& a, double b) { return a - Dl(b); }
+inline Dl operator-(double a, const Dl& b) { return Dl(a) - b; }
+inline Dl operator*(const Dl& a, double b) {
+    Dl r(a.v * b);
+    for (int k = 0; k < kDualN; ++k) r.d[k] = a.d[k] * b;
+    return r;
+}
+inline Dl operator*(double a, const Dl& b) { return b * a; }
+inline Dl operator/(const Dl& a, double b) { return a / Dl(b); }
+inline Dl operator/(double a, const Dl& b) { return Dl(a) / b; }
+inline Dl& operator+=(Dl& a, const Dl& b) { return a = a + b; }
+inline Dl g_sqrt(const Dl& a) {
+    Dl r(std::sqrt(a.v));
+    double s = r.v > 0.0 ? 0.5 / r.v : 0.0;
+    for (int k = 0; k < kDualN; ++k) r.d[k] = s * a.d[k];
+    return r;
+}
+inline Dl g_exp(const Dl& a) {
+    Dl r(std::exp(a.v));
+    for (int k = 0; k < kDualN; ++k) r.d[k] = r.v * a.d[k];
+    return r;
+}
+inline Dl g_fabs(const Dl& a) { return a.v < 0.0 ? -a : a; }
+inline double g_sqrt(double a) { return std::sqrt(a); }
+inline double g_exp(double a) { return std::exp(a); }
+inline double g_fabs(double a) { return std::fabs(a); }
+template <class S>
+inline S g_max(const S& a, double b) { return val(a) >= b ? a : S(b); }
+
+template <class S>
+struct TillG {
+    S rho0, a, b, cap_a, cap_b, e0, e_iv, e_cv, alpha, beta;
+};
+template <class S>
+S till_pressure_g(const TillG<S>& t, S rho_in, S u) {  // eos.rs:52-83
+    S rho = g_max(rho_in, 1.0e-9);
+    S eta = rho / t.rho0;
+    S mu = eta - 1.0;
+    S omega = u / (t.e0 * eta * eta) + 1.0;
+    S pc = (t.a + t.b / omega) * rho * u + t.cap_a * mu + t.cap_b * mu * mu;
+    if (val(eta) >= 1.0 || val(u) <= val(t.e_iv)) return pc;
+    S z = t.rho0 / rho - 1.0;
+    S pe = t.a * rho * u + (t.b * rho * u / omega + t.cap_a * mu * g_exp(-(t.beta * z))) * g_exp(-(t.alpha * z * z));
+    if (val(u) >= val(t.e_cv)) return pe;
+    return ((u - t.e_iv) * pe + (t.e_cv - u) * pc) / (t.e_cv - t.e_iv);
+}
+template <class S>
+S till_c2_g(const TillG<S>& t, S rho_in, S u) {  // eos.rs:88-96
+    S rho = g_max(rho_in, 1.0e-9);
+    S dr = rho * 1.0e-4;
+    S du = g_fabs(u) * 1.0e-4 + 1.0;
+    S dp_drho = (till_pressure_g(t, rho + dr, u) - till_pressure_g(t, rho - dr, u)) / (2.0 * dr);
+    S dp_du = (till_pressure_g(t, rho, u + du) - till_pressure_g(t, rho, u - du)) / (2.0 * du);
+    S p = till_pressure_g(t, rho, u);
+    return g_max(dp_drho + p / (rho * rho) * dp_du, 0.0);
+}
+template <class S>
+S sph_w_g(S r, double h) {  // atmosphere.rs:52-63
+    S q = r / h;
+    double sigma = 8.0 / (M_PI * (h * h * h));
+    if (val(q) < 0.5) return sigma * (6.0 * (q * q * q - q * q) + 1.0);
+    if (val(q) < 1.0) {
+        S t = 1.0 - q;
+        return sigma * 2.0 * (t * t * t);
+    }
+    return S(0.0);
+}
+template <class S>
+S sph_dw_g(S r, double h) {  // atmosphere.rs:65-75
+    S q = r / h;
+    double h2 = h * h;
+    double sigma = 8.0 / (M_PI * (h2 * h2));
+    if (val(q) < 0.5) return sigma * (18.0 * q * q - 12.0 * q);
+    if (val(q) < 1.0) return sigma * -6.0 * (1.0 - q) * (1.0 - q);
+    return S(0.0);
+}
+template <class S>
+struct V3G {
+    S x, y, z;
+};
+template <class S>
+V3G<S> sub(const V3G<S>& a, const V3G<S>& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class S>
+S dotg(const V3G<S>& a, const V3G<S>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+template <class S>
+struct HydroG {
+    std::vector<V3G<S>> pos, vel;
+    std::vector<S> mass, u, rho;
+    std::vector<double> h;
+    std::vector<uint32_t> mat;
+    std::vector<TillG<S>> eos;
+    double softening = 0.0, av_alpha = 1.0, av_beta = 2.0;
+    int gravity_mode = 1;  // 1 direct, 2 none
+
+    size_t n() const { return pos.size(); }
+    std::vector<V3> values() const {
+        std::vector<V3> p(n());
+        for (size_t i = 0; i < n(); ++i) p[i] = V3{val(pos[i].x), val(pos[i].y), val(pos[i].z)};
+        return p;
+    }
+    double max_h() const {
+        double c = 0.0;
+        for (double x : h) c = std::max(c, x);
+        return c;
+    }
+    void compute_density() {  // hydrostatic.rs:338-355
+        const size_t N = n();
+        std::vector<V3> pv = values();
+        Grid grid;
+        grid.build(pv.data(), N, max_h());
+        std::vector<S> r(N);
+        for (size_t i = 0; i < N; ++i) r[i] = mass[i] * sph_w_g(S(0.0), h[i]);
+        grid.for_each_pair(pv.data(), [&](size_t i, size_t j) {
+            V3G<S> d = sub(pos[i], pos[j]);
+            S rr = g_sqrt(dotg(d, d));
+            double hij = 0.5 * (h[i] + h[j]);
+            if (val(rr) < hij) {
+                S w = sph_w_g(rr, hij);
+                r[i] += mass[j] * w;
+                r[j] += mass[i] * w;
+            }
+        });
+        rho.swap(r);
+    }
+    void forces(std::vector<V3G<S>>& acc, std::vector<S>& dudt) const {  // hydrostatic.rs:401-446
+        const size_t N = n();
+        acc.assign(N, V3G<S>{S(0.0), S(0.0), S(0.0)});
+        dudt.assign(N, S(0.0));
+        if (gravity_mode == 1) {
+            const double s2 = softening * softening;
+#pragma omp parallel for schedule(dynamic, 16)
+            for (size_t i = 0; i < N; ++i) {
+                V3G<S> a{S(0.0), S(0.0), S(0.0)};
+                for (size_t j = 0; j < N; ++j) {
+                    if (i == j) continue;
+                    V3G<S> d = sub(pos[j], pos[i]);
+                    S r2 = dotg(d, d) + s2;
+                    S g = G64 * mass[j] / (r2 * g_sqrt(r2));
+                    a.x += d.x * g;
+                    a.y += d.y * g;
+                    a.z += d.z * g;
+                }
+                acc[i] = a;
+            }
+        }
+        std::vector<S> p(N), c(N);
+        for (size_t i = 0; i < N; ++i) {
+            p[i] = till_pressure_g(eos[mat[i]], rho[i], u[i]);
+            c[i] = g_sqrt(till_c2_g(eos[mat[i]], rho[i], u[i]));
+        }
+        std::vector<V3> pv = values();
+        Grid grid;
+        grid.build(pv.data(), N, max_h());
+        grid.for_each_pair(pv.data(), [&](size_t i, size_t j) {
+            V3G<S> dpos = sub(pos[i], pos[j]);
+            S r = g_sqrt(dotg(dpos, dpos));
+            double hij = 0.5 * (h[i] + h[j]);
+            if (val(r) >= hij || val(r) < 1.0e-9) return;
+            V3G<S> dvel = sub(vel[i], vel[j]);
+            S vr = dotg(dvel, dpos);
+            S pi_ij(0.0);
+            if (val(vr) < 0.0) {
+                S mu = hij * vr / (r * r + 0.01 * hij * hij);
+                S c_bar = 0.5 * (c[i] + c[j]);
+                S rho_bar = 0.5 * (rho[i] + rho[j]);
+                pi_ij = (-(av_alpha * c_bar * mu) + av_beta * mu * mu) / rho_bar;
+            }
+            S coeff = p[i] / (rho[i] * rho[i]) + p[j] / (rho[j] * rho[j]) + pi_ij;
+            S dw = sph_dw_g(r, hij);
+            V3G<S> grad{dpos.x / r * dw, dpos.y / r * dw, dpos.z / r * dw};
+            acc[i].x += grad.x * (-(coeff * mass[j]));
+            acc[i].y += grad.y * (-(coeff * mass[j]));
+            acc[i].z += grad.z * (-(coeff * mass[j]));
+            acc[j].x += grad.x * (coeff * mass[i]);
+            acc[j].y += grad.y * (coeff * mass[i]);
+            acc[j].z += grad.z * (coeff * mass[i]);
+            S vdg = dotg(dvel, grad);
+            dudt[i] += 0.5 * mass[j] * coeff * vdg;
+            dudt[j] += 0.5 * mass[i] * coeff * vdg;
+        });
+    }
+    void step(double dt) {  // hydrostatic.rs:451-465
+        const size_t N = n();
+        std::vector<V3G<S>> a;
+        std::vector<S> du;
+        compute_density();
+        forces(a, du);
+        for (size_t i = 0; i < N; ++i) {
+            if (!(val(mass[i]) > 0.0)) continue;
+            vel[i].x += a[i].x * (0.5 * dt);
+            vel[i].y += a[i].y * (0.5 * dt);
+            vel[i].z += a[i].z * (0.5 * dt);
+            u[i] = g_max(u[i] + du[i] * (0.5 * dt), 0.0);
+            pos[i].x += vel[i].x * dt;
+            pos[i].y += vel[i].y * dt;
+            pos[i].z += vel[i].z * dt;
+        }
+        compute_density();
+        forces(a, du);
+        for (size_t i = 0; i < N; ++i) {
+            if (!(val(mass[i]) > 0.0)) continue;
+            vel[i].x += a[i].x * (0.5 * dt);
+            vel[i].y += a[i].y * (0.5 * dt);
+            vel[i].z += a[i].z * (0.5 * dt);
+            u[i] = g_max(u[i] + du[i] * (0.5 * dt), 0.0);
+        }
+    }
+};
+
+// The smoothed disk-mass loss: sum_i m_i sig(k_r (r_i / R - r_cut)) sig(k_j (j_i / sqrt(G M r_i) - 1)) about
+// the centre of mass (same definition as the device loss).
+template <class S>
+S disk_loss(const HydroG<S>& hb, const double* lp) {
+    const double R = lp[0], r_cut = lp[1], k_r = lp[2], k_j = lp[3];
+    const double ax[3] = {lp[4], lp[5], lp[6]};
+    S M(0.0), X(0.0), Y(0.0), Z(0.0), VX(0.0), VY(0.0), VZ(0.0);
+    for (size_t i = 0; i < hb.n(); ++i) {
+        M += hb.mass[i];
+        X += hb.mass[i] * hb.pos[i].x;
+        Y += hb.mass[i] * hb.pos[i].y;
+        Z += hb.mass[i] * hb.pos[i].z;
+        VX += hb.mass[i] * hb.vel[i].x;
+        VY += hb.mass[i] * hb.vel[i].y;
+        VZ += hb.mass[i] * hb.vel[i].z;
+    }
+    X = X / M; Y = Y / M; Z = Z / M; VX = VX / M; VY = VY / M; VZ = VZ / M;
+    S L(0.0);
+    for (size_t i = 0; i < hb.n(); ++i) {
+        if (!(val(hb.mass[i]) > 0.0)) continue;
+        S rx = hb.pos[i].x - X, ry = hb.pos[i].y - Y, rz = hb.pos[i].z - Z;
+        S wx = hb.vel[i].x - VX, wy = hb.vel[i].y - VY, wz = hb.vel[i].z - VZ;
+        S r = g_sqrt(rx * rx + ry * ry + rz * rz);
+        if (!(val(r) > 0.0)) continue;
+        S j = ax[0] * (ry * wz - rz * wy) + ax[1] * (rz * wx - rx * wz) + ax[2] * (rx * wy - ry * wx);
+        S jc = g_sqrt(G64 * M * r);
+        S s1 = 1.0 / (1.0 + g_exp(-(k_r * (r / R - r_cut))));
+        S s2 = 1.0 / (1.0 + g_exp(-(k_j * (j / jc - 1.0))));
+        L += hb.mass[i] * s1 * s2;
+    }
+    return L;
+}
+
+
 extern "C" {
 
 // ---- bbox + Morton (bit-exact f32) ---------------------------------------------------------------
@@ -1008,4 +1290,53 @@ void orc_angular_momentum(uint64_t n, const double* pos3, const double* vel3, co
     }
 }
 
+
+// Loss after `steps` KDK steps (dts given) and its derivative along 3 state tangents (tan_*[k], k < 3) and
+// the Tillotson a, b, A, B of material `eos_mat` (partials 3..6).
+void orc_dual_gradient(uint64_t n, const double* pos3, const double* vel3, const double* mass, const double* u,
+                       const double* h, const uint32_t* mat, uint32_t n_mat, const double* eos10, double soft,
+                       int gravity_mode, const double* dts, uint32_t steps, const double* tan_pos,
+                       const double* tan_vel, const double* tan_mass, uint32_t eos_mat, const double* loss_params,
+                       double* loss_out, double* grad7) {
+    HydroG<Dl> hb;
+    hb.pos.resize(n);
+    hb.vel.resize(n);
+    hb.mass.resize(n);
+    hb.u.resize(n);
+    hb.h.assign(h, h + n);
+    hb.mat.assign(mat, mat + n);
+    hb.softening = soft;
+    hb.gravity_mode = gravity_mode;
+    for (uint64_t i = 0; i < n; ++i) {
+        Dl* pc[3] = {&hb.pos[i].x, &hb.pos[i].y, &hb.pos[i].z};
+        Dl* vc[3] = {&hb.vel[i].x, &hb.vel[i].y, &hb.vel[i].z};
+        for (int c = 0; c < 3; ++c) {
+            *pc[c] = Dl(pos3[3 * i + c]);
+            *vc[c] = Dl(vel3[3 * i + c]);
+            for (int k = 0; k < 3; ++k) {
+                pc[c]->d[k] = tan_pos[(k * n + i) * 3 + c];
+                vc[c]->d[k] = tan_vel[(k * n + i) * 3 + c];
+            }
+        }
+        hb.mass[i] = Dl(mass[i]);
+        for (int k = 0; k < 3; ++k) hb.mass[i].d[k] = tan_mass[k * n + i];
+        hb.u[i] = Dl(u[i]);
+    }
+    hb.eos.resize(n_mat);
+    for (uint32_t m = 0; m < n_mat; ++m) {
+        const double* q = eos10 + 10 * m;
+        TillG<Dl>& t = hb.eos[m];
+        t = TillG<Dl>{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9]};
+        if (m == eos_mat) {
+            t.a.d[3] = 1.0;
+            t.b.d[4] = 1.0;
+            t.cap_a.d[5] = 1.0;
+            t.cap_b.d[6] = 1.0;
+        }
+    }
+    for (uint32_t s = 0; s < steps; ++s) hb.step(dts[s]);
+    Dl L = disk_loss(hb, loss_params);
+    *loss_out = L.v;
+    for (int k = 0; k < kDualN; ++k) grad7[k] = L.d[k];
+}
 }  // extern "C"

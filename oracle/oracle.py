@@ -32,6 +32,8 @@ _SIGS = {
     "orc_get_threads": (ctypes.c_int, []),
     "orc_direct_gravity_f64": (None, [_P, _P, _U64, _D, _P]),
     "orc_lbvh_accel_f64": (None, [_P, _P, _U64, _D, _D, _P]),
+    "orc_dual_gradient": (None, [_U64, _P, _P, _P, _P, _P, _P, ctypes.c_uint32, _P, _D, ctypes.c_int, _P,
+                                 ctypes.c_uint32, _P, _P, _P, ctypes.c_uint32, _P, _P, _P]),
     "orc_octree_accel": (None, [_P, _P, _U64, _D, _D, _P]),
     "orc_octree_potential": (None, [_P, _P, _U64, _D, _D, _P]),
     "orc_grid_pairs": (_U64, [_P, _U64, _D, _P, _U64]),
@@ -173,6 +175,27 @@ def lbvh_accel(pos, mass, soft, theta=0.5):
     acc = np.zeros_like(p)
     lib().orc_lbvh_accel_f64(_p(p), _p(c64(mass)), p.shape[0], float(soft), float(theta), _p(acc))
     return acc
+
+
+def dual_gradient(pos, vel, mass, u, h, mat, eos64, soft, dts, tangents, eos_mat, loss_params,
+                  gravity="direct"):
+    """Loss after len(dts) steps of the f64 CPU step and its exact derivative (forward-mode duals) along the
+    three state tangents [(dpos, dvel, dmass)] and material eos_mat's Tillotson a, b, A, B. Returns
+    (loss, grad[7])."""
+    n = len(mass)
+    tp = np.ascontiguousarray(np.stack([t[0] for t in tangents]), np.float64)
+    tv = np.ascontiguousarray(np.stack([t[1] for t in tangents]), np.float64)
+    tm = np.ascontiguousarray(np.stack([t[2] for t in tangents]), np.float64)
+    eos = c64(eos64).reshape(-1, 10)
+    d = c64(dts)
+    lp = c64(loss_params)
+    out = np.zeros(1)
+    grad = np.zeros(7)
+    lib().orc_dual_gradient(n, _p(c64(pos)), _p(c64(vel)), _p(c64(mass)), _p(c64(u)), _p(c64(h)),
+                            _p(np.ascontiguousarray(mat, np.uint32)), eos.shape[0], _p(eos), float(soft),
+                            {"direct": 1, "none": 2}[gravity], _p(d), d.shape[0], _p(tp), _p(tv), _p(tm),
+                            int(eos_mat), _p(lp), _p(out), _p(grad))
+    return out[0], grad
 
 
 # ---- f64 physics ----
