@@ -29,7 +29,9 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                                                      const float4* __restrict__ nQ0, const float2* __restrict__ nQ1,
                                                      float soft2, float4* __restrict__ sgrav,
                                                      unsigned long long* counters, const uint2* __restrict__ groups,
-                                                     uint32_t* scal, uint32_t count_slot) {
+                                                     uint32_t* scal, uint32_t count_slot,
+                                                     const uint32_t* __restrict__ sinst,
+                                                     const uint32_t* __restrict__ iroot) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 cA[kBuf];
     __shared__ float4 cQ0[QUAD ? kBuf : 1];
@@ -169,7 +171,8 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
 
         int sp = 1, nc = 0, np = 0;
         __syncthreads();
-        if (lane == 0) stk[0] = 0;  // root (a single leaf when L == 1)
+        // root (a single leaf when L == 1), or the group's instance subtree in an ensemble
+        if (lane == 0) stk[0] = sinst ? iroot[sinst[grp.x]] : 0u;
         __syncthreads();
         while (sp > 0) {
             int k = min(kWave, sp);
@@ -349,7 +352,7 @@ void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s) {
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(waves), dim3(kWave), 0, s, d.spos, d.n, d.K, d.L, d.nG, d.nQ0, d.nQ1, p.soft2,
                            d.sgrav, d.counters, own ? d.ggroups : d.groups, d.scal,
-                           (uint32_t)(own ? kGroupsGrav : kGroupsSph));
+                           (uint32_t)(own ? kGroupsGrav : kGroupsSph), d.sinst(), d.iroot);
     };
     const int sel = (p.grav_ppl == 2 ? 4 : 0) | (p.quad ? 2 : 0) | (p.mixed ? 1 : 0);
     switch (sel) {

@@ -91,6 +91,9 @@ void free_all(isph_ctx* c) {
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
+    void* eptrs[] = {d.inst, d.mcode, d.lkey, d.ioff, d.ibox, d.iroot, d.isig, d.idt, d.itime};
+    for (void* q : eptrs)
+        if (q) (void)hipFree(q);
     const Adj& a = c->adj;
     void* aptrs[] = {a.xb, a.vb, a.ub, a.mb, a.ab, a.sa, a.sxb, a.svb, a.srb, a.sqb, a.scb, a.nbar, a.pbar, a.red};
     for (void* q : aptrs)
@@ -232,6 +235,11 @@ void set_leaves(isph_ctx* c, uint32_t n) {
 void build_sorted(isph_ctx* c, const Dev& d, const float4* pos, const float* mass, bool with_sph) {
     Phase ph(c, ISPH_PHASE_SORT);
     launch_reset_scalars(d, c->stream);
+    if (d.ninst > 1) {  // ensemble: per-instance boxes, (instance, code) order
+        launch_reset_seg(d, c->stream, 0);
+        launch_sorted_seg(d, pos, mass, c->stream);
+        return;
+    }
     launch_bbox(d, pos, c->stream);
     launch_morton(d, pos, c->stream);
     launch_radix_sort(d, c->stream);
@@ -248,6 +256,7 @@ void build_tree(isph_ctx* c, const Dev& d) {
     {
         Phase ph(c, ISPH_PHASE_TREE);
         launch_karras(d, c->stream);
+        if (d.ninst > 1) launch_inst_roots(d, c->stream);
     }
     Phase ph(c, ISPH_PHASE_UPSWEEP);
     launch_upsweep(d, c->p, c->stream);
@@ -281,6 +290,7 @@ void eval(isph_ctx* c, bool rebuild) {
     } else {
         Phase ph(c, ISPH_PHASE_DENSITY);
         launch_reset_scalars(d, c->stream, 1);
+        if (d.ninst > 1) launch_reset_seg(d, c->stream, 1);
         launch_gather_vel(d, c->stream);
         launch_eos(d, c->stream);
     }
@@ -293,14 +303,18 @@ void kdk(isph_ctx* c, bool cfl_mode, float cfl) {
     eval(c, !(c->reuse_enabled && c->reuse_valid));
     {
         Phase ph(c, ISPH_PHASE_INTEGRATE);
-        if (cfl_mode) launch_set_dt_cfl(d, cfl, c->stream);
+        if (cfl_mode) {
+            if (d.ninst > 1) launch_set_dt_cfl_seg(d, cfl, c->stream);
+            else launch_set_dt_cfl(d, cfl, c->stream);
+        }
         launch_kick_drift(d, c->stream);
     }
     eval(c, true);
     {
         Phase ph(c, ISPH_PHASE_INTEGRATE);
         launch_kick(d, c->stream);
-        launch_advance_time(d, c->stream);
+        if (d.ninst > 1) launch_advance_time_seg(d, c->stream);
+        else launch_advance_time(d, c->stream);
     }
     c->reuse_valid = true;
 }
@@ -524,6 +538,7 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
         if (ne > 0) widen_eos(h, eos, ne);
         h->n_eos = (uint32_t)ne;
         set_leaves(h, (uint32_t)take);
+        h->d.ninst = 1;
         h->softening = softening;
         h->p.soft2 = softening * softening;
         if (h->precision == ISPH_PRECISION_MIXED && take > 0) {
@@ -534,6 +549,69 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
         h->rb_inflight = false;
         HIP_OK(hipStreamSynchronize(h->stream));
         return (int)take;
+    });
+}
+
+int isph_upload_ensemble(isph_t* h, const isph_particle* particles, const uint32_t* offsets, uint32_t n_inst,
+                         const isph_eos* eos, size_t n_eos, float softening) {
+    if (!h) return ISPH_EINVAL;
+    if (!offsets || n_inst == 0 || n_inst > 256) return fail(h, ISPH_EINVAL, "ensemble of 1..256 instances expected");
+    if (offsets[0] != 0) return fail(h, ISPH_EINVAL, "offsets[0] must be 0");
+    for (uint32_t k = 0; k < n_inst; ++k)
+        if (offsets[k + 1] <= offsets[k]) return fail(h, ISPH_EINVAL, "every instance needs at least one particle");
+    const size_t n = offsets[n_inst];
+    if (n > h->d.cap) return fail(h, ISPH_ERANGE, "ensemble exceeds the handle capacity");
+    if (h->d.K != 1) return fail(h, ISPH_EINVAL, "ensembles use leaf size 1");
+    if (n_inst > 1 && h->gravity_mode == ISPH_GRAVITY_DIRECT)
+        return fail(h, ISPH_EINVAL, "ensembles use tree (or no) gravity: the direct sum would couple instances");
+    int rc = isph_upload(h, particles, n, eos, n_eos, softening);
+    if (rc < 0) return rc;
+    return guarded(h, [&]() -> int {
+        Dev& d = h->d;
+        void* old[] = {d.inst, d.mcode, d.lkey, d.ioff, d.ibox, d.iroot, d.isig, d.idt, d.itime};
+        for (void* q : old)
+            if (q) (void)hipFree(q);
+        d.inst = d.mcode = d.ioff = d.ibox = d.iroot = d.isig = nullptr;
+        d.lkey = nullptr;
+        d.idt = nullptr;
+        d.itime = nullptr;
+        d.ninst = 1;
+        if (n_inst == 1) return (int)n;
+        dalloc(&d.inst, d.cap);
+        dalloc(&d.mcode, d.cap);
+        dalloc(&d.lkey, d.cap);
+        dalloc(&d.ioff, n_inst + 1);
+        dalloc(&d.ibox, 6 * (size_t)n_inst);
+        dalloc(&d.iroot, n_inst);
+        dalloc(&d.isig, n_inst);
+        dalloc(&d.idt, n_inst);
+        dalloc(&d.itime, n_inst);
+        std::vector<uint32_t> inst(n);
+        for (uint32_t k = 0; k < n_inst; ++k)
+            for (uint32_t i = offsets[k]; i < offsets[k + 1]; ++i) inst[i] = k;
+        HIP_OK(hipMemcpy(d.inst, inst.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(d.ioff, offsets, (n_inst + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIP_OK(hipMemset(d.idt, 0, n_inst * sizeof(float)));
+        HIP_OK(hipMemset(d.itime, 0, n_inst * sizeof(double)));
+        d.ninst = n_inst;
+        h->reuse_valid = false;
+        return (int)n;
+    });
+}
+
+int isph_ensemble_times(isph_t* h, double* t, float* dt, uint32_t cap) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (h->d.ninst <= 1) {
+            if (cap >= 1 && t) HIP_OK(hipMemcpy(t, h->d.time, sizeof(double), hipMemcpyDeviceToHost));
+            if (cap >= 1 && dt) HIP_OK(hipMemcpy(dt, h->d.dt, sizeof(float), hipMemcpyDeviceToHost));
+            return 1;
+        }
+        const size_t k = std::min<size_t>(cap, h->d.ninst);
+        if (t && k) HIP_OK(hipMemcpy(t, h->d.itime, k * sizeof(double), hipMemcpyDeviceToHost));
+        if (dt && k) HIP_OK(hipMemcpy(dt, h->d.idt, k * sizeof(float), hipMemcpyDeviceToHost));
+        return (int)h->d.ninst;
     });
 }
 
@@ -576,6 +654,8 @@ int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int q
     return guarded(h, [&]() -> int {
         if (mode < ISPH_GRAVITY_TREE || mode > ISPH_GRAVITY_NONE) return fail(h, ISPH_EINVAL, "bad gravity mode");
         if (!(theta > 0.0f) || leaf_size < 1 || leaf_size > 64) return fail(h, ISPH_EINVAL, "bad theta or leaf size");
+        if (h->d.ninst > 1 && (leaf_size != 1 || mode == ISPH_GRAVITY_DIRECT))
+            return fail(h, ISPH_EINVAL, "ensembles use leaf size 1 and tree (or no) gravity");
         h->gravity_mode = mode;
         h->p.theta = theta;
         h->p.quad = quadrupole ? 1 : 0;
@@ -645,7 +725,8 @@ int isph_encode_kdk(isph_t* h, uint32_t substeps) {
         Dev saved = h->d;
         h->d = integ_view(h);
         try {
-            launch_set_dt_value(h->d, h->dt_set, h->stream);
+            if (h->d.ninst > 1) launch_set_dt_value_seg(h->d, h->dt_set, h->stream);
+            else launch_set_dt_value(h->d, h->dt_set, h->stream);
             for (uint32_t s = 0; s < substeps; ++s) kdk(h, false, 0.0f);
         } catch (...) {
             h->d = saved;
@@ -775,6 +856,7 @@ int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass
         g.n = (uint32_t)n;
         g.L = (g.n + g.K - 1) / g.K;
         g.sh = nullptr;  // no smoothing lengths: the upsweep stores hmax = 0
+        g.ninst = 1;     // the gravity API takes one system
         StepParams saved = h->p;
         h->p.soft2 = (float)(softening * softening);
         try {
@@ -806,6 +888,7 @@ int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass
 int isph_diagnostics(isph_t* h, int with_pe, double* out7) {
     return guarded(h, [&]() -> int {
         if (!out7) return fail(h, ISPH_EINVAL, "NULL output");
+        if (with_pe && h->d.ninst > 1) return fail(h, ISPH_EINVAL, "the pair potential energy is per system");
         for (int k = 0; k < 7; ++k) out7[k] = 0.0;
         if (h->d.n == 0) return ISPH_OK;
         HIP_OK(hipSetDevice(h->device));
@@ -1010,6 +1093,7 @@ int isph_vjp_run(isph_t* h, float cfl, const float* dts_in, uint32_t steps, uint
     return guarded(h, [&]() -> int {
         if (!loss) return fail(h, ISPH_EINVAL, "NULL loss");
         if (h->precision != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "the adjoint needs the mixed form");
+        if (h->d.ninst > 1) return fail(h, ISPH_EINVAL, "the adjoint takes one system, not an ensemble");
         if (h->gravity_mode == ISPH_GRAVITY_TREE && (h->p.quad || h->p.grav_ppl != 1))
             return fail(h, ISPH_EINVAL, "the adjoint's tree gravity is the monopole walk with one particle per lane");
         if (steps == 0 || checkpoint_every == 0) return fail(h, ISPH_EINVAL, "steps and checkpoint_every must be > 0");

@@ -248,6 +248,14 @@ void launch_radix_sort(const Dev& d, hipStream_t s) {
         std::swap(vin, vout);
     }
 }
+void launch_radix_pass(const Dev& d, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                       int shift, hipStream_t s) {
+    uint32_t nb = cdiv(d.n, kSortTile);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, kin, d.n, shift, d.hist, nb);
+    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, d.hist, 256u * nb);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, kin, vin, kout, vout, d.n, shift, d.hist,
+                       nb);
+}
 void launch_exclusive_scan(uint32_t* data, uint32_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, data, count);
 }

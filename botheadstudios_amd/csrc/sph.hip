@@ -53,7 +53,9 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                                                 float* __restrict__ srho, float* __restrict__ rho_orig,
                                                 double* __restrict__ srho64, uint32_t* scal,
                                                 unsigned long long* counters, const uint2* __restrict__ groups,
-                                                const uint2* __restrict__ nrange, uint32_t expand) {
+                                                const uint2* __restrict__ nrange, uint32_t expand,
+                                                const uint32_t* __restrict__ sinst,
+                                                const uint32_t* __restrict__ iroot) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
         }
         rho += acc;
         tested += cnt;
-    });
+    }, sinst ? iroot[sinst[grp.x]] : 0u);
     const bool live = valid && me.w > 0.f;  // retired (mass 0) particles keep their density (:185-189)
     double rho_i;
     if (live) {
@@ -148,9 +150,30 @@ __global__ __launch_bounds__(kEosThreads) void k_eos(const float* __restrict__ s
                                                      const Till* __restrict__ eos, const float* __restrict__ srho,
                                                      float* __restrict__ spr, float* __restrict__ scs,
                                                      const double* __restrict__ srho64, double* __restrict__ spr64,
-                                                     double* __restrict__ scs64, uint32_t* scal) {
+                                                     double* __restrict__ scs64, uint32_t* scal,
+                                                     const uint32_t* __restrict__ sinst, uint32_t* isig) {
     __shared__ float red[kEosThreads / kWave];
     float sig = INFINITY;
+    if (sinst) {  // ensemble: a Courant minimum per instance (waves inside one instance reduce first)
+        for (uint32_t base = blockIdx.x * kEosThreads; base < n; base += gridDim.x * kEosThreads) {
+            const uint32_t t = base + threadIdx.x;
+            const bool valid = t < n;
+            float v = INFINITY;
+            if (valid) {
+                const double rho = srho64 ? srho64[t] : (double)srho[t];
+                v = eos_point(t, rho, sh[t], svel[t], smat[t], eos, spr, scs, spr64, scs64);
+            }
+            const uint32_t k = valid ? sinst[t] : 0xffffffffu;
+            const uint32_t k0 = __builtin_amdgcn_readfirstlane(k);
+            if (__ballot(valid && k != k0) == 0ull) {
+                v = wave_min(v);
+                if (lane_id() == 0 && k0 != 0xffffffffu && v < INFINITY) atomicMin(&isig[k0], __float_as_uint(v));
+            } else if (valid && v < INFINITY) {
+                atomicMin(&isig[k], __float_as_uint(v));
+            }
+        }
+        return;
+    }
     for (uint32_t t = blockIdx.x * kEosThreads + threadIdx.x; t < n; t += gridDim.x * kEosThreads) {
         const double rho = srho64 ? srho64[t] : (double)srho[t];
         sig = fminf(sig, eos_point(t, rho, sh[t], svel[t], smat[t], eos, spr, scs, spr64, scs64));
@@ -177,7 +200,8 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
                                               const double* __restrict__ scs64, float av_alpha, float av_beta, unsigned long long* counters,
                                               const uint2* __restrict__ groups, uint32_t* scal,
                                               const uint2* __restrict__ nrange, uint32_t expand,
-                                              double* __restrict__ acc64) {
+                                              double* __restrict__ acc64, const uint32_t* __restrict__ sinst,
+                                              const uint32_t* __restrict__ iroot) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, mass
@@ -296,7 +320,7 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
             }
         }
         tested += cnt;
-    });
+    }, sinst ? iroot[sinst[grp.x]] : 0u);
     if (F64) {
         ax = (float)ax64;
         ay = (float)ay64;
@@ -324,10 +348,11 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
 __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, float4* velu,
                                                     const float* __restrict__ mass, const float4* __restrict__ acc,
                                                     const float* __restrict__ dtp, double* pos64, double* vel64,
-                                                    const double* __restrict__ acc64) {
+                                                    const double* __restrict__ acc64,
+                                                    const uint32_t* __restrict__ inst, const float* __restrict__ idt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !(mass[i] > 0.f)) return;
-    const float dt = *dtp;
+    const float dt = inst ? idt[inst[i]] : *dtp;
     const float4 a = acc[i];
     float4 v = velu[i];
     float4 p = posh[i];
@@ -364,10 +389,11 @@ __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, fl
 
 __global__ __launch_bounds__(256) void k_kick(uint32_t n, float4* velu, const float* __restrict__ mass,
                                               const float4* __restrict__ acc, const float* __restrict__ dtp,
-                                              double* vel64, const double* __restrict__ acc64) {
+                                              double* vel64, const double* __restrict__ acc64,
+                                              const uint32_t* __restrict__ inst, const float* __restrict__ idt) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !(mass[i] > 0.f)) return;
-    const float dt = *dtp;
+    const float dt = inst ? idt[inst[i]] : *dtp;
     const float4 a = acc[i];
     float4 v = velu[i];
     v.w = fmaxf(v.w + a.w * (0.5f * dt), 0.0f);
@@ -551,17 +577,17 @@ void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     if (p.mixed)
         hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR, d.nB,
                            d.nC, d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
-                           (uint32_t)p.sph_expand);
+                           (uint32_t)p.sph_expand, d.sinst(), d.iroot);
     else
         hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR,
                            d.nB, d.nC, d.vals, d.srho, d.rho, (double*)nullptr, d.scal, d.counters, d.groups,
-                           d.nrange, (uint32_t)p.sph_expand);
+                           d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot);
     launch_eos(d, s);
 }
 void launch_eos(const Dev& d, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kEosBlocks, cdiv(d.n, kEosThreads));
     hipLaunchKernelGGL(k_eos, dim3(blocks), dim3(kEosThreads), 0, s, d.sh, d.svel, d.smat, d.n, d.eos, d.srho, d.spr,
-                       d.scs, (const double*)d.srho64, d.spr64, d.scs64, d.scal);
+                       d.scs, (const double*)d.srho64, d.spr64, d.scs64, d.scal, d.sinst(), d.isig);
 }
 void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
@@ -569,21 +595,21 @@ void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
                            d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
                            d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
-                           (uint32_t)p.sph_expand, d.acc64);
+                           (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
                            d.scs, d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
                            p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand,
-                           (double*)nullptr);
+                           (double*)nullptr, d.sinst(), d.iroot);
 }
 void launch_kick_drift(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,
-                       d.pos64, d.vel64, d.acc64);
+                       d.pos64, d.vel64, d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt);
 }
 void launch_kick(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.velu, d.mass, d.acc, d.dt, d.vel64,
-                       d.acc64);
+                       d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt);
 }
 void launch_set_dt_cfl(const Dev& d, float cfl, hipStream_t s) {
     hipLaunchKernelGGL(k_set_dt_cfl, dim3(1), dim3(1), 0, s, d.scal, cfl, d.dt);

@@ -68,6 +68,20 @@ struct Dev {
     float4* nQ0 = nullptr; // quadrupole xx, yy, zz, xy
     float2* nQ1 = nullptr; // quadrupole xz, yz
 
+    // ensembles (ninst > 1): instance k owns particles [ioff[k], ioff[k+1]) in upload and sorted order
+    uint32_t ninst = 1;
+    uint32_t* inst = nullptr;   // instance of each particle, upload order
+    uint32_t* mcode = nullptr;  // Morton code against the instance box, upload order
+    uint64_t* lkey = nullptr;   // leaf keys (instance << 30 | code), sorted order
+    uint32_t* ioff = nullptr;   // [ninst + 1]
+    uint32_t* ibox = nullptr;   // [6 ninst] bbox keys
+    uint32_t* iroot = nullptr;  // [ninst] root node of each instance's subtree
+    uint32_t* isig = nullptr;   // [ninst] Courant signal minimum (float bits)
+    float* idt = nullptr;       // [ninst] time step
+    double* itime = nullptr;    // [ninst] simulated time
+    // sorted instance ids (the sort keys after the instance pass), or nullptr for one system
+    const uint32_t* sinst() const { return ninst > 1 ? keys : nullptr; }
+
     // scalars
     // [0..5] bbox keys, [6] Courant signal min (float bits), [7] crossing-node count, [8] SPH walk groups,
     // [12] gravity walk groups, [512..) / [1024..) / [1536..) work-queue areas of the gravity, density and
@@ -164,6 +178,17 @@ void launch_morton(const Dev& d, const float4* pos, hipStream_t s);
 void launch_radix_sort(const Dev& d, hipStream_t s);  // 30-bit keys, 4 passes; result in keys/vals
 void launch_gather(const Dev& d, const float4* pos, const float* mass, hipStream_t s);
 void launch_gather_vel(const Dev& d, hipStream_t s);
+
+void launch_radix_pass(const Dev& d, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                       int shift, hipStream_t s);
+
+// ensembles
+void launch_reset_seg(const Dev& d, hipStream_t s, int mode);
+void launch_sorted_seg(const Dev& d, const float4* pos, const float* mass, hipStream_t s);
+void launch_inst_roots(const Dev& d, hipStream_t s);
+void launch_set_dt_cfl_seg(const Dev& d, float cfl, hipStream_t s);
+void launch_set_dt_value_seg(const Dev& d, float dt, hipStream_t s);
+void launch_advance_time_seg(const Dev& d, hipStream_t s);
 
 void launch_karras(const Dev& d, hipStream_t s);
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s);

@@ -18,16 +18,23 @@ namespace isph {
 constexpr uint32_t kUpChunk = 1024;
 constexpr uint32_t kUpChunk2 = 65536;
 
-__device__ __forceinline__ int delta_codes(const uint32_t* __restrict__ codes, int L, int i, int j) {
+__device__ __forceinline__ int clz_key(uint32_t x) { return __clz(x); }
+__device__ __forceinline__ int clz_key(uint64_t x) { return __clzll(x); }
+
+// Common-prefix length of the index-extended keys (code, index) of leaves i and j; -1 outside [0, L).
+template <class Key>
+__device__ __forceinline__ int delta_codes(const Key* __restrict__ codes, int L, int i, int j) {
     if (j < 0 || j >= L) return -1;
-    uint32_t ci = codes[i], cj = codes[j];
-    if (ci == cj) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clz(ci ^ cj);
+    Key ci = codes[i], cj = codes[j];
+    if (ci == cj) return (int)(8 * sizeof(Key)) + __clz((uint32_t)i ^ (uint32_t)j);
+    return clz_key(ci ^ cj);
 }
 
 // One thread per internal node (bh_gravity.wgsl:134-184). Also clears the climb counters and the root's
-// parent (cs_tree_reset, :122-131, restricted to what is read later).
-__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ codes, uint32_t L,
+// parent (cs_tree_reset, :122-131, restricted to what is read later). Keys are the 30-bit leaf codes, or
+// (instance << 30 | code) for an ensemble, where each instance's leaves then form one subtree.
+template <class Key>
+__global__ __launch_bounds__(256) void k_karras(const Key* __restrict__ codes, uint32_t L,
                                                 uint32_t* __restrict__ left, uint32_t* __restrict__ right,
                                                 uint32_t* __restrict__ parent, uint32_t* __restrict__ ready,
                                                 uint2* __restrict__ nrange, uint2* __restrict__ nlr,
@@ -47,13 +54,13 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cod
     int j = i + l * d;
     int first = min(i, j), last = max(i, j);
     int split = first;
-    uint32_t fc = codes[first], lc = codes[last];
+    Key fc = codes[first], lc = codes[last];
     // findSplit on the index-extended key (Karras 2012). For first/last codes that differ this is exactly
     // the shader's search (:162-172); for an all-equal range the shader's midpoint split (:159-160)
     // disagrees with the index tie-break of determineRange and breaks the tree on runs of >= 3 equal
     // codes, so the extended delta is used there too.
     {
-        int cpfx = (fc == lc) ? delta_codes(codes, Li, first, last) : __clz(fc ^ lc);
+        int cpfx = (fc == lc) ? delta_codes(codes, Li, first, last) : clz_key(fc ^ lc);
         int step = last - first;
         do {
             step = (step + 1) >> 1;
@@ -251,9 +258,35 @@ __global__ __launch_bounds__(1024) void k_upsweep_cross(const uint32_t* __restri
 // the largest smoothing length of the scene (a leaf always qualifies). Both conditions only get easier down
 // the tree, so these nodes partition the leaves into spatially compact, Morton-contiguous particle runs
 // (runs of consecutive indices can straddle a Morton seam and span distant regions).
-__device__ __forceinline__ bool group_ok(uint32_t v, uint32_t n, uint32_t K, uint32_t L, const uint2* nrange,
-                                         const float4* nB, const float4* nC, float lim2, uint32_t maxc,
-                                         uint32_t& start, uint32_t& count) {
+// Instance context of the group construction: the length unit of each node's limit comes from its
+// instance's root (the whole tree for one system), and no group may span two instances.
+struct GroupCtx {
+    const float4* nB;
+    const float4* nC;
+    const uint32_t* sinst;  // sorted instance ids, or nullptr for one system
+    const uint32_t* iroot;
+    const uint32_t* ioff;
+    uint32_t n;
+    float extent;
+    // Largest h of the instance (carried by its root box), or for gravity-only builds without smoothing
+    // lengths the root diagonal / cbrt(n) (about twice the mean spacing).
+    __device__ __forceinline__ float lim2(uint32_t first_particle) const {
+        uint32_t root = 0, cnt = n;
+        if (sinst) {
+            const uint32_t k = sinst[first_particle];
+            root = iroot[k];
+            cnt = ioff[k + 1] - ioff[k];
+        }
+        const float4 b0 = nB[root], c0 = nC[root];
+        const float ex0 = c0.x - b0.x, ey0 = c0.y - b0.y, ez0 = c0.z - b0.z;
+        const float spacing = sqrtf(ex0 * ex0 + ey0 * ey0 + ez0 * ez0) / cbrtf((float)cnt);
+        const float lim = extent * (b0.w > 0.0f ? b0.w : spacing);
+        return lim * lim;
+    }
+};
+
+__device__ __forceinline__ bool group_ok(uint32_t v, uint32_t K, uint32_t L, const uint2* nrange, const GroupCtx& g,
+                                         uint32_t maxc, uint32_t& start, uint32_t& count) {
     uint32_t first, last;
     bool leaf = v >= L - 1;
     if (leaf) {
@@ -264,33 +297,26 @@ __device__ __forceinline__ bool group_ok(uint32_t v, uint32_t n, uint32_t K, uin
         last = r.y;
     }
     start = first * K;
-    uint32_t end = min((last + 1) * K, n);
+    uint32_t end = min((last + 1) * K, g.n);
     count = end - start;
     if (leaf) return true;
     if (count > maxc) return false;
-    float4 b = nB[v], c = nC[v];
+    if (g.sinst && g.sinst[start] != g.sinst[end - 1]) return false;
+    float4 b = g.nB[v], c = g.nC[v];
     float ex = c.x - b.x, ey = c.y - b.y, ez = c.z - b.z;
-    return ex * ex + ey * ey + ez * ez <= lim2;
+    return ex * ex + ey * ey + ez * ez <= g.lim2(start);
 }
 
 // Mark each group at its first particle (gmark[start] = count); gmark is all-zero on entry.
-__global__ __launch_bounds__(256) void k_group_mark(uint32_t n, uint32_t K, uint32_t L, const uint32_t* __restrict__ parent,
-                                                    const uint2* __restrict__ nrange, const float4* __restrict__ nB,
-                                                    const float4* __restrict__ nC, float extent, uint32_t maxc,
+__global__ __launch_bounds__(256) void k_group_mark(uint32_t K, uint32_t L, const uint32_t* __restrict__ parent,
+                                                    const uint2* __restrict__ nrange, GroupCtx g, uint32_t maxc,
                                                     uint32_t* __restrict__ gmark) {
     uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= 2 * L - 1) return;
-    // Length unit of the limit: the scene's largest h (carried by the root box), or, for gravity-only
-    // builds without smoothing lengths, the root diagonal / cbrt(n) (about twice the mean spacing).
-    const float4 b0 = nB[0], c0 = nC[0];
-    const float ex0 = c0.x - b0.x, ey0 = c0.y - b0.y, ez0 = c0.z - b0.z;
-    const float spacing = sqrtf(ex0 * ex0 + ey0 * ey0 + ez0 * ez0) / cbrtf((float)n);
-    const float lim = extent * (b0.w > 0.0f ? b0.w : spacing);
-    const float lim2 = lim * lim;
     uint32_t start, count, ps, pc;
-    if (!group_ok(v, n, K, L, nrange, nB, nC, lim2, maxc, start, count)) return;
+    if (!group_ok(v, K, L, nrange, g, maxc, start, count)) return;
     uint32_t p = parent[v];
-    if (p != kNoParent && group_ok(p, n, K, L, nrange, nB, nC, lim2, maxc, ps, pc)) return;
+    if (p != kNoParent && group_ok(p, K, L, nrange, g, maxc, ps, pc)) return;
     gmark[start] = count;
 }
 
@@ -321,8 +347,9 @@ __global__ __launch_bounds__(256) void k_group_emit(uint32_t n, uint32_t* __rest
 void launch_groups(const Dev& d, const StepParams& p, uint32_t maxc, uint2* out, uint32_t* total, hipStream_t s) {
     uint32_t nodes = 2 * d.L - 1;
     uint32_t nb = (d.n + kWave - 1) / kWave;
-    hipLaunchKernelGGL(k_group_mark, dim3((nodes + 255) / 256), dim3(256), 0, s, d.n, d.K, d.L, d.parent, d.nrange, d.nB,
-                       d.nC, p.group_extent, maxc, d.gflag);
+    GroupCtx g{d.nB, d.nC, d.sinst(), d.iroot, d.ioff, d.n, p.group_extent};
+    hipLaunchKernelGGL(k_group_mark, dim3((nodes + 255) / 256), dim3(256), 0, s, d.K, d.L, d.parent, d.nrange, g,
+                       maxc, d.gflag);
     hipLaunchKernelGGL(k_group_count, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.gflag, d.gscan);
     launch_exclusive_scan(d.gscan, nb, s);
     hipLaunchKernelGGL(k_group_emit, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.gflag, d.gscan, out, total);
@@ -330,8 +357,12 @@ void launch_groups(const Dev& d, const StepParams& p, uint32_t maxc, uint2* out,
 
 void launch_karras(const Dev& d, hipStream_t s) {
     uint32_t threads = d.L > 1 ? d.L - 1 : 1;
-    hipLaunchKernelGGL(k_karras, dim3((threads + 255) / 256), dim3(256), 0, s, d.lcode, d.L, d.left, d.right, d.parent,
-                       d.ready, d.nrange, d.nLR, d.cross, d.scal);
+    if (d.ninst > 1)
+        hipLaunchKernelGGL(k_karras<uint64_t>, dim3((threads + 255) / 256), dim3(256), 0, s, d.lkey, d.L, d.left,
+                           d.right, d.parent, d.ready, d.nrange, d.nLR, d.cross, d.scal);
+    else
+        hipLaunchKernelGGL(k_karras<uint32_t>, dim3((threads + 255) / 256), dim3(256), 0, s, d.lcode, d.L, d.left,
+                           d.right, d.parent, d.ready, d.nrange, d.nLR, d.cross, d.scal);
 }
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s) {
     TreeView tv{d.left, d.right, d.parent, d.nrange, d.ready, d.nG, d.nA, d.nB, d.nC, d.nQ0, d.nQ1};

@@ -12,21 +12,23 @@ constexpr int kSphStack = ISPH_SPH_STACK;  // LDS traversal stack (entries); the
 constexpr int kSphStackMargin = 128;       // > maximum Karras depth
 constexpr int kSphBuf = 128;               // candidate list (two flushes of 64)
 
-// Wave-cooperative neighbour traversal. `flush(cnt)` is called with cand[0..cnt) holding candidate sorted
-// indices (cnt <= 64); it must evaluate them for the calling lane.
+// Wave-cooperative neighbour traversal from node `root` (the whole tree, or an ensemble instance's
+// subtree). `flush(cnt)` is called with cand[0..cnt) holding candidate sorted indices (cnt <= 64); it must
+// evaluate them for the calling lane.
 template <class Flush>
 __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, uint32_t n, uint32_t K, uint32_t L,
                                                const uint2* __restrict__ nlr, const uint2* __restrict__ nrange,
                                                uint32_t expand,
                                                const float4* __restrict__ nB, const float4* __restrict__ nC,
                                                float gx0, float gy0, float gz0, float gx1, float gy1, float gz1,
-                                               float hgmax, unsigned long long* stats, Flush&& flush) {
+                                               float hgmax, unsigned long long* stats, Flush&& flush,
+                                               uint32_t root = 0) {
     const int lane = threadIdx.x;
     const uint32_t leaf0 = L - 1;
     int sp = 1, nq = 0;
     unsigned long long rounds = 0, popped = 0;
     int max_sp = 1;
-    if (lane == 0) stk[0] = 0;
+    if (lane == 0) stk[0] = root;
     __syncthreads();
     auto shift = [&](int cnt) {
         __syncthreads();

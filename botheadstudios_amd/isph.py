@@ -55,6 +55,8 @@ _SIGS = {
     "isph_probe_morton": (ctypes.c_int, [_P, _P, _P, _P]),
     "isph_probe_sort": (ctypes.c_int, [_P, _P, _P]),
     "isph_probe_tree": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "isph_upload_ensemble": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32, _P, ctypes.c_size_t, ctypes.c_float]),
+    "isph_ensemble_times": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32]),
     "isph_loss_value": (ctypes.c_int, [_P, _P, _P]),
     "isph_set_state_f64": (ctypes.c_int, [_P, _P, _P]),
     "isph_vjp_run": (ctypes.c_int, [_P, ctypes.c_float, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
@@ -202,6 +204,27 @@ class GpuSph:
         rho = np.zeros(n, np.float32)
         self._check(self._lib.isph_force_eval(self._h, _ptr(acc), _ptr(rho)))
         return acc[:, :3].copy(), acc[:, 3].copy(), rho
+
+    # ---- ensembles ----
+    def upload_ensemble(self, instances, eos, softening):
+        """Upload a list of record arrays as independent instances of one ensemble; returns the offsets."""
+        recs = [np.ascontiguousarray(r, dtype=PARTICLE_DTYPE) for r in instances]
+        offsets = np.zeros(len(recs) + 1, np.uint32)
+        offsets[1:] = np.cumsum([len(r) for r in recs])
+        allp = np.concatenate(recs)
+        e = np.ascontiguousarray(eos, dtype=EOS_DTYPE)
+        rc = self._lib.isph_upload_ensemble(self._h, _ptr(allp), _ptr(offsets), len(recs), _ptr(e), len(e),
+                                            float(softening))
+        self._check(rc)
+        self._n_inst = len(recs)
+        return offsets
+
+    def ensemble_times(self):
+        k = getattr(self, "_n_inst", 1)
+        t = np.zeros(k, np.float64)
+        dt = np.zeros(k, np.float32)
+        self._check(self._lib.isph_ensemble_times(self._h, _ptr(t), _ptr(dt), k))
+        return t, dt
 
     # ---- adjoint ----
     def loss_value(self, loss):

@@ -97,6 +97,16 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
 int isph_set_eos_f64(isph_t* h, const double* params10, size_t n_eos);
 /* GpuSph::count gpu_sph.rs:1449 */
 uint32_t isph_count(const isph_t* h);
+/* Ensemble upload (no reference counterpart: the batching form). `n_inst` independent systems share one
+ * handle, one EOS table, softening and step controls: instance k owns particles [offsets[k], offsets[k+1])
+ * (offsets[0] = 0, each instance non-empty, offsets[n_inst] <= capacity, n_inst <= 256). Every launch of a
+ * step advances all instances; each has its own Morton box, its own subtree of the tree (walks never
+ * cross instances), its own Courant dt and simulated time. Leaf size 1, tree or no gravity; the adjoint
+ * and the pair potential energy take one system. Returns the particle count. */
+int isph_upload_ensemble(isph_t* h, const isph_particle* particles, const uint32_t* offsets, uint32_t n_inst,
+                         const isph_eos* eos, size_t n_eos, float softening);
+/* Per-instance simulated time and last dt (up to `cap` instances); returns the instance count. */
+int isph_ensemble_times(isph_t* h, double* t, float* dt, uint32_t cap);
 /* GpuSph::set_dt(queue, dt, damp) gpu_sph.rs:1343 */
 int isph_set_dt(isph_t* h, float dt, float damp);
 /* GpuSph::set_av(queue, alpha, beta) gpu_sph.rs:1353 */
