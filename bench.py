@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--expand", type=int, default=4, help="SPH walks take subtrees of <= this many particles whole")
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance ensemble measurement")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--adjoint-n", type=int, default=262144)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20, help="checkpoint interval of the reverse pass")
@@ -179,6 +182,67 @@ def run_adjoint(args):
             "granite_dloss_d_a_b_A_B": [float(x) for x in g]}
 
 
+def batch_scenes(count, seed=0xC4):
+    """The batching workload: `count` impacts, N log-uniform in [16384, 262144] (7:1 target:impactor),
+    angle U(30, 60) deg, speed U(1.0, 1.4) v_esc, impactor / target mass ratio U(0.05, 0.15), core mass
+    fraction U(0.25, 0.35); seeded."""
+    from botheadstudios_amd import scene
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(16384), np.log(262144), count)).astype(np.int64)
+    out = []
+    for k in range(count):
+        n = int(sizes[k])
+        ni = n // 8
+        out.append(scene.impact_scene(n - ni, ni, seed + 1 + k, m_i=float(rng.uniform(0.05, 0.15)),
+                                      core_t=float(rng.uniform(0.25, 0.35)), core_i=0.3,
+                                      angle_deg=float(rng.uniform(30.0, 60.0)),
+                                      v_over_vesc=float(rng.uniform(1.0, 1.4))))
+    return out
+
+
+def run_batch(args):
+    """All instances as one ensemble (mixed form, captured step graph, autotuned walk shape) against the same
+    instances run one after another in the plain form, same number of steps each."""
+    import tempfile
+    from botheadstudios_amd import isph
+    scs = batch_scenes(args.batch_count)
+    total = sum(len(sc.particles) for sc in scs)
+    cache = os.path.join(tempfile.gettempdir(), "isph_autotune.txt")
+    with isph.GpuSph(total) as s:
+        s.upload_ensemble([sc.particles for sc in scs], scs[0].eos, scs[0].softening)
+        s.set_eos_f64(scs[0].eos64)
+        s.set_precision(isph.PRECISION_MIXED)
+        s.set_graph(True)
+        from_cache, tun = s.autotune(args.cfl, 3, cache)
+        s.step_cfl(args.cfl, 2)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step_cfl(args.cfl, args.batch_steps)
+        s.synchronize()
+        ens = time.perf_counter() - t0
+    seq = 0.0
+    for sc in scs:
+        n = len(sc.particles)
+        with isph.GpuSph(n) as s:
+            s.upload(sc.particles, sc.eos, sc.softening)
+            s.set_eos_f64(sc.eos64)
+            s.step_cfl(args.cfl, 1)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step_cfl(args.cfl, args.batch_steps)
+            s.synchronize()
+            seq += time.perf_counter() - t0
+    return {"workload": f"{len(scs)} two-body impacts, N log-uniform in [16384, 262144] (total {total}), mixed form, "
+                        f"{args.batch_steps} KDK steps each at CFL {args.cfl}",
+            "instances": len(scs), "particles": total, "steps": args.batch_steps,
+            "value": total * args.batch_steps / ens, "unit": "particle-steps/s", "ms_per_step": ens * 1e3 / args.batch_steps,
+            "sequential_plain_value": total * args.batch_steps / seq,
+            "speedup_vs_sequential_plain": seq / ens,
+            "tuning": {"leaf_size": tun.leaf_size, "group_extent": tun.group_extent,
+                       "grav_particles_per_lane": tun.grav_particles_per_lane, "sph_expand": tun.sph_expand,
+                       "from_cache": from_cache}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -241,6 +305,8 @@ def main():
                    "dE_rel": rv.get("dE_rel"), "dL_rel": rv.get("dL_rel")}
     if not args.no_adjoint:
         line["adjoint"] = run_adjoint(args)
+    if not args.no_batch:
+        line["batch"] = run_batch(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)

@@ -7,6 +7,8 @@
 
 #include <cmath>
 #include <cstdio>
+#include <fstream>
+#include <sstream>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -74,6 +76,14 @@ struct isph_ctx {
 
     // adjoint arrays (allocated on first use)
     Adj adj;
+
+    // captured step (isph_set_graph): one KDK step as a hipGraph, replayed per step. Any change of the
+    // configuration or the particle set drops it; cfl / dt values are part of its key.
+    bool graph_on = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_mode = -1;  // 0 fixed dt, 1 Courant dt
+    float graph_value = 0.0f;
 };
 
 namespace {
@@ -98,6 +108,8 @@ void free_all(isph_ctx* c) {
     void* aptrs[] = {a.xb, a.vb, a.ub, a.mb, a.ab, a.sa, a.sxb, a.svb, a.srb, a.sqb, a.scb, a.nbar, a.pbar, a.red};
     for (void* q : aptrs)
         if (q) (void)hipFree(q);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->rb_event) (void)hipEventDestroy(c->rb_event);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -319,6 +331,55 @@ void kdk(isph_ctx* c, bool cfl_mode, float cfl) {
     c->reuse_valid = true;
 }
 
+void drop_graph(isph_ctx* c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_mode = -1;
+}
+
+// One KDK step, launched from the captured graph when graph mode is on (capturing it on first use).
+// The step must be in its steady state when captured: with evaluation reuse on, the first step after a
+// change runs eagerly so that the captured step is the reusing one.
+void step_once(isph_ctx* c, bool cfl_mode, float value) {
+    const Dev& d = c->d;
+    auto body = [&]() {
+        if (!cfl_mode) {
+            if (d.ninst > 1) launch_set_dt_value_seg(d, value, c->stream);
+            else launch_set_dt_value(d, value, c->stream);
+        }
+        kdk(c, cfl_mode, value);
+    };
+    if (!c->graph_on || (c->reuse_enabled && !c->reuse_valid)) {
+        body();
+        return;
+    }
+    const int mode = cfl_mode ? 1 : 0;
+    if (!c->graph_exec || c->graph_mode != mode || c->graph_value != value) {
+        drop_graph(c);
+        const bool timing = c->timing;
+        c->timing = false;  // no event records inside the graph
+        HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            c->timing = timing;
+            throw;
+        }
+        c->timing = timing;
+        HIP_OK(hipStreamEndCapture(c->stream, &c->graph));
+        HIP_OK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_mode = mode;
+        c->graph_value = value;
+    }
+    Phase ph(c, ISPH_PHASE_INTEGRATE);  // with timing on, the whole replayed step is one sample
+    HIP_OK(hipGraphLaunch(c->graph_exec, c->stream));
+}
+
 int fail(isph_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
     return code;
@@ -524,6 +585,7 @@ uint32_t isph_count(const isph_t* h) { return h ? h->d.n : 0; }
 int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_eos* eos, size_t n_eos,
                 float softening) {
     return guarded(h, [&]() -> int {
+        drop_graph(h);
         if (n > 0 && !particles) return fail(h, ISPH_EINVAL, "particles is NULL");
         size_t ne = std::min<size_t>(n_eos, kMaxMaterials);  // gpu_sph.rs:1337
         size_t take = std::min<size_t>(n, h->d.cap);          // gpu_store.rs:33
@@ -576,6 +638,7 @@ int isph_upload_ensemble(isph_t* h, const isph_particle* particles, const uint32
         d.idt = nullptr;
         d.itime = nullptr;
         d.ninst = 1;
+        drop_graph(h);
         if (n_inst == 1) return (int)n;
         dalloc(&d.inst, d.cap);
         dalloc(&d.mcode, d.cap);
@@ -595,6 +658,7 @@ int isph_upload_ensemble(isph_t* h, const isph_particle* particles, const uint32
         HIP_OK(hipMemset(d.itime, 0, n_inst * sizeof(double)));
         d.ninst = n_inst;
         h->reuse_valid = false;
+        drop_graph(h);
         return (int)n;
     });
 }
@@ -643,6 +707,7 @@ int isph_set_dt(isph_t* h, float dt, float damp) {
 
 int isph_set_av(isph_t* h, float alpha, float beta) {
     return guarded(h, [&]() -> int {
+        drop_graph(h);
         h->p.av_alpha = alpha;
         h->p.av_beta = beta;
         h->reuse_valid = false;
@@ -652,6 +717,7 @@ int isph_set_av(isph_t* h, float alpha, float beta) {
 
 int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int quadrupole) {
     return guarded(h, [&]() -> int {
+        drop_graph(h);
         if (mode < ISPH_GRAVITY_TREE || mode > ISPH_GRAVITY_NONE) return fail(h, ISPH_EINVAL, "bad gravity mode");
         if (!(theta > 0.0f) || leaf_size < 1 || leaf_size > 64) return fail(h, ISPH_EINVAL, "bad theta or leaf size");
         if (h->d.ninst > 1 && (leaf_size != 1 || mode == ISPH_GRAVITY_DIRECT))
@@ -668,6 +734,7 @@ int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int q
 
 int isph_set_walk(isph_t* h, float group_extent, int grav_particles_per_lane, int sph_expand) {
     return guarded(h, [&]() -> int {
+        drop_graph(h);
         if (!(group_extent > 0.0f) || (grav_particles_per_lane != 1 && grav_particles_per_lane != 2) ||
             sph_expand < 1 || sph_expand > 64)
             return fail(h, ISPH_EINVAL, "bad walk shape");
@@ -681,6 +748,7 @@ int isph_set_walk(isph_t* h, float group_extent, int grav_particles_per_lane, in
 
 int isph_set_precision(isph_t* h, int mode) {
     return guarded(h, [&]() -> int {
+        drop_graph(h);
         if (mode != ISPH_PRECISION_FP32 && mode != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "bad precision");
         HIP_OK(hipSetDevice(h->device));
         if (mode == ISPH_PRECISION_MIXED) {
@@ -697,8 +765,19 @@ int isph_set_precision(isph_t* h, int mode) {
     });
 }
 
+int isph_set_graph(isph_t* h, int enable) {
+    return guarded(h, [&]() -> int {
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        drop_graph(h);
+        h->graph_on = enable != 0;
+        return ISPH_OK;
+    });
+}
+
 int isph_set_reuse(isph_t* h, int enable) {
     return guarded(h, [&]() -> int {
+        drop_graph(h);
         h->reuse_enabled = enable != 0;
         h->reuse_valid = false;
         return ISPH_OK;
@@ -725,9 +804,7 @@ int isph_encode_kdk(isph_t* h, uint32_t substeps) {
         Dev saved = h->d;
         h->d = integ_view(h);
         try {
-            if (h->d.ninst > 1) launch_set_dt_value_seg(h->d, h->dt_set, h->stream);
-            else launch_set_dt_value(h->d, h->dt_set, h->stream);
-            for (uint32_t s = 0; s < substeps; ++s) kdk(h, false, 0.0f);
+            for (uint32_t s = 0; s < substeps; ++s) step_once(h, false, h->dt_set);
         } catch (...) {
             h->d = saved;
             throw;
@@ -746,7 +823,7 @@ int isph_step_cfl(isph_t* h, float cfl, uint32_t steps) {
         Dev saved = h->d;
         h->d = integ_view(h);
         try {
-            for (uint32_t s = 0; s < steps; ++s) kdk(h, true, cfl);
+            for (uint32_t s = 0; s < steps; ++s) step_once(h, true, cfl);
         } catch (...) {
             h->d = saved;
             throw;
@@ -1209,6 +1286,167 @@ int isph_vjp_run(isph_t* h, float cfl, const float* dts_in, uint32_t steps, uint
         fetch(deos4, h->adj.pbar, 4 * (size_t)kMaxMaterials);
         if (dts_out) std::copy(dts.begin(), dts.end(), dts_out);
         return ISPH_OK;
+    });
+}
+
+}  // extern "C"
+
+// ---- autotuner --------------------------------------------------------------------------------------------
+namespace {
+
+struct Tuning {
+    uint32_t K;
+    float extent;
+    int ppl, expand;
+};
+
+// Snapshot of the integrated state, restored after each trial.
+struct Snapshot {
+    float4 *posh = nullptr, *velu = nullptr;
+    double *pos64 = nullptr, *vel64 = nullptr, *itime = nullptr, time = 0.0;
+    void take(isph_ctx* c) {
+        const Dev& d = c->d;
+        const size_t n = d.n;
+        dalloc(&posh, n);
+        dalloc(&velu, n);
+        HIP_OK(hipMemcpyAsync(posh, d.posh, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(velu, d.velu, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        if (d.pos64 && c->precision == ISPH_PRECISION_MIXED) {
+            dalloc(&pos64, 3 * n);
+            dalloc(&vel64, 3 * n);
+            HIP_OK(hipMemcpyAsync(pos64, d.pos64, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            HIP_OK(hipMemcpyAsync(vel64, d.vel64, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        if (d.ninst > 1) {
+            dalloc(&itime, d.ninst);
+            HIP_OK(hipMemcpyAsync(itime, d.itime, d.ninst * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        HIP_OK(hipMemcpyAsync(&time, d.time, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
+    void restore(isph_ctx* c) {
+        const Dev& d = c->d;
+        const size_t n = d.n;
+        HIP_OK(hipMemcpyAsync(d.posh, posh, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(d.velu, velu, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
+        if (pos64) {
+            HIP_OK(hipMemcpyAsync(d.pos64, pos64, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            HIP_OK(hipMemcpyAsync(d.vel64, vel64, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        if (itime)
+            HIP_OK(hipMemcpyAsync(d.itime, itime, d.ninst * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(d.time, &time, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->reuse_valid = false;
+    }
+    ~Snapshot() {
+        void* ptrs[] = {posh, velu, pos64, vel64, itime};
+        for (void* q : ptrs)
+            if (q) (void)hipFree(q);
+    }
+};
+
+void apply(isph_ctx* c, const Tuning& t) {
+    drop_graph(c);
+    c->d.K = t.K;
+    set_leaves(c, c->d.n);
+    c->p.group_extent = t.extent;
+    c->p.grav_ppl = t.ppl;
+    c->p.sph_expand = t.expand;
+    c->reuse_valid = false;
+}
+
+std::string tune_key(isph_ctx* c) {
+    uint32_t bucket = 0;
+    while ((1u << (bucket + 1)) <= c->d.n) ++bucket;
+    std::ostringstream k;
+    k << "n2^" << bucket << " inst" << (c->d.ninst > 1 ? 1 : 0) << " prec" << c->precision << " grav"
+      << c->gravity_mode << " quad" << c->p.quad;
+    return k.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int isph_autotune(isph_t* h, float cfl, uint32_t trial_steps, const char* cache_path, isph_tuning* chosen) {
+    return guarded(h, [&]() -> int {
+        if (!(cfl > 0.0f) || trial_steps == 0) return fail(h, ISPH_EINVAL, "cfl and trial_steps must be > 0");
+        if (h->d.n == 0) return fail(h, ISPH_EINVAL, "no particles");
+        HIP_OK(hipSetDevice(h->device));
+        const std::string key = tune_key(h);
+        if (cache_path) {  // "key | K extent ppl expand ms" lines
+            std::ifstream in(cache_path);
+            std::string line;
+            while (std::getline(in, line)) {
+                const size_t bar = line.find(" | ");
+                if (bar == std::string::npos || line.substr(0, bar) != key) continue;
+                std::istringstream v(line.substr(bar + 3));
+                Tuning t{};
+                double ms = 0.0;
+                if (!(v >> t.K >> t.extent >> t.ppl >> t.expand >> ms)) continue;
+                apply(h, t);
+                if (chosen) *chosen = isph_tuning{t.K, t.extent, t.ppl, t.expand, ms};
+                return 1;
+            }
+        }
+        Snapshot snap;
+        snap.take(h);
+        const bool graph = h->graph_on;
+        h->graph_on = false;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        struct EvGuard {
+            hipEvent_t a, b;
+            ~EvGuard() {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+            }
+        } eg{e0, e1};
+        auto trial = [&](const Tuning& t) -> double {
+            apply(h, t);
+            step_once(h, true, cfl);  // warm-up step (group lists, caches)
+            HIP_OK(hipEventRecord(e0, h->stream));
+            for (uint32_t s = 0; s < trial_steps; ++s) step_once(h, true, cfl);
+            HIP_OK(hipEventRecord(e1, h->stream));
+            HIP_OK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+            snap.restore(h);
+            return ms / trial_steps;
+        };
+        // Coordinate descent from the current shape: leaf size (one system only), group extent, gravity
+        // particles per lane, SPH whole-subtree size.
+        Tuning best{h->d.K, h->p.group_extent, h->p.grav_ppl, h->p.sph_expand};
+        double best_ms = trial(best);
+        auto try_axis = [&](auto setter, std::initializer_list<double> values) {
+            Tuning base = best;
+            for (double v : values) {
+                Tuning t = base;
+                setter(t, v);
+                if (t.K == best.K && t.extent == best.extent && t.ppl == best.ppl && t.expand == best.expand) continue;
+                double ms = trial(t);
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = t;
+                }
+            }
+        };
+        if (h->d.ninst == 1 && h->gravity_mode == ISPH_GRAVITY_TREE)
+            try_axis([](Tuning& t, double v) { t.K = (uint32_t)v; }, {1, 2, 4});
+        try_axis([](Tuning& t, double v) { t.extent = (float)v; }, {3.0, 4.0, 6.0});
+        if (h->gravity_mode == ISPH_GRAVITY_TREE) try_axis([](Tuning& t, double v) { t.ppl = (int)v; }, {1, 2});
+        try_axis([](Tuning& t, double v) { t.expand = (int)v; }, {1, 4, 8});
+        apply(h, best);
+        h->graph_on = graph;
+        if (cache_path) {
+            std::ofstream out(cache_path, std::ios::app);
+            out << key << " | " << best.K << " " << best.extent << " " << best.ppl << " " << best.expand << " "
+                << best_ms << "\n";
+        }
+        if (chosen) *chosen = isph_tuning{best.K, best.extent, best.ppl, best.expand, best_ms};
+        return 0;
     });
 }
 

@@ -37,6 +37,8 @@ _SIGS = {
     "isph_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_walk": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int, ctypes.c_int]),
     "isph_set_reuse": (ctypes.c_int, [_P, ctypes.c_int]),
+    "isph_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
+    "isph_autotune": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32, ctypes.c_char_p, _P]),
     "isph_encode_kdk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "isph_step_cfl": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32]),
     "isph_sim_time": (ctypes.c_int, [_P, _P, _P]),
@@ -62,6 +64,13 @@ _SIGS = {
     "isph_vjp_run": (ctypes.c_int, [_P, ctypes.c_float, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
                                     _P, _P, _P, _P]),
 }
+
+
+class Tuning(ctypes.Structure):
+    """`isph_tuning`: the walk shape chosen by the autotuner."""
+    _fields_ = [("leaf_size", ctypes.c_uint32), ("group_extent", ctypes.c_float),
+                ("grav_particles_per_lane", ctypes.c_int), ("sph_expand", ctypes.c_int),
+                ("ms_per_step", ctypes.c_double)]
 
 
 class Loss(ctypes.Structure):
@@ -160,6 +169,17 @@ class GpuSph:
 
     def set_precision(self, mode):
         self._check(self._lib.isph_set_precision(self._h, int(mode)))
+
+    def set_graph(self, enable):
+        self._check(self._lib.isph_set_graph(self._h, int(bool(enable))))
+
+    def autotune(self, cfl=0.3, trial_steps=3, cache_path=None):
+        """Tune the walk shape on the current state; returns (from_cache, Tuning)."""
+        t = Tuning()
+        rc = self._lib.isph_autotune(self._h, float(cfl), int(trial_steps),
+                                     None if cache_path is None else str(cache_path).encode(), ctypes.byref(t))
+        self._check(rc)
+        return rc == 1, t
 
     def set_reuse(self, enable):
         self._check(self._lib.isph_set_reuse(self._h, int(bool(enable))))

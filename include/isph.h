@@ -125,6 +125,25 @@ int isph_set_precision(isph_t* h, int mode);
 /* Reuse the tree, gravity and density of the previous evaluation when positions have not moved since
  * (eval 2 of step n and eval 1 of step n+1 see identical positions). Results are bit-identical. */
 int isph_set_reuse(isph_t* h, int enable);
+/* Graph mode (no reference counterpart): isph_encode_kdk / isph_step_cfl capture one KDK step (all its
+ * launches) as a hipGraph on first use and replay it per step. Any configuration change or upload drops
+ * the graph; the dt (encode) or cfl (step_cfl) value is part of its key. */
+int isph_set_graph(isph_t* h, int enable);
+/* Autotuner (no reference counterpart). Times `trial_steps` Courant steps per candidate from the current
+ * state (restored after every trial) and keeps the fastest walk shape by coordinate descent over the leaf
+ * size (one system, tree gravity: 1, 2, 4), the group extent (3, 4, 6), the gravity particles per lane
+ * (1, 2) and the SPH whole-subtree size (1, 4, 8); the radix width is fixed (8-bit digits). Results are
+ * cached per size bucket (floor(log2 n)), ensemble flag, precision and gravity form in the text file
+ * `cache_path` (may be NULL): a cached entry is applied without trials (returns 1), a tuned one is appended
+ * (returns 0). The chosen shape is applied to the handle and written to `chosen` (may be NULL). */
+typedef struct isph_tuning {
+    uint32_t leaf_size;
+    float group_extent;
+    int grav_particles_per_lane;
+    int sph_expand;
+    double ms_per_step;
+} isph_tuning;
+int isph_autotune(isph_t* h, float cfl, uint32_t trial_steps, const char* cache_path, isph_tuning* chosen);
 
 /* GpuSph::encode_kdk(enc, substeps) gpu_sph.rs:1484-1491: `substeps` KDK steps at the set_dt dt, each
  * = eval -> half-kick + drift -> eval -> half-kick (sph_step.wgsl:304-333). Enqueued, returns at once. */

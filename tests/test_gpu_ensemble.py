@@ -55,3 +55,58 @@ def test_ensemble_rejects_what_would_couple_instances():
             s.set_gravity(isph.GRAVITY_TREE, 0.5, 4)
         with pytest.raises(isph.IsphError):
             s.diagnostics(with_pe=True)
+
+
+def _stepper(recs, precision, graph):
+    s = isph.GpuSph(sum(len(r) for r in recs))
+    if len(recs) > 1:
+        s.upload_ensemble(recs, EOS, _scenes()[0].softening)
+    else:
+        s.upload(recs[0], EOS, _scenes()[0].softening)
+    s.set_eos_f64(EOS64)
+    s.set_precision(precision)
+    s.set_graph(graph)
+    return s
+
+
+@pytest.mark.parametrize("ensemble", [False, True])
+@pytest.mark.parametrize("precision", [isph.PRECISION_FP32, isph.PRECISION_MIXED])
+def test_graph_replays_equal_eager_steps(ensemble, precision):
+    """A captured step replayed several times, with eager work (read-backs, synchronisation) between the
+    replays, gives the eager step's bits every time, in Courant and in fixed-dt mode."""
+    recs = [sc.particles for sc in (_scenes() if ensemble else _scenes()[:1])]
+    a = _stepper(recs, precision, False)
+    b = _stepper(recs, precision, True)
+    with a, b:
+        for k in range(4):
+            a.step_cfl(0.3, 1)
+            b.step_cfl(0.3, 1)
+            ra, rb = a.readback_sync(), b.readback_sync()
+            assert np.array_equal(ra["pos"], rb["pos"]) and np.array_equal(ra["vel"], rb["vel"]), k
+            assert np.array_equal(ra["u"], rb["u"]), k
+        a.set_dt(25.0, 1.0)
+        b.set_dt(25.0, 1.0)
+        a.encode_kdk(3)
+        b.encode_kdk(3)
+        ra, rb = a.readback_sync(), b.readback_sync()
+        assert np.array_equal(ra["pos"], rb["pos"]) and np.array_equal(ra["vel"], rb["vel"])
+
+
+def test_autotune_picks_and_caches_a_walk_shape(tmp_path):
+    cache = tmp_path / "tune.txt"
+    sc = scene.two_body(32768, seed=0xE4)
+    with isph.GpuSph(len(sc.particles)) as s:
+        s.upload(sc.particles, EOS, sc.softening)
+        before = s.readback_sync()
+        from_cache, t = s.autotune(0.3, 2, cache)
+        after = s.readback_sync()
+    assert not from_cache and cache.exists()
+    assert t.leaf_size in (1, 2, 4) and t.group_extent in (3.0, 4.0, 6.0) and t.grav_particles_per_lane in (1, 2)
+    assert t.sph_expand in (1, 4, 8) and t.ms_per_step > 0
+    assert np.array_equal(before["pos"], after["pos"])  # the trials leave the state as it was
+    with isph.GpuSph(len(sc.particles)) as s:
+        s.upload(sc.particles, EOS, sc.softening)
+        from_cache, t2 = s.autotune(0.3, 2, cache)
+    assert from_cache
+    assert (t2.leaf_size, t2.group_extent, t2.grav_particles_per_lane, t2.sph_expand) == \
+        (t.leaf_size, t.group_extent, t.grav_particles_per_lane, t.sph_expand)
