@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance ensemble measurement")
+    ap.add_argument("--no-4m", action="store_true", help="skip the plain / mixed throughput at N = 4,194,304")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--adjoint-n", type=int, default=262144)
@@ -182,6 +183,28 @@ def run_adjoint(args):
             "granite_dloss_d_a_b_A_B": [float(x) for x in g]}
 
 
+def run_4m(args, steps=10):
+    """Plain vs mixed throughput on the 4,194,304-particle two-body impact (the precision form's size)."""
+    from botheadstudios_amd import isph, scene
+    sc = scene.two_body(4194304, seed=0xC3)
+    n = len(sc.particles)
+    res = {}
+    for name, prec in (("fp32", isph.PRECISION_FP32), ("mixed", isph.PRECISION_MIXED)):
+        with isph.GpuSph(n) as s:
+            s.upload(sc.particles, sc.eos, sc.softening)
+            s.set_eos_f64(sc.eos64)
+            s.set_precision(prec)
+            s.step_cfl(args.cfl, 2)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step_cfl(args.cfl, steps)
+            s.synchronize()
+            res[name] = n * steps / (time.perf_counter() - t0)
+    return {"workload": f"two-body impact N={n}, LBVH theta={args.theta}, {steps} KDK steps at CFL {args.cfl}",
+            "fp32_value": res["fp32"], "mixed_value": res["mixed"], "mixed_over_fp32": res["mixed"] / res["fp32"],
+            "unit": "particle-steps/s"}
+
+
 def batch_scenes(count, seed=0xC4):
     """The batching workload: `count` impacts, N log-uniform in [16384, 262144] (7:1 target:impactor),
     angle U(30, 60) deg, speed U(1.0, 1.4) v_esc, impactor / target mass ratio U(0.05, 0.15), core mass
@@ -307,6 +330,8 @@ def main():
         line["adjoint"] = run_adjoint(args)
     if not args.no_batch:
         line["batch"] = run_batch(args)
+    if not args.no_4m:
+        line["precision_4m"] = run_4m(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc)
     print(json.dumps(line), flush=True)
