@@ -205,6 +205,60 @@ __device__ __forceinline__ float spline_dw(float q, float inv_h) {
     return 0.0f;
 }
 
+// ---- the un-resolved bulk and its floor (sph_step.wgsl:120-162) ----------------------------------------
+// Gravity of the bulk: monopole outside R_core, linear inside (-> 0 at the centre); planar mode: uniform g
+// along -up. f32 as the shader.
+__device__ __forceinline__ float3 bulk_gravity(float3 pos, float4 cr, float4 vm) {
+    const float R = cr.w;
+    if (R == 0.0f) return make_float3(0.f, 0.f, 0.f);
+    if (R < 0.0f) {
+        const float il = rsqrtf(vm.x * vm.x + vm.y * vm.y + vm.z * vm.z);
+        return make_float3(-vm.x * il * vm.w, -vm.y * il * vm.w, -vm.z * il * vm.w);
+    }
+    const float dx = pos.x - cr.x, dy = pos.y - cr.y, dz = pos.z - cr.z;
+    const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (r < 1.0f) return make_float3(0.f, 0.f, 0.f);
+    const float gg = r >= R ? kG * vm.w / (r * r) : kG * vm.w * r / (R * R * R);
+    return make_float3(-dx / r * gg, -dy / r * gg, -dz / r * gg);
+}
+// The non-injecting floor: a particle below the plane / inside R_core is projected onto it and its
+// velocity into the floor (relative to the bulk) removed. T is float or double.
+template <class T>
+__device__ __forceinline__ void bulk_floor(T& x, T& y, T& z, T& vx, T& vy, T& vz, float4 cr, float4 vm) {
+    const float R = cr.w;
+    if (R == 0.0f) return;
+    if (R < 0.0f) {
+        const T il = (T)1 / sqrt((T)vm.x * vm.x + (T)vm.y * vm.y + (T)vm.z * vm.z);
+        const T ux = vm.x * il, uy = vm.y * il, uz = vm.z * il;
+        const T depth = (x - cr.x) * ux + (y - cr.y) * uy + (z - cr.z) * uz;
+        if (depth < (T)0) {
+            x -= ux * depth;
+            y -= uy * depth;
+            z -= uz * depth;
+            const T vn = vx * ux + vy * uy + vz * uz;
+            if (vn < (T)0) {
+                vx -= ux * vn;
+                vy -= uy * vn;
+                vz -= uz * vn;
+            }
+        }
+        return;
+    }
+    const T dx = x - cr.x, dy = y - cr.y, dz = z - cr.z;
+    const T r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r >= (T)R || r < (T)1) return;
+    const T nx = dx / r, ny = dy / r, nz = dz / r;
+    x = cr.x + nx * R;
+    y = cr.y + ny * R;
+    z = cr.z + nz * R;
+    const T vn = (vx - vm.x) * nx + (vy - vm.y) * ny + (vz - vm.z) * nz;
+    if (vn < (T)0) {
+        vx -= nx * vn;
+        vy -= ny * vn;
+        vz -= nz * vn;
+    }
+}
+
 // Compensated accumulation (Neumaier/Kahan) for the mixed-precision form.
 struct KahanF {
     float s, c;

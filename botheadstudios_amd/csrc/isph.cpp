@@ -101,7 +101,7 @@ void free_all(isph_ctx* c) {
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
-    void* eptrs[] = {d.inst, d.mcode, d.lkey, d.ioff, d.ibox, d.iroot, d.isig, d.idt, d.itime};
+    void* eptrs[] = {d.inst, d.mcode, d.lkey, d.ioff, d.ibox, d.iroot, d.isig, d.idt, d.itime, d.ext, d.mtarget};
     for (void* q : eptrs)
         if (q) (void)hipFree(q);
     const Adj& a = c->adj;
@@ -167,6 +167,8 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.counters, 16);
     dalloc(&d.red, 1024 * 16);
     dalloc(&d.eos, kMaxMaterials);
+    dalloc(&d.ext, kMaxExtBodies);
+    dalloc(&d.mtarget, cap);
     HIP_OK(hipMalloc(&d.aos, (size_t)cap * sizeof(isph_particle)));
     HIP_OK(hipHostMalloc((void**)&c->pinned, (size_t)cap * sizeof(isph_particle), hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&c->rb_event, hipEventDisableTiming));
@@ -319,7 +321,7 @@ void kdk(isph_ctx* c, bool cfl_mode, float cfl) {
             if (d.ninst > 1) launch_set_dt_cfl_seg(d, cfl, c->stream);
             else launch_set_dt_cfl(d, cfl, c->stream);
         }
-        launch_kick_drift(d, c->stream);
+        launch_kick_drift(d, c->p, c->stream);
     }
     eval(c, true);
     {
@@ -765,6 +767,119 @@ int isph_set_precision(isph_t* h, int mode) {
     });
 }
 
+static Dev integ_view(isph_ctx* c);
+
+int isph_set_bulk(isph_t* h, double cx, double cy, double cz, double r_core, double vx, double vy, double vz,
+                  double mass) {
+    return guarded(h, [&]() -> int {
+        drop_graph(h);
+        h->p.bulk_cr = make_float4((float)cx, (float)cy, (float)cz, (float)r_core);  // gpu_sph.rs:1372-1378
+        h->p.bulk_vm = make_float4((float)vx, (float)vy, (float)vz, (float)mass);
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_bulk_planar(isph_t* h, double px, double py, double pz, double ux, double uy, double uz, double g) {
+    return guarded(h, [&]() -> int {
+        drop_graph(h);
+        double l = std::sqrt(ux * ux + uy * uy + uz * uz);
+        if (!(l > 0.0) || !std::isfinite(l)) {  // normalize_or(+Y), gpu_sph.rs:1394
+            ux = 0.0;
+            uy = 1.0;
+            uz = 0.0;
+            l = 1.0;
+        }
+        h->p.bulk_cr = make_float4((float)px, (float)py, (float)pz, -1.0f);
+        h->p.bulk_vm = make_float4((float)(ux / l), (float)(uy / l), (float)(uz / l), (float)g);
+        h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_external_masses(isph_t* h, const double* pos_mass4, size_t n) {
+    return guarded(h, [&]() -> int {
+        if (n > 0 && !pos_mass4) return fail(h, ISPH_EINVAL, "NULL bodies");
+        const size_t k = std::min<size_t>(n, kMaxExtBodies);  // overflow dropped, count returned (:1417)
+        std::vector<float4> packed(k);
+        for (size_t i = 0; i < k; ++i)
+            packed[i] = make_float4((float)pos_mass4[4 * i], (float)pos_mass4[4 * i + 1], (float)pos_mass4[4 * i + 2],
+                                    (float)pos_mass4[4 * i + 3]);
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (k) HIP_OK(hipMemcpy(h->d.ext, packed.data(), k * sizeof(float4), hipMemcpyHostToDevice));
+        drop_graph(h);
+        h->p.n_ext = (uint32_t)k;
+        h->reuse_valid = false;
+        return (int)k;
+    });
+}
+
+int isph_set_merge_budget(isph_t* h, uint32_t budget) {
+    return guarded(h, [&]() -> int {
+        h->p.merge_budget = budget;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_omega(isph_t* h, float omega) {
+    return guarded(h, [&]() -> int {
+        drop_graph(h);
+        h->p.omega = omega;
+        return ISPH_OK;
+    });
+}
+
+int isph_encode_relax(isph_t* h, uint32_t steps) {
+    return guarded(h, [&]() -> int {
+        if (h->d.n == 0 || steps == 0) return ISPH_OK;
+        if (h->d.ninst > 1) return fail(h, ISPH_EINVAL, "relaxation takes one system");
+        HIP_OK(hipSetDevice(h->device));
+        Dev saved = h->d;
+        h->d = integ_view(h);
+        try {
+            launch_set_dt_value(h->d, h->dt_set, h->stream);
+            for (uint32_t s = 0; s < steps; ++s) {
+                eval(h, true);  // one force evaluation (gpu_sph.rs:1476-1481)
+                Phase ph(h, ISPH_PHASE_INTEGRATE);
+                launch_relax(h->d, h->p, h->damp, h->stream);
+            }
+        } catch (...) {
+            h->d = saved;
+            throw;
+        }
+        h->d = saved;
+        h->reuse_valid = false;
+        HIP_OK(hipGetLastError());
+        return ISPH_OK;
+    });
+}
+
+int isph_encode_merge(isph_t* h) {
+    return guarded(h, [&]() -> int {
+        // gpu_sph.rs:1439-1447: neighbours + density, then pick -> apply -> retire; a no-op unless the
+        // particle count exceeds the budget (sph_step.wgsl:386-388)
+        if (h->d.n == 0 || h->p.merge_budget == 0 || h->d.n <= h->p.merge_budget) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        Dev saved = h->d;
+        h->d = integ_view(h);
+        try {
+            build_sorted(h, h->d, h->d.posh, h->d.mass, true);
+            build_tree(h, h->d);
+            Phase ph(h, ISPH_PHASE_DENSITY);
+            launch_density(h->d, h->p, h->stream);
+            launch_merge(h->d, h->p, h->stream);
+        } catch (...) {
+            h->d = saved;
+            throw;
+        }
+        h->d = saved;
+        h->reuse_valid = false;
+        HIP_OK(hipGetLastError());
+        return ISPH_OK;
+    });
+}
+
 int isph_set_graph(isph_t* h, int enable) {
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));
@@ -1171,6 +1286,8 @@ int isph_vjp_run(isph_t* h, float cfl, const float* dts_in, uint32_t steps, uint
         if (!loss) return fail(h, ISPH_EINVAL, "NULL loss");
         if (h->precision != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "the adjoint needs the mixed form");
         if (h->d.ninst > 1) return fail(h, ISPH_EINVAL, "the adjoint takes one system, not an ensemble");
+        if (h->p.bulk_cr.w != 0.0f || h->p.n_ext != 0)
+            return fail(h, ISPH_EINVAL, "the adjoint does not differentiate the bulk or external masses");
         if (h->gravity_mode == ISPH_GRAVITY_TREE && (h->p.quad || h->p.grav_ppl != 1))
             return fail(h, ISPH_EINVAL, "the adjoint's tree gravity is the monopole walk with one particle per lane");
         if (steps == 0 || checkpoint_every == 0) return fail(h, ISPH_EINVAL, "steps and checkpoint_every must be > 0");
@@ -1248,7 +1365,7 @@ int isph_vjp_run(isph_t* h, float cfl, const float* dts_in, uint32_t steps, uint
                 launch_save_state(d, w.start[i].x, w.start[i].v, w.start[i].u, st);
                 launch_set_dt_value(d, dts[s], st);
                 eval_into(h, w.ev[2 * i]);
-                launch_kick_drift(d, st);
+                launch_kick_drift(d, h->p, st);
                 launch_save_state(d, w.mid[i].x, w.mid[i].v, w.mid[i].u, st);
                 eval_into(h, w.ev[2 * i + 1]);
                 launch_kick(d, st);

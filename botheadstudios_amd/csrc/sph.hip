@@ -201,7 +201,7 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
                                               const uint2* __restrict__ groups, uint32_t* scal,
                                               const uint2* __restrict__ nrange, uint32_t expand,
                                               double* __restrict__ acc64, const uint32_t* __restrict__ sinst,
-                                              const uint32_t* __restrict__ iroot) {
+                                              const uint32_t* __restrict__ iroot, Boundary bnd) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, mass
@@ -329,6 +329,20 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
     }
     if (valid) {
         float4 gr = sgrav[t];
+        // the un-resolved bulk and the external point masses (sph_step.wgsl:242-254), f32 as the shader
+        float3 gb = bulk_gravity(make_float3(me.x, me.y, me.z), bnd.bulk_cr, bnd.bulk_vm);
+        for (uint32_t k = 0; k < bnd.n_ext; ++k) {
+            const float4 e = bnd.ext[k];
+            const float dx = e.x - me.x, dy = e.y - me.y, dz = e.z - me.z;
+            const float r2 = dx * dx + dy * dy + dz * dz + bnd.soft2;
+            const float w = kG * e.w / (r2 * sqrtf(r2));
+            gb.x += dx * w;
+            gb.y += dy * w;
+            gb.z += dz * w;
+        }
+        gr.x += gb.x;
+        gr.y += gb.y;
+        gr.z += gb.z;
         float4 out = me.w > 0.f ? make_float4(gr.x + ax, gr.y + ay, gr.z + az, de) : make_float4(0.f, 0.f, 0.f, 0.f);
         const uint32_t o = order[t];
         acc_orig[o] = out;
@@ -349,7 +363,8 @@ __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, fl
                                                     const float* __restrict__ mass, const float4* __restrict__ acc,
                                                     const float* __restrict__ dtp, double* pos64, double* vel64,
                                                     const double* __restrict__ acc64,
-                                                    const uint32_t* __restrict__ inst, const float* __restrict__ idt) {
+                                                    const uint32_t* __restrict__ inst, const float* __restrict__ idt,
+                                                    float4 bulk_cr, float4 bulk_vm) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !(mass[i] > 0.f)) return;
     const float dt = inst ? idt[inst[i]] : *dtp;
@@ -363,6 +378,7 @@ __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, fl
                vz = vel64[3 * i + 2] + acc64[3 * i + 2] * hd;
         double x = pos64[3 * i] + vx * (double)dt, y = pos64[3 * i + 1] + vy * (double)dt,
                z = pos64[3 * i + 2] + vz * (double)dt;
+        bulk_floor(x, y, z, vx, vy, vz, bulk_cr, bulk_vm);  // sph_step.wgsl:319
         vel64[3 * i] = vx;
         vel64[3 * i + 1] = vy;
         vel64[3 * i + 2] = vz;
@@ -382,6 +398,54 @@ __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, fl
         p.x += v.x * dt;
         p.y += v.y * dt;
         p.z += v.z * dt;
+        bulk_floor(p.x, p.y, p.z, v.x, v.y, v.z, bulk_cr, bulk_vm);  // sph_step.wgsl:319
+    }
+    velu[i] = v;
+    posh[i] = p;
+}
+
+// Damped relaxation (sph_step.wgsl:340-359): v = (v + (a + w^2 (x, y, 0)) dt) * damp, x += v dt, floor.
+// Internal energy is held. One relaxation step is one force evaluation and this kernel.
+__global__ __launch_bounds__(256) void k_relax(uint32_t n, float4* posh, float4* velu, const float* __restrict__ mass,
+                                               const float4* __restrict__ acc, const float* __restrict__ dtp,
+                                               float damp, float omega, double* pos64, double* vel64,
+                                               const double* __restrict__ acc64, float4 bulk_cr, float4 bulk_vm) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !(mass[i] > 0.f)) return;
+    const float dt = *dtp;
+    float4 v = velu[i];
+    float4 p = posh[i];
+    const float w2 = omega * omega;
+    if (pos64) {
+        double x = pos64[3 * i], y = pos64[3 * i + 1], z = pos64[3 * i + 2];
+        double vx = (vel64[3 * i] + (acc64[3 * i] + (double)w2 * x) * dt) * damp;
+        double vy = (vel64[3 * i + 1] + (acc64[3 * i + 1] + (double)w2 * y) * dt) * damp;
+        double vz = (vel64[3 * i + 2] + acc64[3 * i + 2] * dt) * damp;
+        x += vx * dt;
+        y += vy * dt;
+        z += vz * dt;
+        bulk_floor(x, y, z, vx, vy, vz, bulk_cr, bulk_vm);
+        pos64[3 * i] = x;
+        pos64[3 * i + 1] = y;
+        pos64[3 * i + 2] = z;
+        vel64[3 * i] = vx;
+        vel64[3 * i + 1] = vy;
+        vel64[3 * i + 2] = vz;
+        p.x = (float)x;
+        p.y = (float)y;
+        p.z = (float)z;
+        v.x = (float)vx;
+        v.y = (float)vy;
+        v.z = (float)vz;
+    } else {
+        const float4 a = acc[i];
+        v.x = (v.x + (a.x + w2 * p.x) * dt) * damp;
+        v.y = (v.y + (a.y + w2 * p.y) * dt) * damp;
+        v.z = (v.z + a.z * dt) * damp;
+        p.x += v.x * dt;
+        p.y += v.y * dt;
+        p.z += v.z * dt;
+        bulk_floor(p.x, p.y, p.z, v.x, v.y, v.z, bulk_cr, bulk_vm);
     }
     velu[i] = v;
     posh[i] = p;
@@ -572,6 +636,10 @@ __global__ __launch_bounds__(256) void k_diag_pe(uint32_t n, const float4* posh,
 
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+static Boundary boundary(const Dev& d, const StepParams& p) {
+    return Boundary{p.bulk_cr, p.bulk_vm, d.ext, p.n_ext, p.soft2};
+}
+
 void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
     if (p.mixed)
@@ -595,17 +663,21 @@ void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
                            d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
                            d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
-                           (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot);
+                           (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot, boundary(d, p));
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
                            d.scs, d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
                            p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand,
-                           (double*)nullptr, d.sinst(), d.iroot);
+                           (double*)nullptr, d.sinst(), d.iroot, boundary(d, p));
 }
-void launch_kick_drift(const Dev& d, hipStream_t s) {
+void launch_kick_drift(const Dev& d, const StepParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,
-                       d.pos64, d.vel64, d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt);
+                       d.pos64, d.vel64, d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt, p.bulk_cr, p.bulk_vm);
+}
+void launch_relax(const Dev& d, const StepParams& p, float damp, hipStream_t s) {
+    hipLaunchKernelGGL(k_relax, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,
+                       damp, p.omega, d.pos64, d.vel64, d.acc64, p.bulk_cr, p.bulk_vm);
 }
 void launch_kick(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.velu, d.mass, d.acc, d.dt, d.vel64,

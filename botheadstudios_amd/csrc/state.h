@@ -68,6 +68,9 @@ struct Dev {
     float4* nQ0 = nullptr; // quadrupole xx, yy, zz, xy
     float2* nQ1 = nullptr; // quadrupole xz, yz
 
+    float4* ext = nullptr;       // [kMaxExtBodies] external point masses (xyz, mass)
+    uint32_t* mtarget = nullptr; // merge target of each particle (upload order)
+
     // ensembles (ninst > 1): instance k owns particles [ioff[k], ioff[k+1]) in upload and sorted order
     uint32_t ninst = 1;
     uint32_t* inst = nullptr;   // instance of each particle, upload order
@@ -108,6 +111,24 @@ struct StepParams {
     float group_extent = 4.0f;  // walk groups: box diagonal <= this many times the largest h
     int grav_ppl = 1;           // gravity walk: particles per lane (1 or 2)
     int sph_expand = 4;         // SPH walks take subtrees of <= this many particles whole as candidates
+    // un-resolved bulk (sph_step.wgsl:120-162): bulk_cr = (centre, R_core), bulk_vm = (velocity, mass);
+    // R_core == 0 none, < 0 planar (bulk_cr.xyz = plane point, bulk_vm = (up, g))
+    float4 bulk_cr = {0.f, 0.f, 0.f, 0.f};
+    float4 bulk_vm = {0.f, 0.f, 0.f, 0.f};
+    uint32_t n_ext = 0;          // external point masses (sph_step.wgsl:249-254)
+    float omega = 0.0f;          // relaxation frame spin about +z (sph_step.wgsl:353)
+    uint32_t merge_budget = 0;   // pairwise merging while n > budget (sph_step.wgsl:415-541); 0 = never
+};
+
+constexpr uint32_t kMaxExtBodies = 32;  // gpu_sph.rs:155
+constexpr uint32_t kNoMerge = 0xffffffffu;
+
+// What a force evaluation adds beyond self-gravity and SPH: the bulk and the external masses.
+struct Boundary {
+    float4 bulk_cr, bulk_vm;
+    const float4* ext;
+    uint32_t n_ext;
+    float soft2;
 };
 
 constexpr uint32_t kGroupMax = 64;    // particles per SPH walk group (one per lane)
@@ -204,7 +225,9 @@ void launch_density(const Dev& d, const StepParams& p, hipStream_t s);
 void launch_eos(const Dev& d, hipStream_t s);
 void launch_force(const Dev& d, const StepParams& p, hipStream_t s);
 
-void launch_kick_drift(const Dev& d, hipStream_t s);
+void launch_kick_drift(const Dev& d, const StepParams& p, hipStream_t s);
+void launch_relax(const Dev& d, const StepParams& p, float damp, hipStream_t s);
+void launch_merge(const Dev& d, const StepParams& p, hipStream_t s);  // pick, apply, retire (after density)
 void launch_kick(const Dev& d, hipStream_t s);
 void launch_set_dt_cfl(const Dev& d, float cfl, hipStream_t s);
 void launch_set_dt_value(const Dev& d, float dt, hipStream_t s);

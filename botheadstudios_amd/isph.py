@@ -38,6 +38,13 @@ _SIGS = {
     "isph_set_walk": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int, ctypes.c_int]),
     "isph_set_reuse": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
+    "isph_set_bulk": (ctypes.c_int, [_P] + [ctypes.c_double] * 8),
+    "isph_set_bulk_planar": (ctypes.c_int, [_P] + [ctypes.c_double] * 7),
+    "isph_set_external_masses": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    "isph_set_merge_budget": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "isph_encode_merge": (ctypes.c_int, [_P]),
+    "isph_encode_relax": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "isph_set_omega": (ctypes.c_int, [_P, ctypes.c_float]),
     "isph_autotune": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32, ctypes.c_char_p, _P]),
     "isph_encode_kdk": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "isph_step_cfl": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_uint32]),
@@ -169,6 +176,33 @@ class GpuSph:
 
     def set_precision(self, mode):
         self._check(self._lib.isph_set_precision(self._h, int(mode)))
+
+    def set_bulk(self, center, r_core, vel, mass):
+        self._check(self._lib.isph_set_bulk(self._h, *[float(x) for x in center], float(r_core),
+                                            *[float(x) for x in vel], float(mass)))
+
+    def set_bulk_planar(self, plane_point, up, g):
+        self._check(self._lib.isph_set_bulk_planar(self._h, *[float(x) for x in plane_point],
+                                                   *[float(x) for x in up], float(g)))
+
+    def set_external_masses(self, bodies):
+        """bodies: sequence of (position (3,), mass); returns how many were kept (at most 32)."""
+        arr = np.ascontiguousarray([[p[0], p[1], p[2], m] for p, m in bodies], np.float64).reshape(-1, 4)
+        rc = self._lib.isph_set_external_masses(self._h, _ptr(arr), arr.shape[0])
+        self._check(rc)
+        return rc
+
+    def set_merge_budget(self, budget):
+        self._check(self._lib.isph_set_merge_budget(self._h, int(budget)))
+
+    def encode_merge(self):
+        self._check(self._lib.isph_encode_merge(self._h))
+
+    def encode_relax(self, steps):
+        self._check(self._lib.isph_encode_relax(self._h, int(steps)))
+
+    def set_omega(self, omega):
+        self._check(self._lib.isph_set_omega(self._h, float(omega)))
 
     def set_graph(self, enable):
         self._check(self._lib.isph_set_graph(self._h, int(bool(enable))))

@@ -95,6 +95,28 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
  * of isph_upload is otherwise widened exactly; this lets the per-particle FP64 EOS use the catalogue
  * values unrounded, as the f64 CPU path (`eos::Tillotson`) does. */
 int isph_set_eos_f64(isph_t* h, const double* params10, size_t n_eos);
+/* GpuSph::set_bulk(queue, center, r_core, vel, mass) gpu_sph.rs:1364-1380: the un-resolved bulk planet, a
+ * Gauss gravity source (monopole outside r_core, linear inside) plus a non-injecting spherical floor;
+ * r_core = 0 disables it. */
+int isph_set_bulk(isph_t* h, double cx, double cy, double cz, double r_core, double vx, double vy, double vz,
+                  double mass);
+/* GpuSph::set_bulk_planar(queue, plane_point, up, g) gpu_sph.rs:1387-1403: uniform gravity g along -up and
+ * a flat floor through plane_point (up normalised, +Y when zero). */
+int isph_set_bulk_planar(isph_t* h, double px, double py, double pz, double ux, double uy, double uz, double g);
+/* GpuSph::set_external_masses(queue, bodies) gpu_sph.rs:1412-1428: point masses (x, y, z, m per body) added
+ * to every particle's gravity with the same softened form; at most 32 are kept, the count kept is returned. */
+int isph_set_external_masses(isph_t* h, const double* pos_mass4, size_t n);
+/* GpuSph::set_merge_budget(queue, budget) gpu_sph.rs:1432: merging is active while the particle count exceeds
+ * `budget` (0 = never). */
+int isph_set_merge_budget(isph_t* h, uint32_t budget);
+/* GpuSph::encode_merge(enc) gpu_sph.rs:1439-1447: one merge pass (neighbours + density, pick, apply, retire,
+ * sph_step.wgsl:374-541). */
+int isph_encode_merge(isph_t* h);
+/* GpuSph::encode_relax(enc, steps) gpu_sph.rs:1476-1481: `steps` damped relaxation steps (one force
+ * evaluation each, then v = (v + (a + w^2 (x, y, 0)) dt) damp, x += v dt; u held) at the set_dt dt and damp. */
+int isph_encode_relax(isph_t* h, uint32_t steps);
+/* SphParams::omega gpu_sph.rs:70-76: the relaxation frame's spin about +z (rad/s), 0 = none. */
+int isph_set_omega(isph_t* h, float omega);
 /* GpuSph::count gpu_sph.rs:1449 */
 uint32_t isph_count(const isph_t* h);
 /* Ensemble upload (no reference counterpart: the batching form). `n_inst` independent systems share one

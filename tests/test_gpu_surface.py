@@ -1,0 +1,165 @@
+"""The rest of the stepper surface: bulk gravity and floors, external masses, relaxation, pairwise merging.
+
+Each case restates a check of the reference's own GPU harness (tools/sph-verify/src/main.rs:370-560) with
+its configuration and bounds, plus closed-form recurrences for the relaxation kernel.
+"""
+import numpy as np
+import pytest
+
+from botheadstudios_amd import isph, materials, scene
+
+pytestmark = pytest.mark.gpu
+
+G = 6.674e-11
+SOFT = 1.0e4  # the harness's softening (main.rs:296-299)
+
+
+def _one(pos, vel=(0.0, 0.0, 0.0), h=2.0, u=1.0e5, mass=1.0e3, names=("basalt",)):
+    rec = np.zeros(1, dtype=scene.PARTICLE_DTYPE)
+    rec["pos"] = pos
+    rec["vel"] = vel
+    rec["h"] = h
+    rec["u"] = u
+    rec["mass"] = mass
+    rec["rho"] = 2700.0
+    s = isph.GpuSph(1)
+    s.upload(rec, materials.eos_table(names), SOFT)
+    s.set_eos_f64(materials.eos_f64(names))
+    return s
+
+
+def test_external_mass_pulls_at_the_softened_newtonian_rate():
+    """main.rs:489-509: one particle at rest, an Earth mass 1e7 m away; 100 KDK steps of 0.1 s."""
+    m_ext, r = 5.972e24, 1.0e7
+    with _one((0.0, 0.0, 0.0)) as s:
+        assert s.set_external_masses([((r, 0.0, 0.0), m_ext)]) == 1
+        s.set_dt(0.1, 1.0)
+        s.encode_kdk(100)
+        out = s.readback_sync()
+    a_ref = G * m_ext * r / (r * r + SOFT * SOFT) ** 1.5
+    vx = float(out["vel"][0, 0])
+    assert vx > 0 and abs(vx - a_ref * 10.0) / (a_ref * 10.0) < 1e-3
+
+
+def test_external_masses_are_capped_at_32():
+    with _one((0.0, 0.0, 0.0)) as s:
+        assert s.set_external_masses([((1e7 + k, 0.0, 0.0), 1e20) for k in range(40)]) == 32
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_planar_floor_catches_without_launch(graph):
+    """main.rs:511-531: released 100 m above a flat floor under g = 9.8: falls, rests on it, no bounce."""
+    with _one((0.0, 100.0, 0.0)) as s:
+        s.set_bulk_planar((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 9.8)
+        s.set_graph(graph)
+        s.set_dt(0.01, 1.0)
+        s.encode_kdk(3000)
+        out = s.readback_sync()
+    y = float(out["pos"][0, 1])
+    speed = float(np.linalg.norm(out["vel"][0]))
+    assert y < 99.0 and abs(y) < 0.5 and y >= -0.1 and speed < 1.0
+
+
+def test_spherical_bulk_floor_holds():
+    """main.rs:539-560: released at 1.6 R_core above a 5e22 kg bulk: falls, rests at R_core, no launch."""
+    r_core, bulk_m = 1.0e6, 5.0e22
+    start = 1.6 * r_core
+    g_surf = G * bulk_m / r_core ** 2
+    dt = 0.2 * np.sqrt(r_core / g_surf) / 20.0
+    with _one((start, 0.0, 0.0), h=4.0e4, mass=1.0e17) as s:
+        s.set_bulk((0.0, 0.0, 0.0), r_core, (0.0, 0.0, 0.0), bulk_m)
+        s.set_graph(True)
+        s.set_dt(float(dt), 1.0)
+        s.encode_kdk(6000)
+        out = s.readback_sync()
+    r = float(np.linalg.norm(out["pos"][0].astype(np.float64)))
+    speed = float(np.linalg.norm(out["vel"][0].astype(np.float64)))
+    escape = np.sqrt(2 * G * bulk_m / r_core)
+    assert r < start - 0.1 * r_core
+    assert abs(r - r_core) < 0.05 * r_core and r >= r_core - 1.0e3
+    assert speed < 0.05 * escape
+
+
+@pytest.mark.parametrize("omega", [0.0, 1.0e-3])
+def test_relaxation_follows_its_recurrence(omega):
+    """sph_step.wgsl:340-359 for one particle above a planar floor (g along -y) in a frame spinning at omega:
+    v <- (v + (a + w^2 (x, y, 0)) dt) damp, x <- x + v dt, against the same recurrence in f64."""
+    g, dt, damp, steps = 9.8, 0.05, 0.9, 40
+    x0 = np.array([50.0, 100.0, 0.0])
+    with _one(tuple(x0)) as s:
+        s.set_bulk_planar((0.0, -1.0e4, 0.0), (0.0, 1.0, 0.0), g)
+        s.set_omega(omega)
+        s.set_dt(dt, damp)
+        s.encode_relax(steps)
+        out = s.readback_sync()
+    x = x0.copy()
+    v = np.zeros(3)
+    for _ in range(steps):
+        a = np.array([0.0, -g, 0.0]) + omega ** 2 * np.array([x[0], x[1], 0.0])
+        v = (v + a * dt) * damp
+        x = x + v * dt
+    assert np.allclose(out["pos"][0], x, rtol=1e-5, atol=1e-3)
+    assert np.allclose(out["vel"][0], v, rtol=1e-4, atol=1e-4)
+
+
+def _merge_lattice(mixed=False):
+    """main.rs:391-410: a 4x4x4 lattice at 1 m spacing (h = 2 m), each site a pair 0.2 m apart with ~1 m/s
+    relative motion, hot enough for a real sound speed; site mass ~2700 kg reproduces rho0."""
+    rows = []
+    for ix in range(4):
+        for iy in range(4):
+            for iz in range(4):
+                for t in range(2):
+                    d = 0.2 * t
+                    rows.append(((ix + d, iy + d, iz), (1.0 + 0.5 * t, -0.5, 0.25 * t), 1.0e6 + 1.0e4 * t,
+                                 1350.0 + 5.0 * t))
+    rec = np.zeros(len(rows), dtype=scene.PARTICLE_DTYPE)
+    for i, (p, v, u, m) in enumerate(rows):
+        rec[i]["pos"] = p
+        rec[i]["vel"] = v
+        rec[i]["u"] = u
+        rec[i]["mass"] = m
+        rec[i]["h"] = 2.0
+        rec[i]["rho"] = 2700.0
+        rec[i]["mat"] = (i % 2) if mixed else 0
+    return rec
+
+
+def _totals(rec):
+    live = rec["mass"] > 0
+    m = rec["mass"][live].astype(np.float64)
+    v = rec["vel"][live].astype(np.float64)
+    u = rec["u"][live].astype(np.float64)
+    return m.sum(), (m[:, None] * v).sum(0), (0.5 * m * (v ** 2).sum(1) + m * u).sum(), int(live.sum())
+
+
+def _merged(rec, budget, passes=4):
+    s = isph.GpuSph(len(rec))
+    with s:
+        s.upload(rec, materials.eos_table(("basalt", "iron")), SOFT)
+        s.set_eos_f64(materials.eos_f64(("basalt", "iron")))
+        s.set_merge_budget(budget)
+        for _ in range(passes):
+            s.encode_merge()
+        return s.readback_sync()
+
+
+def test_merge_conserves_mass_momentum_energy():
+    """main.rs:411-447: over budget, pairs coalesce; mass 1e-5, momentum 1e-4, kinetic + internal 1e-4;
+    every particle stays finite (retired ones included)."""
+    rec = _merge_lattice()
+    m0, p0, e0, n0 = _totals(rec)
+    out = _merged(rec, budget=1)
+    m1, p1, e1, n1 = _totals(out)
+    assert n1 < n0
+    assert abs(m1 - m0) / m0 < 1e-5
+    assert np.abs(p1 - p0).max() / max(np.abs(p0).max(), 1.0) < 1e-4
+    assert abs(e1 - e0) / e0 < 1e-4
+    for f in ("pos", "vel", "u", "mass"):
+        assert np.isfinite(out[f]).all()
+
+
+def test_merge_gates_materials_and_budget():
+    """main.rs:449-482: iron + basalt pairs never merge; with budget 0 nothing merges."""
+    assert _totals(_merged(_merge_lattice(mixed=True), budget=1))[3] == 128
+    assert _totals(_merged(_merge_lattice(), budget=0))[3] == 128
