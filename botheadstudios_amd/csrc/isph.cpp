@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "adjoint.h"
+#include "measure.h"
 #include "state.h"
 
 using namespace isph;
@@ -84,6 +86,16 @@ struct isph_ctx {
     hipGraphExec_t graph_exec = nullptr;
     int graph_mode = -1;  // 0 fixed dt, 1 Courant dt
     float graph_value = 0.0f;
+
+    // measurement (isph_disk_view / isph_find_clumps): device buffers allocated on first use, the instance
+    // offsets of the current upload, and the last measurement's host copies
+    Meas meas;
+    std::vector<uint32_t> hioff{0u, 0u};
+    double clump_soft = 1.0e4;  // gpu_sph.rs:805
+    double link_factor = 2.0;   // gpu_sph.rs:803
+    std::vector<double> mview;
+    std::vector<uint32_t> mmoff;
+    std::vector<unsigned long long> mcsum;
 };
 
 namespace {
@@ -103,6 +115,12 @@ void free_all(isph_ctx* c) {
         if (q) (void)hipFree(q);
     void* eptrs[] = {d.inst, d.mcode, d.lkey, d.ioff, d.ibox, d.iroot, d.isig, d.idt, d.itime, d.ext, d.mtarget};
     for (void* q : eptrs)
+        if (q) (void)hipFree(q);
+    const Meas& m = c->meas;
+    void* mptrs[] = {m.chunks, m.cinst, m.ichunk, m.part, m.coff, m.view, m.cross, m.k0, m.k1, m.v0, m.v1, m.khi,
+                     m.scum, m.slot, m.mbase, m.moff, m.mp, m.mv, m.minst, m.midx, m.iparam, m.parent, m.label,
+                     m.cflag, m.cstart, m.tcnt, m.cp, m.cv, m.crec, m.tpe, m.word, m.csum, m.sel, m.srec};
+    for (void* q : mptrs)
         if (q) (void)hipFree(q);
     const Adj& a = c->adj;
     void* aptrs[] = {a.xb, a.vb, a.ub, a.mb, a.ab, a.sa, a.sxb, a.svb, a.srb, a.sqb, a.scb, a.nbar, a.pbar, a.red};
@@ -603,6 +621,8 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
         h->n_eos = (uint32_t)ne;
         set_leaves(h, (uint32_t)take);
         h->d.ninst = 1;
+        h->hioff = {0u, (uint32_t)take};
+        h->meas.valid = false;
         h->softening = softening;
         h->p.soft2 = softening * softening;
         if (h->precision == ISPH_PRECISION_MIXED && take > 0) {
@@ -659,6 +679,7 @@ int isph_upload_ensemble(isph_t* h, const isph_particle* particles, const uint32
         HIP_OK(hipMemset(d.idt, 0, n_inst * sizeof(float)));
         HIP_OK(hipMemset(d.itime, 0, n_inst * sizeof(double)));
         d.ninst = n_inst;
+        h->hioff.assign(offsets, offsets + n_inst + 1);
         h->reuse_valid = false;
         drop_graph(h);
         return (int)n;
@@ -1019,6 +1040,325 @@ int isph_force_eval(isph_t* h, float* acc4, float* rho) {
         if (acc4) HIP_OK(hipMemcpy(acc4, h->d.acc, (size_t)h->d.n * sizeof(float4), hipMemcpyDeviceToHost));
         if (rho) HIP_OK(hipMemcpy(rho, h->d.rho, (size_t)h->d.n * sizeof(float), hipMemcpyDeviceToHost));
         return ISPH_OK;
+    });
+}
+
+// ---- measurement -------------------------------------------------------------------------------------
+namespace {
+
+void ensure_meas(isph_ctx* h) {
+    Meas& m = h->meas;
+    if (m.cap == h->d.cap && m.word) return;
+    const size_t cap = std::max<uint32_t>(h->d.cap, 1u);
+    const size_t cap_inst = 256, cap_chunks = (cap + kMeasChunk - 1) / kMeasChunk + cap_inst;
+    dalloc(&m.chunks, cap_chunks);
+    dalloc(&m.cinst, cap_chunks);
+    dalloc(&m.ichunk, cap_inst + 1);
+    dalloc(&m.part, cap_chunks * kPartFields);
+    dalloc(&m.coff, cap_chunks);
+    dalloc(&m.view, cap_inst * kViewFields);
+    dalloc(&m.cross, cap_inst);
+    dalloc(&m.k0, cap);
+    dalloc(&m.k1, cap);
+    dalloc(&m.v0, cap);
+    dalloc(&m.v1, cap);
+    dalloc(&m.khi, cap);
+    dalloc(&m.scum, cap);
+    dalloc(&m.slot, cap);
+    dalloc(&m.mbase, cap_chunks);
+    dalloc(&m.moff, cap_inst + 1);
+    dalloc(&m.mp, cap);
+    dalloc(&m.mv, cap);
+    dalloc(&m.minst, cap);
+    dalloc(&m.midx, cap);
+    dalloc(&m.iparam, cap_inst * kParamFields);
+    dalloc(&m.parent, cap);
+    dalloc(&m.label, cap);
+    dalloc(&m.cflag, cap + 1);
+    dalloc(&m.cstart, cap + 1);
+    dalloc(&m.tcnt, cap + 1);
+    dalloc(&m.cp, cap);
+    dalloc(&m.cv, cap);
+    dalloc(&m.crec, cap * kClumpFields);
+    dalloc(&m.word, 24);
+    dalloc(&m.csum, 3 * cap_inst);
+    dalloc(&m.sel, cap);
+    dalloc(&m.srec, cap * kClumpFields);
+    m.cap = h->d.cap;
+    m.cap_inst = (uint32_t)cap_inst;
+    m.cap_chunks = (uint32_t)cap_chunks;
+}
+
+// Chunks of at most kMeasChunk particles that never span two instances.
+void meas_chunks(isph_ctx* h, const std::vector<uint32_t>& ioff) {
+    Meas& m = h->meas;
+    const uint32_t ninst = (uint32_t)ioff.size() - 1;
+    std::vector<uint2> ch;
+    std::vector<uint32_t> ci, ic(ninst + 1);
+    for (uint32_t k = 0; k < ninst; ++k) {
+        ic[k] = (uint32_t)ch.size();
+        for (uint32_t s = ioff[k]; s < ioff[k + 1]; s += kMeasChunk) {
+            ch.push_back(make_uint2(s, std::min(s + kMeasChunk, ioff[k + 1])));
+            ci.push_back(k);
+        }
+    }
+    ic[ninst] = (uint32_t)ch.size();
+    if (ch.size() > m.cap_chunks) throw std::runtime_error("measurement chunk table overflow");
+    m.nchunks = (uint32_t)ch.size();
+    if (!ch.empty()) {
+        HIP_OK(hipMemcpy(m.chunks, ch.data(), ch.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(m.cinst, ci.data(), ci.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    HIP_OK(hipMemcpy(m.ichunk, ic.data(), ic.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+}
+
+uint32_t read_word(isph_ctx* h, int k) {
+    uint32_t w = 0;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(&w, h->meas.word + k, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return w;
+}
+
+// Friends-of-friends, statistics, self energy and gates over members [0, n) of h->meas.
+void run_clumps(isph_ctx* h, uint32_t n, uint32_t ninst) {
+    Meas& m = h->meas;
+    m.n_clumps = 0;
+    if (n > 0) {
+        launch_fof(m, n, h->d.hist, h->stream);
+        if (read_word(h, 9) != 0) throw std::runtime_error("friends-of-friends: a union did not settle");
+        const uint32_t nc = read_word(h, 8);
+        launch_clump_stats(m, nc, h->stream);
+        uint32_t tiles = 0;
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpy(&tiles, m.tcnt + nc, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (tiles > m.tpe_cap) {
+            if (m.tpe) HIP_OK(hipFree(m.tpe));
+            m.tpe = nullptr;
+            m.tpe_cap = std::max<size_t>(tiles, 2 * m.tpe_cap);
+            dalloc(&m.tpe, m.tpe_cap);
+        }
+        launch_clump_pe(m, nc, tiles, h->stream);
+        m.n_clumps = nc;
+    }
+    launch_clump_summary(m, m.n_clumps, ninst, h->stream);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipGetLastError());
+}
+
+// The records of instance `inst`'s clumps with >= min_members members, heaviest first (ties by id), with
+// ids made relative to `base`.
+uint32_t fetch_clumps(isph_ctx* h, uint32_t inst, uint32_t min_members, uint32_t base, isph_clump* out,
+                      uint32_t cap) {
+    Meas& m = h->meas;
+    launch_clump_select(m, m.n_clumps, inst, min_members, h->stream);
+    const uint32_t count = read_word(h, 16);
+    if (count == 0) return 0;
+    launch_clump_gather(m, count, h->stream);
+    std::vector<double> rec((size_t)count * kClumpFields);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(rec.data(), m.srec, rec.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> order(count);
+    for (uint32_t k = 0; k < count; ++k) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const double ma = rec[(size_t)a * kClumpFields + kCMass], mb = rec[(size_t)b * kClumpFields + kCMass];
+        if (ma != mb) return ma > mb;
+        return rec[(size_t)a * kClumpFields + kCId] < rec[(size_t)b * kClumpFields + kCId];
+    });
+    for (uint32_t k = 0; k < std::min(count, cap); ++k) {
+        const double* r = &rec[(size_t)order[k] * kClumpFields];
+        isph_clump& c = out[k];
+        c.id = (uint32_t)r[kCId] - base;
+        c.n_members = (uint32_t)r[kCN];
+        c.bound = r[kCBound] != 0.0;
+        c.outside_roche = r[kCOutside] != 0.0;
+        c.mass = r[kCMass];
+        for (int a = 0; a < 3; ++a) {
+            c.com_pos[a] = r[kCCom + a];
+            c.com_vel[a] = r[kCVel + a];
+            c.ang_mom[a] = r[kCL + a];
+        }
+        c.rho = r[kCRho];
+        c.radius = r[kCRadius];
+        c.internal_ke = r[kCKe];
+        c.self_pe = r[kCPe];
+        c.thermal_ke = r[kCThermal];
+    }
+    return count;
+}
+
+}  // namespace
+
+int isph_set_measure(isph_t* h, double clump_softening, double link_factor) {
+    return guarded(h, [&]() -> int {
+        if (!(clump_softening >= 0.0) || !(link_factor > 0.0)) return fail(h, ISPH_EINVAL, "bad measure parameters");
+        h->clump_soft = clump_softening;
+        h->link_factor = link_factor;
+        return ISPH_OK;
+    });
+}
+
+int isph_disk_view(isph_t* h, isph_disk* out, uint32_t cap) {
+    return guarded(h, [&]() -> int {
+        if (!out && cap > 0) return fail(h, ISPH_EINVAL, "NULL output");
+        HIP_OK(hipSetDevice(h->device));
+        Dev& d = h->d;
+        Meas& m = h->meas;
+        const uint32_t ninst = d.n == 0 ? 1u : d.ninst;
+        m.valid = false;
+        h->mview.assign((size_t)ninst * kViewFields, 0.0);
+        h->mmoff.assign(ninst + 1, 0u);
+        h->mcsum.assign(3 * (size_t)ninst, 0ull);
+        if (d.n > 0) {
+            ensure_meas(h);
+            HIP_OK(hipStreamSynchronize(h->stream));
+            meas_chunks(h, h->hioff);
+            launch_meas_live(d, m, ninst, h->stream);
+            launch_meas_remnant(d, m, ninst, h->stream);
+            launch_meas_classify(d, m, ninst, h->clump_soft, h->link_factor, h->stream);
+            m.n_members = read_word(h, 0);
+            run_clumps(h, m.n_members, ninst);
+            HIP_OK(hipMemcpy(h->mview.data(), m.view, h->mview.size() * sizeof(double), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(h->mmoff.data(), m.moff, h->mmoff.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(h->mcsum.data(), m.csum, h->mcsum.size() * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+        }
+        m.ninst = ninst;
+        m.valid = true;
+        for (uint32_t k = 0; k < std::min(cap, ninst); ++k) {
+            const double* v = &h->mview[(size_t)k * kViewFields];
+            isph_disk& o = out[k];
+            std::memset(&o, 0, sizeof(o));
+            o.valid = v[kVValid] != 0.0;
+            o.n_live = (uint32_t)v[kVNLive];
+            if (!o.valid) continue;
+            o.n_disk = (uint32_t)v[kVNDisk];
+            const bool clumps = o.n_disk >= 2;  // gpu_sph.rs:794
+            o.n_clumps = clumps ? (uint32_t)h->mcsum[3 * k] : 0u;
+            o.n_bound = clumps ? (uint32_t)h->mcsum[3 * k + 1] : 0u;
+            if (clumps) {
+                const unsigned long long bits = h->mcsum[3 * k + 2];
+                std::memcpy(&o.moon_mass, &bits, sizeof(double));
+            }
+            for (int a = 0; a < 3; ++a) {
+                o.com[a] = v[kVCom + a];
+                o.v_com[a] = v[kVVCom + a];
+            }
+            o.r_remnant = v[kVRRem];
+            o.m_remnant = v[kVMRem];
+            o.mu = v[kVMu];
+            o.disk_prov0 = v[kVDisk0];
+            o.disk_other = v[kVDiskO];
+            o.escaped = v[kVEsc];
+            o.mean_h = v[kVMeanH];
+            o.remnant_prov0 = v[kVRem0];
+            o.remnant_other = v[kVRemO];
+            o.escaped_prov0 = v[kVEsc0];
+        }
+        return (int)ninst;
+    });
+}
+
+namespace {
+int disk_instance(isph_ctx* h, uint32_t inst) {
+    if (!h->meas.valid) return fail(h, ISPH_EINVAL, "no disk view: call isph_disk_view first");
+    if (inst >= h->meas.ninst) return fail(h, ISPH_ERANGE, "instance out of range");
+    return ISPH_OK;
+}
+}  // namespace
+
+int isph_disk_members(isph_t* h, uint32_t inst, uint32_t* idx, uint32_t* clump_id, uint32_t cap) {
+    return guarded(h, [&]() -> int {
+        int rc = disk_instance(h, inst);
+        if (rc < 0) return rc;
+        if (h->mview[(size_t)inst * kViewFields + kVValid] == 0.0) return 0;
+        const uint32_t b = h->mmoff[inst], n = h->mmoff[inst + 1] - b, k = std::min(n, cap);
+        if (k > 0 && idx)
+            HIP_OK(hipMemcpy(idx, h->meas.midx + b, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (k > 0 && clump_id) {
+            HIP_OK(hipMemcpy(clump_id, h->meas.label + b, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < k; ++i) clump_id[i] -= b;
+        }
+        return (int)n;
+    });
+}
+
+int isph_disk_clumps(isph_t* h, uint32_t inst, uint32_t min_members, isph_clump* out, uint32_t cap) {
+    return guarded(h, [&]() -> int {
+        int rc = disk_instance(h, inst);
+        if (rc < 0) return rc;
+        if (!out && cap > 0) return fail(h, ISPH_EINVAL, "NULL output");
+        const double* v = &h->mview[(size_t)inst * kViewFields];
+        if (v[kVValid] == 0.0 || v[kVNDisk] < 2.0) return 0;  // gpu_sph.rs:794
+        HIP_OK(hipSetDevice(h->device));
+        return (int)fetch_clumps(h, inst, std::max(min_members, 1u), h->mmoff[inst], out, cap);
+    });
+}
+
+int isph_disk_stats_json(isph_t* h, uint32_t inst, char* buf, size_t cap) {
+    return guarded(h, [&]() -> int {
+        int rc = disk_instance(h, inst);
+        if (rc < 0) return rc;
+        const double* v = &h->mview[(size_t)inst * kViewFields];
+        std::string txt = "null";
+        if (v[kVValid] != 0.0) {  // gpu_sph.rs:825-851
+            constexpr double kMMoon = 7.342e22;
+            const double disk = v[kVDisk0] + v[kVDiskO];
+            const double earth_pct = disk > 0.0 ? 100.0 * v[kVDisk0] / disk : 0.0;
+            double moon = 0.0;
+            if (v[kVNDisk] >= 2.0) std::memcpy(&moon, &h->mcsum[3 * (size_t)inst + 2], sizeof(double));
+            char tmp[256];
+            std::snprintf(tmp, sizeof(tmp),
+                          "{\"disk\":%.3f,\"earth_pct\":%.0f,\"remnant_km\":%.0f,\"escaped\":%.3f,\"moon\":%.3f}",
+                          disk / kMMoon, earth_pct, v[kVRRem] / 1e3, v[kVEsc] / kMMoon, moon / kMMoon);
+            txt = tmp;
+        }
+        if (buf && cap > 0) {
+            const size_t k = std::min(cap - 1, txt.size());
+            std::memcpy(buf, txt.data(), k);
+            buf[k] = '\0';
+        }
+        return (int)txt.size();
+    });
+}
+
+int isph_find_clumps(isph_t* h, const double* pos3, const double* vel3, const double* mass, const double* rho,
+                     size_t n, double linking_length, double g, double softening, const double* central_pos3,
+                     double central_mass, double central_radius, uint32_t min_members, isph_clump* out,
+                     uint32_t cap, uint32_t* labels) {
+    return guarded(h, [&]() -> int {
+        if (n > 0 && (!pos3 || !vel3 || !mass || !rho)) return fail(h, ISPH_EINVAL, "NULL array");
+        if (!central_pos3) return fail(h, ISPH_EINVAL, "NULL central position");
+        if (!out && cap > 0) return fail(h, ISPH_EINVAL, "NULL output");
+        if (n > h->d.cap) return fail(h, ISPH_ERANGE, "n exceeds the handle capacity");
+        if (!(linking_length >= 0.0) || !(softening >= 0.0)) return fail(h, ISPH_EINVAL, "bad linking length");
+        HIP_OK(hipSetDevice(h->device));
+        ensure_meas(h);
+        Meas& m = h->meas;
+        m.valid = false;
+        HIP_OK(hipStreamSynchronize(h->stream));
+        if (n == 0) return 0;
+        std::vector<double4> p(n), v(n);
+        for (size_t i = 0; i < n; ++i) {
+            p[i] = make_double4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], mass[i]);
+            v[i] = make_double4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], rho[i]);
+        }
+        double pr[kParamFields] = {0};
+        pr[kPLink] = linking_length;
+        pr[kPSoft2] = softening * softening;
+        pr[kPG] = g;
+        for (int a = 0; a < 3; ++a) pr[kPCPos + a] = central_pos3[a];
+        pr[kPCMass] = central_mass;
+        pr[kPCRad] = central_radius;
+        HIP_OK(hipMemcpy(m.mp, p.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(m.mv, v.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+        HIP_OK(hipMemset(m.minst, 0, n * sizeof(uint32_t)));
+        HIP_OK(hipMemcpy(m.iparam, pr, sizeof(pr), hipMemcpyHostToDevice));
+        run_clumps(h, (uint32_t)n, 1u);
+        if (labels) HIP_OK(hipMemcpy(labels, m.label, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        return (int)fetch_clumps(h, 0u, std::max(min_members, 1u), 0u, out, cap);
     });
 }
 

@@ -256,6 +256,14 @@ void launch_radix_pass(const Dev& d, const uint32_t* kin, const uint32_t* vin, u
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, kin, vin, kout, vout, d.n, shift, d.hist,
                        nb);
 }
+void launch_radix_pass_n(uint32_t n, uint32_t* hist, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                         uint32_t* vout, int shift, hipStream_t s) {
+    if (n == 0) return;
+    uint32_t nb = cdiv(n, kSortTile);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, kin, n, shift, hist, nb);
+    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, hist, 256u * nb);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n, shift, hist, nb);
+}
 void launch_exclusive_scan(uint32_t* data, uint32_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, data, count);
 }

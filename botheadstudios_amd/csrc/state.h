@@ -215,6 +215,9 @@ void launch_karras(const Dev& d, hipStream_t s);
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s);
 void launch_groups(const Dev& d, const StepParams& p, uint32_t maxc, uint2* out, uint32_t* total, hipStream_t s);
 void launch_exclusive_scan(uint32_t* data, uint32_t count, hipStream_t s);
+// One stable 8-bit LSD pass over n (key, value) pairs; hist holds 256 * ceil(n / 4096) counters.
+void launch_radix_pass_n(uint32_t n, uint32_t* hist, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                         uint32_t* vout, int shift, hipStream_t s);
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s);
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s);

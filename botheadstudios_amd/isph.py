@@ -70,6 +70,14 @@ _SIGS = {
     "isph_set_state_f64": (ctypes.c_int, [_P, _P, _P]),
     "isph_vjp_run": (ctypes.c_int, [_P, ctypes.c_float, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
                                     _P, _P, _P, _P]),
+    "isph_set_measure": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
+    "isph_disk_view": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
+    "isph_disk_members": (ctypes.c_int, [_P, ctypes.c_uint32, _P, _P, ctypes.c_uint32]),
+    "isph_disk_clumps": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, _P, ctypes.c_uint32]),
+    "isph_disk_stats_json": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]),
+    "isph_find_clumps": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_size_t, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, _P, ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _P,
+                                        ctypes.c_uint32, _P]),
 }
 
 
@@ -89,6 +97,34 @@ class Loss(ctypes.Structure):
     @classmethod
     def make(cls, radius, r_cut=2.9, k_r=20.0, k_j=10.0, axis=(0.0, 0.0, 1.0)):
         return cls(float(radius), float(r_cut), float(k_r), float(k_j), (ctypes.c_double * 3)(*axis))
+
+class Disk(ctypes.Structure):
+    """`isph_disk`: DiskView (gpu_sph.rs:661-686) of one instance plus impact-run's per-source split."""
+    _fields_ = [("valid", ctypes.c_int32), ("n_live", ctypes.c_uint32), ("n_disk", ctypes.c_uint32),
+                ("n_clumps", ctypes.c_uint32), ("n_bound", ctypes.c_uint32), ("_pad", ctypes.c_uint32),
+                ("com", ctypes.c_double * 3), ("v_com", ctypes.c_double * 3), ("r_remnant", ctypes.c_double),
+                ("m_remnant", ctypes.c_double), ("mu", ctypes.c_double), ("disk_prov0", ctypes.c_double),
+                ("disk_other", ctypes.c_double), ("escaped", ctypes.c_double), ("mean_h", ctypes.c_double),
+                ("remnant_prov0", ctypes.c_double), ("remnant_other", ctypes.c_double),
+                ("escaped_prov0", ctypes.c_double), ("moon_mass", ctypes.c_double)]
+
+    def disk_mass(self):
+        """gpu_sph.rs:788-790."""
+        return self.disk_prov0 + self.disk_other
+
+
+class Clump(ctypes.Structure):
+    """`isph_clump`: accretion.rs:37-51 without the member list."""
+    _fields_ = [("id", ctypes.c_uint32), ("n_members", ctypes.c_uint32), ("bound", ctypes.c_int32),
+                ("outside_roche", ctypes.c_int32), ("mass", ctypes.c_double), ("com_pos", ctypes.c_double * 3),
+                ("com_vel", ctypes.c_double * 3), ("rho", ctypes.c_double), ("radius", ctypes.c_double),
+                ("internal_ke", ctypes.c_double), ("self_pe", ctypes.c_double), ("ang_mom", ctypes.c_double * 3),
+                ("thermal_ke", ctypes.c_double)]
+
+    def accretes(self):
+        """accretion.rs:55-57."""
+        return self.n_members >= 2 and bool(self.bound) and bool(self.outside_roche)
+
 
 _lib = None
 
@@ -336,6 +372,61 @@ class GpuSph:
 
     def reset_stats(self):
         self._check(self._lib.isph_reset_stats(self._h))
+
+    # ---- measurement (DiskView, accretion::find_clumps) ----
+    def set_measure(self, clump_softening=1.0e4, link_factor=2.0):
+        self._check(self._lib.isph_set_measure(self._h, float(clump_softening), float(link_factor)))
+
+    def disk_view(self):
+        """DiskView::of (gpu_sph.rs:689-785) of every instance, measured on the device: a list of Disk."""
+        k = 256  # the most instances an ensemble holds
+        out = (Disk * k)()
+        rc = self._lib.isph_disk_view(self._h, out, k)
+        self._check(rc)
+        return list(out)[:rc]
+
+    def disk_members(self, inst=0):
+        """DiskView's idx (indices into the instance's particles) and each member's clump id."""
+        n = self._lib.isph_disk_members(self._h, int(inst), None, None, 0)
+        self._check(n)
+        idx = np.zeros(n, np.uint32)
+        cid = np.zeros(n, np.uint32)
+        self._check(self._lib.isph_disk_members(self._h, int(inst), _ptr(idx), _ptr(cid), n))
+        return idx, cid
+
+    def disk_clumps(self, inst=0, min_members=1):
+        """DiskView::clumps (gpu_sph.rs:793-809), heaviest first; min_members filters by size."""
+        n = self._lib.isph_disk_clumps(self._h, int(inst), int(min_members), None, 0)
+        self._check(n)
+        out = (Clump * max(n, 1))()
+        self._check(self._lib.isph_disk_clumps(self._h, int(inst), int(min_members), out, n))
+        return list(out)[:n]
+
+    def disk_stats_json(self, inst=0):
+        """disk_stats_json (gpu_sph.rs:825-851)."""
+        n = self._lib.isph_disk_stats_json(self._h, int(inst), None, 0)
+        self._check(n)
+        buf = ctypes.create_string_buffer(n + 1)
+        self._check(self._lib.isph_disk_stats_json(self._h, int(inst), buf, n + 1))
+        return buf.value.decode()
+
+    def find_clumps(self, pos, vel, mass, rho, linking_length, g, softening, central_pos, central_mass,
+                    central_radius, min_members=1):
+        """accretion::find_clumps (accretion.rs:158-301) on the device: (clumps heaviest first, labels)."""
+        p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
+        m = np.ascontiguousarray(mass, np.float64)
+        r = np.ascontiguousarray(rho, np.float64)
+        c = np.ascontiguousarray(central_pos, np.float64).reshape(3)
+        n = p.shape[0]
+        labels = np.zeros(n, np.uint32)
+        cap = n
+        out = (Clump * max(cap, 1))()
+        rc = self._lib.isph_find_clumps(self._h, _ptr(p), _ptr(v), _ptr(m), _ptr(r), n, float(linking_length),
+                                        float(g), float(softening), _ptr(c), float(central_mass),
+                                        float(central_radius), int(min_members), out, cap, _ptr(labels))
+        self._check(rc)
+        return list(out)[:rc], labels
 
     # -- gravity (GpuGravity) --
     def gravity_accelerations(self, pos, mass, softening, mode=GRAVITY_DIRECT):

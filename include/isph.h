@@ -244,6 +244,67 @@ int isph_vjp_run(isph_t* h, float cfl, const float* dts_in, uint32_t steps, uint
                  const isph_loss* loss, double* loss_out, double* dpos3, double* dvel3, double* du, double* dmass,
                  double* deos4, float* dts_out, double* ms2);
 
+/* ---- measurement (DiskView, gpu_sph.rs:652-823; accretion::find_clumps, accretion.rs:158-301) --------- */
+/* The disk of one system: DiskView's fields (gpu_sph.rs:661-686) plus impact-run's split of the remnant and
+ * escaped mass by source body and its clump counts (tools/impact-run/src/main.rs:732-778). f64, SI. */
+typedef struct isph_disk {
+    int32_t valid;     /* 0 where DiskView::of is None (fewer than 2 live particles, or no mass): rest unset */
+    uint32_t n_live;   /* particles with positive finite mass and finite position and velocity */
+    uint32_t n_disk;   /* disk members: bound to the remnant with perigee above its radius */
+    uint32_t n_clumps; /* disk clumps with at least 2 members (impact-run's n_clumps) */
+    uint32_t n_bound;  /* of those, self-bound (impact-run's n_bound) */
+    uint32_t _pad;
+    double com[3], v_com[3];                 /* frame: centre of mass of the live field and its velocity */
+    double r_remnant, m_remnant, mu;         /* smallest radius enclosing 85 % of the mass, that mass, G m */
+    double disk_prov0, disk_other, escaped;  /* disk mass from source body 0 / the rest; unbound mass */
+    double mean_h;                           /* mean smoothing length of the live field */
+    double remnant_prov0, remnant_other;     /* bound with perigee inside the remnant, by source body */
+    double escaped_prov0;                    /* unbound mass from source body 0 */
+    double moon_mass;                        /* largest clump that accretes (disk_stats_json's "moon"), or 0 */
+} isph_disk;
+
+/* One friends-of-friends clump: `Clump` accretion.rs:37-51 (members are returned separately).
+ * Clump::accretes() == n_members >= 2 && bound && outside_roche. */
+typedef struct isph_clump {
+    uint32_t id;        /* smallest member index: a position in isph_disk_members' list, or an input index */
+    uint32_t n_members;
+    int32_t bound;         /* internal_ke + self_pe < 0 */
+    int32_t outside_roche; /* COM beyond the remnant's fluid Roche limit for this clump's density */
+    double mass, com_pos[3], com_vel[3], rho, radius, internal_ke, self_pe, ang_mom[3], thermal_ke;
+} isph_clump;
+
+#ifdef __cplusplus
+static_assert(sizeof(isph_disk) == 160, "isph_disk layout");
+static_assert(sizeof(isph_clump) == 136, "isph_clump layout");
+#endif
+
+/* DiskView::clumps' softening (1e4 m) and linking length factor (2 x mean h), gpu_sph.rs:796-808. */
+int isph_set_measure(isph_t* h, double clump_softening, double link_factor);
+/* DiskView::of (gpu_sph.rs:689-785) of the current state, on the device, for every instance (one record for
+ * a single system), with the disk's clumps (DiskView::clumps, gpu_sph.rs:793-809). Writes min(cap, count)
+ * records and returns the instance count. The members and clumps below refer to this measurement until the
+ * next isph_disk_view or isph_find_clumps call. */
+int isph_disk_view(isph_t* h, isph_disk* out, uint32_t cap);
+/* DiskView's `idx` (indices into the instance's own particles, ascending) and each member's clump id.
+ * Either output may be NULL. Returns the member count. */
+int isph_disk_members(isph_t* h, uint32_t inst, uint32_t* idx, uint32_t* clump_id, uint32_t cap);
+/* The instance's clumps with at least min_members members, heaviest first (ties by id); an instance with
+ * fewer than 2 disk members has none (gpu_sph.rs:794). moonlets(k) = the ones that accrete with
+ * n_members >= k (:813-820). Returns the count; writes min(cap, count). */
+int isph_disk_clumps(isph_t* h, uint32_t inst, uint32_t min_members, isph_clump* out, uint32_t cap);
+/* disk_stats_json (gpu_sph.rs:825-851) of the measured instance, snprintf-style: returns the length of the
+ * full text ("null" when the view is None). */
+int isph_disk_stats_json(isph_t* h, uint32_t inst, char* buf, size_t cap);
+/* accretion::find_clumps (accretion.rs:158-301) on caller arrays (f64, n <= capacity): friends-of-friends
+ * within linking_length, then per clump the statistics, the softened self potential energy (an exact pair
+ * sum at every size) times g / G, and the Roche gate against the given central body. Writes the clumps
+ * with >= min_members members, heaviest first, and labels[n] = each particle's clump id (the smallest
+ * member index). Returns the count. Leaves no disk view behind. */
+int isph_find_clumps(isph_t* h, const double* pos3, const double* vel3, const double* mass, const double* rho,
+                     size_t n, double linking_length, double g, double softening, const double* central_pos3,
+                     double central_mass, double central_radius, uint32_t min_members, isph_clump* out,
+                     uint32_t cap, uint32_t* labels);
+
 /* ---- stage probes (parity of the bit-exact stages against the oracle) ------------------------------- */
 /* bbox + Morton of the current positions: codes in upload order, lo3/hi3 the f32 bbox (cs_bbox/cs_morton,
  * bh_gravity.wgsl:67-108). */

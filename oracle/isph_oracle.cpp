@@ -1340,3 +1340,283 @@ void orc_dual_gradient(uint64_t n, const double* pos3, const double* vel3, const
     for (int k = 0; k < kDualN; ++k) grad7[k] = L.d[k];
 }
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------------------
+// Disk measurement: orbit::is_bound / perigee (orbit.rs:91-117), DiskView::of (gpu_sph.rs:689-785) with
+// impact-run's split of the remnant and escaped mass by source body (tools/impact-run/src/main.rs:749-760),
+// and accretion::find_clumps (accretion.rs:158-301).
+namespace {
+
+constexpr double kFourThirdsPi = 4.0 / 3.0 * 3.14159265358979323846;  // accretion.rs:31
+constexpr double kClumpTheta = 0.5;                                 // accretion.rs:34
+
+inline V3 cross3(V3 a, V3 b) { return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+
+bool orbit_is_bound(V3 p, V3 v, double mu) {  // orbit.rs:91-94
+    double r = length(p);
+    return r > 0.0 && 0.5 * dot(v, v) - mu / r < 0.0;
+}
+bool orbit_perigee(V3 p, V3 v, double mu, double* q) {  // orbit.rs:96-117; false = None
+    double r = length(p);
+    if (r == 0.0) {
+        *q = 0.0;
+        return true;
+    }
+    double energy = 0.5 * dot(v, v) - mu / r;
+    double h = length(cross3(p, v));
+    double e = std::sqrt(std::fmax(1.0 + 2.0 * energy * h * h / (mu * mu), 0.0));
+    if (energy >= 0.0) {
+        if (e <= 1.0 || h == 0.0) return false;
+        *q = h * h / mu / (1.0 + e);
+        return true;
+    }
+    double a = -mu / (2.0 * energy);
+    *q = a * (1.0 - e);
+    return true;
+}
+
+struct OrcParticle {  // gpu_sph.rs:27-37
+    float pos[3];
+    float h;
+    float vel[3];
+    float u;
+    float mass;
+    uint32_t mat;
+    float rho;
+    uint32_t prov;
+};
+
+bool part_live(const OrcParticle& p) {  // gpu_sph.rs:700-707
+    return p.mass > 0.0f && std::isfinite(p.mass) && std::isfinite(p.pos[0]) && std::isfinite(p.pos[1]) &&
+           std::isfinite(p.pos[2]) && std::isfinite(p.vel[0]) && std::isfinite(p.vel[1]) && std::isfinite(p.vel[2]);
+}
+
+// Clump record fields (doubles), shared with the tests.
+enum {
+    CR_ID = 0, CR_N, CR_MASS, CR_COM, CR_VEL = 6, CR_RHO = 9, CR_RADIUS, CR_KE, CR_PE, CR_L, CR_THERMAL = 16,
+    CR_BOUND, CR_OUTSIDE, CR_FIELDS = 20
+};
+
+}  // namespace
+
+extern "C" {
+
+bool orc_orbit_is_bound(const double* p3, const double* v3, double mu) {
+    return orbit_is_bound(V3{p3[0], p3[1], p3[2]}, V3{v3[0], v3[1], v3[2]}, mu);
+}
+// Returns 0 for None, 1 with *q set.
+int orc_orbit_perigee(const double* p3, const double* v3, double mu, double* q) {
+    return orbit_perigee(V3{p3[0], p3[1], p3[2]}, V3{v3[0], v3[1], v3[2]}, mu, q) ? 1 : 0;
+}
+
+// DiskView::of over `n` 48-byte particles. view[20]: [0] valid, [1] live count, [2..4] com, [5..7] v_com,
+// [8] r_remnant, [9] m_remnant, [10] mu, [11] disk_prov0, [12] disk_other, [13] escaped, [14] mean_h,
+// [15] remnant prov 0, [16] remnant other, [17] escaped prov 0, [18] disk count, [19] total live mass.
+// idx (capacity n) receives the disk members' indices in order.
+void orc_disk_view(const void* particles, uint64_t n, double* view, uint32_t* idx) {
+    const OrcParticle* ps = static_cast<const OrcParticle*>(particles);
+    for (int k = 0; k < 20; ++k) view[k] = 0.0;
+    auto pos = [](const OrcParticle& p) { return V3{(double)p.pos[0], (double)p.pos[1], (double)p.pos[2]}; };
+    auto vel = [](const OrcParticle& p) { return V3{(double)p.vel[0], (double)p.vel[1], (double)p.vel[2]}; };
+    std::vector<const OrcParticle*> live;
+    for (uint64_t i = 0; i < n; ++i)
+        if (part_live(ps[i])) live.push_back(&ps[i]);
+    view[1] = (double)live.size();
+    if (live.size() < 2) return;
+    double m_total = 0.0;
+    for (auto* p : live) m_total += (double)p->mass;
+    view[19] = m_total;
+    if (m_total <= 0.0) return;
+    V3 com{0, 0, 0}, v_com{0, 0, 0};
+    for (auto* p : live) com = com + pos(*p) * (double)p->mass;
+    for (auto* p : live) v_com = v_com + vel(*p) * (double)p->mass;
+    com = V3{com.x / m_total, com.y / m_total, com.z / m_total};
+    v_com = V3{v_com.x / m_total, v_com.y / m_total, v_com.z / m_total};
+    std::vector<std::pair<double, double>> radii;
+    radii.reserve(live.size());
+    for (auto* p : live) radii.push_back({length(pos(*p) - com), (double)p->mass});
+    std::stable_sort(radii.begin(), radii.end(),
+                     [](const std::pair<double, double>& a, const std::pair<double, double>& b) { return a.first < b.first; });
+    double cum = 0.0, r_remnant = radii.back().first, m_remnant = m_total;
+    for (auto& rm : radii) {
+        cum += rm.second;
+        if (cum >= 0.85 * m_total) {
+            r_remnant = rm.first;
+            m_remnant = cum;
+            break;
+        }
+    }
+    double mu = G64 * m_remnant;
+    double disk0 = 0, disko = 0, esc = 0, rem0 = 0, remo = 0, esc0 = 0;
+    uint64_t nd = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const OrcParticle& p = ps[i];
+        if (!part_live(p)) continue;
+        double m = (double)p.mass;
+        V3 rp = pos(p) - com, rv = vel(p) - v_com;
+        if (!orbit_is_bound(rp, rv, mu)) {
+            esc += m;
+            if (p.prov == 0) esc0 += m;
+            continue;
+        }
+        double pg = 0.0;
+        if (orbit_perigee(rp, rv, mu, &pg) && pg > r_remnant) {
+            if (p.prov == 0)
+                disk0 += m;
+            else
+                disko += m;
+            idx[nd++] = (uint32_t)i;
+        } else if (p.prov == 0) {
+            rem0 += m;
+        } else {
+            remo += m;
+        }
+    }
+    double hs = 0.0;
+    for (auto* p : live) hs += (double)p->h;
+    view[0] = 1.0;
+    view[2] = com.x, view[3] = com.y, view[4] = com.z;
+    view[5] = v_com.x, view[6] = v_com.y, view[7] = v_com.z;
+    view[8] = r_remnant, view[9] = m_remnant, view[10] = mu;
+    view[11] = disk0, view[12] = disko, view[13] = esc;
+    view[14] = hs / (double)live.size();
+    view[15] = rem0, view[16] = remo, view[17] = esc0;
+    view[18] = (double)nd;
+}
+
+// accretion::find_clumps over n particles. pe_mode 0: the reference's self energy (exact below 1024
+// members, the theta = 0.5 octree above, bhtree.rs:21, 45-67, 192-216); 1: the exact pair sum at every
+// size. Writes every clump's record (CR_FIELDS doubles, capacity n records), heaviest first with ties by
+// id (the smallest member index), and labels[n] = each particle's clump id. Returns the clump count.
+uint64_t orc_find_clumps(const double* pos3, const double* vel3, const double* mass, const double* rho, uint64_t n,
+                         double linking_length, double g, double softening, const double* central_pos3,
+                         double central_mass, double central_radius, int pe_mode, double* rec, uint32_t* labels) {
+    const V3* pos = reinterpret_cast<const V3*>(pos3);
+    const V3* vel = reinterpret_cast<const V3*>(vel3);
+    std::vector<size_t> parent(n);
+    for (size_t i = 0; i < n; ++i) parent[i] = i;
+    auto find = [&](size_t i) {  // accretion.rs:178-191
+        size_t r = i;
+        while (parent[r] != r) r = parent[r];
+        size_t c = i;
+        while (parent[c] != r) {
+            size_t nx = parent[c];
+            parent[c] = r;
+            c = nx;
+        }
+        return r;
+    };
+    const double ll2 = linking_length * linking_length;
+    Grid grid;
+    grid.build(pos, n, linking_length);
+    grid.for_each_pair(pos, [&](size_t i, size_t j) {  // accretion.rs:194-201
+        V3 d = pos[i] - pos[j];
+        if (dot(d, d) <= ll2) {
+            size_t a = find(i), b = find(j);
+            if (a != b) parent[a] = b;
+        }
+    });
+    std::unordered_map<size_t, std::vector<size_t>> groups;
+    for (size_t i = 0; i < n; ++i) groups[find(i)].push_back(i);
+    const V3 cpos{central_pos3[0], central_pos3[1], central_pos3[2]};
+    std::vector<std::vector<double>> out;
+    for (auto& kv : groups) {
+        const std::vector<size_t>& members = kv.second;
+        double m = 0.0;
+        for (size_t i : members) m += mass[i];
+        if (m <= 0.0) continue;
+        V3 com{0, 0, 0}, cv{0, 0, 0};
+        for (size_t i : members) com = com + pos[i] * mass[i];
+        com = V3{com.x / m, com.y / m, com.z / m};
+        for (size_t i : members) cv = cv + vel[i] * mass[i];
+        cv = V3{cv.x / m, cv.y / m, cv.z / m};
+        double vol = 0.0;
+        for (size_t i : members) vol += mass[i] / rho[i];
+        double crho = vol > 0.0 ? m / vol : rho[members[0]];
+        double radius = std::cbrt(m / (kFourThirdsPi * crho));
+        double ike = 0.0;
+        for (size_t i : members) {
+            V3 dv = vel[i] - cv;
+            ike += 0.5 * mass[i] * dot(dv, dv);
+        }
+        V3 L{0, 0, 0};
+        for (size_t i : members) L = L + cross3(pos[i] - com, vel[i] - cv) * mass[i];
+        // inertia tensor, glam column layout: I[col][row]
+        double I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (size_t i : members) {
+            V3 r = pos[i] - com;
+            double mi = mass[i];
+            double rr[3] = {r.x, r.y, r.z};
+            double d = mi * dot(r, r);
+            for (int c = 0; c < 3; ++c)
+                for (int w = 0; w < 3; ++w) I[c][w] = I[c][w] + ((c == w ? d : 0.0) - rr[w] * (mi * rr[c]));
+        }
+        V3 c0{I[0][0], I[0][1], I[0][2]}, c1{I[1][0], I[1][1], I[1][2]}, c2{I[2][0], I[2][1], I[2][2]};
+        V3 t0 = cross3(c1, c2), t1 = cross3(c2, c0), t2 = cross3(c0, c1);
+        double det = dot(c2, t2);
+        double e_rot = 0.0;
+        if (members.size() >= 3 && std::isfinite(det) && det > 0.0) {
+            double id = 1.0 / det;
+            // inverse columns (t0[j], t1[j], t2[j]) * id; inv * L = col0 L.x + col1 L.y + col2 L.z
+            V3 w = V3{t0.x * id, t1.x * id, t2.x * id} * L.x + V3{t0.y * id, t1.y * id, t2.y * id} * L.y;
+            w = w + V3{t0.z * id, t1.z * id, t2.z * id} * L.z;
+            e_rot = 0.5 * dot(w, L);
+        }
+        double thermal = std::fmax(ike - e_rot, 0.0);
+        double pe = 0.0;
+        if (members.size() >= 2) {
+            std::vector<V3> mp;
+            std::vector<double> mm;
+            for (size_t i : members) {
+                mp.push_back(pos[i]);
+                mm.push_back(mass[i]);
+            }
+            std::vector<double> phi(members.size(), 0.0);
+            if (pe_mode == 0) {
+                Octree t;
+                t.build(mp.data(), mm.data(), mp.size(), kClumpTheta, softening);
+                t.potentials(mp.data(), mm.data(), phi.data());
+            } else {
+                const double s2 = softening * softening;
+                for (size_t i = 0; i < mp.size(); ++i)
+                    for (size_t j = 0; j < mp.size(); ++j) {
+                        if (i == j) continue;
+                        V3 d = mp[j] - mp[i];
+                        double r2 = dot(d, d) + s2;
+                        phi[i] -= G64 * mm[j] / std::sqrt(r2);
+                    }
+            }
+            double s = 0.0;
+            for (size_t i = 0; i < mp.size(); ++i) s += mm[i] * phi[i];
+            pe = 0.5 * s * (g / G64);
+        }
+        bool bound = ike + pe < 0.0;
+        double rho_c = central_mass / (kFourThirdsPi * (central_radius * central_radius * central_radius));
+        double d_roche = 2.44 * central_radius * std::cbrt(rho_c / crho);
+        bool outside = length(com - cpos) > d_roche;
+        std::vector<double> r(CR_FIELDS, 0.0);
+        r[CR_ID] = (double)members[0];
+        r[CR_N] = (double)members.size();
+        r[CR_MASS] = m;
+        r[CR_COM] = com.x, r[CR_COM + 1] = com.y, r[CR_COM + 2] = com.z;
+        r[CR_VEL] = cv.x, r[CR_VEL + 1] = cv.y, r[CR_VEL + 2] = cv.z;
+        r[CR_RHO] = crho;
+        r[CR_RADIUS] = radius;
+        r[CR_KE] = ike;
+        r[CR_PE] = pe;
+        r[CR_L] = L.x, r[CR_L + 1] = L.y, r[CR_L + 2] = L.z;
+        r[CR_THERMAL] = thermal;
+        r[CR_BOUND] = bound ? 1.0 : 0.0;
+        r[CR_OUTSIDE] = outside ? 1.0 : 0.0;
+        out.push_back(std::move(r));
+        for (size_t i : members) labels[i] = (uint32_t)members[0];
+    }
+    std::sort(out.begin(), out.end(), [](const std::vector<double>& a, const std::vector<double>& b) {
+        if (a[CR_MASS] != b[CR_MASS]) return a[CR_MASS] > b[CR_MASS];
+        return a[CR_ID] < b[CR_ID];
+    });
+    for (size_t k = 0; k < out.size(); ++k) std::memcpy(rec + k * CR_FIELDS, out[k].data(), CR_FIELDS * sizeof(double));
+    return out.size();
+}
+
+}  // extern "C"

@@ -54,7 +54,19 @@ _SIGS = {
     "orc_total_energy": (None, [_U64, _P, _P, _P, _P, _D, _P]),
     "orc_sum_oi": (_D, [_P, _U64]),
     "orc_angular_momentum": (None, [_U64, _P, _P, _P, _P]),
+    "orc_orbit_is_bound": (ctypes.c_bool, [_P, _P, _D]),
+    "orc_orbit_perigee": (ctypes.c_int, [_P, _P, _D, _P]),
+    "orc_disk_view": (None, [_P, _U64, _P, _P]),
+    "orc_find_clumps": (_U64, [_P, _P, _P, _P, _U64, _D, _D, _D, _P, _D, _D, ctypes.c_int, _P, _P]),
 }
+
+# orc_disk_view record fields
+DISK_FIELDS = ("valid", "n_live", "com_x", "com_y", "com_z", "vcom_x", "vcom_y", "vcom_z", "r_remnant",
+               "m_remnant", "mu", "disk_prov0", "disk_other", "escaped", "mean_h", "remnant_prov0",
+               "remnant_other", "escaped_prov0", "n_disk", "m_total")
+# orc_find_clumps record fields
+CLUMP_FIELDS = ("id", "n_members", "mass", "com_x", "com_y", "com_z", "vel_x", "vel_y", "vel_z", "rho", "radius",
+                "internal_ke", "self_pe", "L_x", "L_y", "L_z", "thermal_ke", "bound", "outside_roche", "_r")
 
 _lib = None
 
@@ -324,3 +336,40 @@ def angular_momentum(pos, vel, mass):
     lib().orc_angular_momentum(len(mass), _p(c64(pos).reshape(-1, 3)), _p(c64(vel).reshape(-1, 3)),
                                _p(c64(mass)), _p(out))
     return out
+
+
+# ---- measurement: orbit.rs:91-117, gpu_sph.rs:689-785, accretion.rs:158-301 ----
+def orbit_is_bound(p, v, mu):
+    return bool(lib().orc_orbit_is_bound(_p(c64(p)), _p(c64(v)), float(mu)))
+
+
+def orbit_perigee(p, v, mu):
+    """orbit::perigee: the periapsis distance, or None."""
+    q = np.zeros(1)
+    ok = lib().orc_orbit_perigee(_p(c64(p)), _p(c64(v)), float(mu), _p(q))
+    return float(q[0]) if ok else None
+
+
+def disk_view(particles):
+    """DiskView::of over a PARTICLE_DTYPE record array: (dict of DISK_FIELDS, disk member indices)."""
+    a = np.ascontiguousarray(particles)
+    assert a.dtype.itemsize == 48
+    view = np.zeros(len(DISK_FIELDS))
+    idx = np.zeros(max(len(a), 1), np.uint32)
+    lib().orc_disk_view(_p(a), len(a), _p(view), _p(idx))
+    return dict(zip(DISK_FIELDS, view.tolist())), idx[:int(view[18])].copy()
+
+
+def find_clumps(pos, vel, mass, rho, linking_length, g, softening, central_pos, central_mass, central_radius,
+                exact_pe=False):
+    """accretion::find_clumps: (records (k, 20) heaviest first, labels). exact_pe uses the exact pair sum
+    at every clump size instead of the reference's octree above 1024 members."""
+    p = c64(pos).reshape(-1, 3)
+    n = p.shape[0]
+    rec = np.zeros((max(n, 1), len(CLUMP_FIELDS)))
+    labels = np.zeros(max(n, 1), np.uint32)
+    k = lib().orc_find_clumps(_p(p), _p(c64(vel).reshape(-1, 3)), _p(c64(mass)), _p(c64(rho)), n,
+                              float(linking_length), float(g), float(softening), _p(c64(central_pos)),
+                              float(central_mass), float(central_radius), 1 if exact_pe else 0, _p(rec),
+                              _p(labels))
+    return rec[:k].copy(), labels[:n].copy()
