@@ -101,7 +101,49 @@ def cpu_baseline(sc):
                       f"{dt:.1f} s on 1 thread of {lscpu_model()}"}
 
 
-def run_gpu(sc, args, precision, steps, warmup, reuse):
+def time_disk_view(s, reps=5):
+    """Wall time of one blocking on-device disk measurement (DiskView::of of every instance with its
+    clumps) plus the clump query of the first instance, averaged over `reps`."""
+    s.disk_view()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        views = s.disk_view()
+        clumps = s.disk_clumps(0, 2)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    return ms, views, clumps
+
+
+def measure_cpu(rec):
+    """The oracle's DiskView::of and DiskView::clumps (reference self energy) on the same read-back state,
+    1 thread."""
+    from oracle import oracle
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    v, idx = oracle.disk_view(rec)
+    if len(idx) >= 2:
+        p = rec["pos"][idx].astype(np.float64)
+        oracle.find_clumps(p, rec["vel"][idx].astype(np.float64), rec["mass"][idx].astype(np.float64),
+                           np.maximum(rec["rho"][idx], np.float32(1.0)).astype(np.float64), 2.0 * v["mean_h"],
+                           6.674e-11, 1.0e4, [v["com_x"], v["com_y"], v["com_z"]], v["m_remnant"], v["r_remnant"])
+    return (time.perf_counter() - t0) * 1e3
+
+
+def run_debris(n):
+    """The disk measurement on a synthetic post-impact debris field of n particles (a percolating diffuse
+    disk plus cold clumps), so friends-of-friends, clump statistics and the self-energy sums run at scale."""
+    from botheadstudios_amd import isph, materials, scene
+    rec = scene.debris_field(n)
+    with isph.GpuSph(n) as s:
+        s.upload(rec, materials.eos_table(("granite",)), 1.0e4)
+        ms, views, clumps = time_disk_view(s)
+    d = views[0]
+    return {"workload": f"DiskView and clumps of a synthetic post-impact debris field, N={n} (central body 80 % "
+                        f"of the particles, orbiting disk with cold 50-particle clumps, plunging and escaping "
+                        f"debris)", "gpu_ms": ms, "n_disk": d.n_disk, "n_clumps": d.n_clumps,
+            "n_bound": d.n_bound, "largest_clump_members": max((c.n_members for c in clumps), default=0)}, rec
+
+
+def run_gpu(sc, args, precision, steps, warmup, reuse, measure=False):
     from botheadstudios_amd import isph
     rec = sc.particles
     n = len(rec)
@@ -130,8 +172,15 @@ def run_gpu(sc, args, precision, steps, warmup, reuse):
     s.set_timing(False)
     t_sim1, dt_last = s.sim_time()
     d1 = None if args.no_energy else s.diagnostics(with_pe=True)
-    s.close()
     out = {"elapsed": elapsed, "phases": phases, "stats": stats, "sim_time": t_sim1 - t_sim0, "dt_last": dt_last}
+    if measure:
+        ms, views, clumps = time_disk_view(s)
+        d = views[0]
+        out["measure"] = {"workload": f"DiskView of the N={n} state after the timed steps, with its clumps",
+                          "gpu_ms": ms, "n_live": d.n_live, "n_disk": d.n_disk, "n_clumps": d.n_clumps,
+                          "disk_mass": d.disk_mass(), "r_remnant": d.r_remnant}
+        out["measure_rec"] = s.readback_sync()
+    s.close()
     if d0 is not None:
         e0 = d0["ke"] + d0["ie"] + d0["pe"]
         e1 = d1["ke"] + d1["ie"] + d1["pe"]
@@ -243,6 +292,7 @@ def run_batch(args):
         s.step_cfl(args.cfl, args.batch_steps)
         s.synchronize()
         ens = time.perf_counter() - t0
+        measure_ms, views, _ = time_disk_view(s)
     seq = 0.0
     for sc in scs:
         n = len(sc.particles)
@@ -261,6 +311,7 @@ def run_batch(args):
             "value": total * args.batch_steps / ens, "unit": "particle-steps/s", "ms_per_step": ens * 1e3 / args.batch_steps,
             "sequential_plain_value": total * args.batch_steps / seq,
             "speedup_vs_sequential_plain": seq / ens,
+            "disk_view_all_instances_ms": measure_ms, "disk_members_total": int(sum(v.n_disk for v in views)),
             "tuning": {"leaf_size": tun.leaf_size, "group_extent": tun.group_extent,
                        "grav_particles_per_lane": tun.grav_particles_per_lane, "sph_expand": tun.sph_expand,
                        "from_cache": from_cache}}
@@ -276,7 +327,7 @@ def main():
 
     sc = scene.two_body_1m() if args.n == 1048576 else scene.two_body(args.n)
     n = len(sc.particles)
-    r = run_gpu(sc, args, args.precision, args.steps, args.warmup, args.reuse)
+    r = run_gpu(sc, args, args.precision, args.steps, args.warmup, args.reuse, measure=True)
     st = r["stats"]
     launches = max(st["gravity_launches"], 1.0)
     flops = st["gravity_flops"]
@@ -317,6 +368,9 @@ def main():
     if "dE_rel" in r:
         line["dE_rel"] = r["dE_rel"]
         line["dL_rel"] = r["dL_rel"]
+    line["measure"] = {"impact_state": r["measure"]}
+    debris, debris_rec = run_debris(n)
+    line["measure"]["debris_field"] = debris
     for v in [x for x in args.variants.split(",") if x]:
         if v not in ("mixed", "reuse"):
             sys.exit(f"unknown variant {v!r}")
@@ -334,6 +388,8 @@ def main():
         line["precision_4m"] = run_4m(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc)
+        line["measure"]["impact_state"]["cpu_oracle_ms"] = measure_cpu(r["measure_rec"])
+        line["measure"]["debris_field"]["cpu_oracle_ms"] = measure_cpu(debris_rec)
     print(json.dumps(line), flush=True)
 
 

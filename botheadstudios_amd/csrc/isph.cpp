@@ -1021,6 +1021,21 @@ int isph_readback_sync(isph_t* h, isph_particle* out, size_t cap) {
     });
 }
 
+int isph_particle_buffer(isph_t* h, void** dev_ptr, uint32_t* count) {
+    return guarded(h, [&]() -> int {
+        if (!dev_ptr) return fail(h, ISPH_EINVAL, "NULL output");
+        HIP_OK(hipSetDevice(h->device));
+        if (h->d.n > 0) launch_pack(h->d, h->stream);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipGetLastError());
+        *dev_ptr = h->d.aos;
+        if (count) *count = h->d.n;
+        return ISPH_OK;
+    });
+}
+
+void* isph_stream(isph_t* h) { return h ? (void*)h->stream : nullptr; }
+
 int isph_synchronize(isph_t* h) {
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));

@@ -53,6 +53,8 @@ _SIGS = {
     "isph_take_readback": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "isph_readback_sync": (ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     "isph_synchronize": (ctypes.c_int, [_P]),
+    "isph_particle_buffer": (ctypes.c_int, [_P, _P, _P]),
+    "isph_stream": (_P, [_P]),
     "isph_force_eval": (ctypes.c_int, [_P, _P, _P]),
     "isph_gravity_accelerations": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, ctypes.c_double, _P, ctypes.c_int]),
     "isph_diagnostics": (ctypes.c_int, [_P, ctypes.c_int, _P]),
@@ -283,6 +285,17 @@ class GpuSph:
         out = np.zeros(self.capacity, dtype=PARTICLE_DTYPE)
         rc = self._check(self._lib.isph_readback_sync(self._h, _ptr(out), out.shape[0]))
         return out[:rc]
+
+    def particle_buffer(self):
+        """gpu_sph.rs:1452-1456: (device address, count) of the packed 48-byte particle records."""
+        ptr = ctypes.c_void_p(0)
+        cnt = ctypes.c_uint32(0)
+        self._check(self._lib.isph_particle_buffer(self._h, ctypes.byref(ptr), ctypes.byref(cnt)))
+        return ptr.value, cnt.value
+
+    def stream(self):
+        """The handle's hipStream_t as an integer."""
+        return self._lib.isph_stream(self._h)
 
     def synchronize(self):
         self._check(self._lib.isph_synchronize(self._h))

@@ -252,3 +252,57 @@ class ImpactFamily:
         dm[nt:] = self.m_t / self.n_impactor * M_U
         out.append((np.zeros((n, 3)), dvel, dm))
         return out
+
+
+def debris_field(n, seed=0xD15C, clump_size=50):
+    """A synthetic post-impact field for the measurement workloads (SI, granite): a central body holding
+    95 % of the mass in 80 % of the particles, and the rest as an orbiting debris disk (a third of it in
+    cold clumps of `clump_size` particles on circular orbits, the rest diffuse and near-circular), plus
+    plunging and escaping debris. Returns the particle records."""
+    rng = np.random.default_rng(seed)
+    R, M = L_U, 6.0e24
+    mu = G_SI * M
+    n_core = int(0.8 * n)
+    n_deb = n - n_core
+    m_core = 0.95 * M / n_core
+    m_deb = 0.05 * M / n_deb
+    pos = np.empty((n, 3))
+    vel = np.empty((n, 3))
+    d = rng.normal(size=(n_core, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pos[:n_core] = d * (R * np.cbrt(rng.random(n_core)))[:, None]
+    vel[:n_core] = rng.normal(scale=30.0, size=(n_core, 3))
+    n_cl = (n_deb // 3) // clump_size
+    k = n_core
+    golden = np.pi * (3.0 - np.sqrt(5.0))
+    i = np.arange(clump_size, dtype=np.float64)
+    rr = 0.02 * R * np.cbrt((i + 0.5) / clump_size)
+    y = 1.0 - 2.0 * (i + 0.5) / clump_size
+    shell = np.stack([np.cos(golden * i) * np.sqrt(1 - y * y) * rr, y * rr, np.sin(golden * i) * np.sqrt(1 - y * y) * rr], 1)
+    r_c = rng.uniform(2.0 * R, 8.0 * R, n_cl)
+    ph = rng.uniform(0, 2 * np.pi, n_cl)
+    for c in range(n_cl):
+        ctr = np.array([r_c[c] * np.cos(ph[c]), r_c[c] * np.sin(ph[c]), 0.0])
+        vc = np.sqrt(mu / r_c[c]) * np.array([-np.sin(ph[c]), np.cos(ph[c]), 0.0])
+        pos[k:k + clump_size] = ctr + shell
+        vel[k:k + clump_size] = vc
+        k += clump_size
+    rest = n - k
+    kind = rng.random(rest)  # 0.8 diffuse disk, 0.1 plunging, 0.1 escaping
+    r = rng.uniform(1.5 * R, 8.0 * R, rest)
+    ph = rng.uniform(0, 2 * np.pi, rest)
+    pos[k:] = np.stack([r * np.cos(ph), r * np.sin(ph), rng.normal(scale=0.05, size=rest) * r], 1)
+    speed = np.sqrt(mu / r) * np.where(kind < 0.8, rng.uniform(0.95, 1.1, rest), np.where(kind < 0.9, 0.2, 2.0))
+    vel[k:] = np.stack([-speed * np.sin(ph), speed * np.cos(ph), np.zeros(rest)], 1)
+    mass = np.concatenate([np.full(n_core, m_core), np.full(n - n_core, m_deb)])
+    h = np.concatenate([np.full(n_core, 2.0 * np.cbrt(m_core / 3000.0)), np.full(n - n_core, 2.0 * np.cbrt(m_deb / 2700.0))])
+    rec = np.zeros(n, dtype=PARTICLE_DTYPE)
+    perm = rng.permutation(n)
+    rec["pos"] = pos[perm]
+    rec["vel"] = vel[perm]
+    rec["mass"] = mass[perm]
+    rec["h"] = h[perm]
+    rec["u"] = 1.0e5
+    rec["rho"] = 2700.0
+    rec["prov"] = (rng.random(n) < 0.3).astype(np.uint32)
+    return rec

@@ -184,6 +184,15 @@ int isph_take_readback(isph_t* h, isph_particle* out, size_t cap);
 /* Blocking read-back for native callers (the offline driver's read_particles, main.rs:332-343). */
 int isph_readback_sync(isph_t* h, isph_particle* out, size_t cap);
 int isph_synchronize(isph_t* h);
+/* GpuSph::particle_buffer (gpu_sph.rs:1452-1456): the device array of 48-byte particle records that the
+ * renderer draws (shaders/sph_render.wgsl) without a copy. The state lives in per-field arrays on the
+ * device, so this packs the current state into the handle's record array on its stream and waits; *dev_ptr
+ * receives that array's device address and *count the particle count. The array belongs to the handle and
+ * holds this snapshot until the next isph_particle_buffer or read-back call. */
+int isph_particle_buffer(isph_t* h, void** dev_ptr, uint32_t* count);
+/* The handle's HIP stream (a hipStream_t), for interop callers that order their own device work after the
+ * handle's. */
+void* isph_stream(isph_t* h);
 
 /* One force evaluation at the current state (the reference's force_eval gpu_sph.rs:1468-1474:
  * density + forces). Writes, in upload order: acc4 = (ax, ay, az, du/dt) per particle, rho. Either

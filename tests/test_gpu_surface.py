@@ -163,3 +163,24 @@ def test_merge_gates_materials_and_budget():
     """main.rs:449-482: iron + basalt pairs never merge; with budget 0 nothing merges."""
     assert _totals(_merged(_merge_lattice(mixed=True), budget=1))[3] == 128
     assert _totals(_merged(_merge_lattice(), budget=0))[3] == 128
+
+
+def test_particle_buffer_holds_the_packed_records():
+    """gpu_sph.rs:1452-1456: the renderer's 48-byte records on the device equal the read-back."""
+    import ctypes
+    sc = scene.two_body(4096)
+    with isph.GpuSph(len(sc.particles)) as s:
+        # the HIP runtime the library already runs on (dlopen by soname returns the loaded copy)
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        hip.hipMemcpy.restype = ctypes.c_int
+        s.upload(sc.particles, sc.eos, sc.softening)
+        s.set_eos_f64(sc.eos64)
+        s.step_cfl(0.3, 3)
+        ptr, n = s.particle_buffer()
+        assert n == len(sc.particles) and ptr and s.stream() is not None
+        host = np.zeros(n, dtype=scene.PARTICLE_DTYPE)
+        assert hip.hipMemcpy(host.ctypes.data_as(ctypes.c_void_p), ptr, n * 48, 2) == 0  # device to host
+        rb = s.readback_sync()
+    assert host.tobytes() == rb.tobytes()
+    assert not np.array_equal(rb["pos"], sc.particles["pos"])  # the steps moved the state
