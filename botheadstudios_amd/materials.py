@@ -23,6 +23,16 @@ CATALOGUE = {
     "ice": ICE,
 }
 
+# data/materials.json "thermal.specific_heat" (J/kg/K): u = c T (hydrostatic.rs:236-242).
+SPECIFIC_HEAT = {
+    "granite": 790.0,
+    "basalt": 840.0,
+    "peridotite": 1000.0,
+    "iron": 449.0,
+    "water": 4186.0,
+    "ice": 2090.0,
+}
+
 # Material indices used by the impact scenes (index into the EOS table passed to upload).
 MAT_GRANITE = 0
 MAT_IRON = 1
