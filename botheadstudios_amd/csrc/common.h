@@ -186,6 +186,11 @@ __device__ __forceinline__ void till_p_c2(const Till& t, double rho_in, double u
 
 // ---- cubic spline with support h (sph_step.wgsl:73-86, atmosphere.rs:52-75) ---------------------------
 // W(r,h) and dW/dr from q = r/h and 1/h.
+// Hardware square root and reciprocal (1 ulp) for the plain form's pair terms: the correctly rounded
+// library forms expand into refinement sequences several times longer.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ float spline_w(float q, float inv_h) {
     float sig = (8.0f / kPi) * inv_h * inv_h * inv_h;
     if (q < 0.5f) return sig * (1.0f - 6.0f * q * q + 6.0f * q * q * q);

@@ -45,6 +45,38 @@ __device__ __forceinline__ float eos_point(uint32_t t, double rho, float h, floa
     return h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
 }
 
+// The pair test of one lane against the candidates in LDS (x, y, z, h per entry): bit jj is set when
+// r < (h_i + h_j) / 2 (sph_step.wgsl:201,210 / :262-272); `not_coincident` adds the force loop's r > 1e-9.
+// `margin` widens the radius test (the mixed form's f32 pre-filter keeps a superset of its exact f64 test).
+// The lane's own particle is removed afterwards with `self_slot`.
+__device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 me, float hi, bool not_coincident,
+                                              float margin = 1.0f) {
+    uint64_t hit = 0ull;
+    for (int jj = 0; jj < cnt; ++jj) {
+        const float4 pj = cP[jj];
+        const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float hij = 0.5f * (hi + pj.w);
+        bool ok = r2 < margin * (hij * hij);
+        if (not_coincident) ok = ok && r2 > 1.0e-18f;
+        hit |= ok ? (1ull << jj) : 0ull;
+    }
+    return hit;
+}
+
+// While loading candidate `cand[lane]`, record where a particle of the lane's own group sits in this batch
+// (own[d] = slot for group member d). A lane then finds its own particle with one LDS read and a check,
+// instead of comparing indices at every candidate. Stale entries from earlier batches fail the check.
+__device__ __forceinline__ void note_owner(uint32_t* own, uint32_t j, uint32_t g0, uint32_t gn, int lane) {
+    const uint32_t d = j - g0;
+    if (d < gn) own[d] = (uint32_t)lane;
+}
+__device__ __forceinline__ uint64_t self_bit(const uint32_t* own, const uint32_t* cand, int cnt, int lane,
+                                             uint32_t t) {
+    const uint32_t ms = own[lane];
+    return (ms < (uint32_t)cnt && cand[ms] == t) ? (1ull << ms) : 0ull;
+}
+
 template <bool F64>
 __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                 uint32_t n, uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
@@ -58,9 +90,11 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                                                 const uint32_t* __restrict__ iroot) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
-    __shared__ float4 cP[kWave];
-    __shared__ float cH[kWave];
+    __shared__ float4 cP[kWave];  // x, y, z, h
+    __shared__ float cM[kWave];   // mass
+    __shared__ uint32_t own[kWave];
     const int lane = threadIdx.x;
+    own[lane] = kWave;
     unsigned long long tested_all = 0;
     WalkQueue queue(scal + kQueueDensity, scal[kGroupsSph]);
     uint32_t gi;
@@ -86,30 +120,42 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                    counters ? counters + 8 : nullptr, [&](int cnt) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
-            cP[lane] = spos[j];
-            cH[lane] = sh[j];
+            const float4 p = spos[j];
+            cP[lane] = make_float4(p.x, p.y, p.z, sh[j]);
+            cM[lane] = p.w;
+            note_owner(own, j, grp.x, grp.y, lane);
         }
         __syncthreads();
+        // 1) The radius test against every candidate (LDS broadcast reads): one bit per true pair, the
+        //    lane's own particle excluded (sph_step.wgsl:210).
+        const uint64_t hit = pair_mask(cP, cnt, me, hi, false) & ~self_bit(own, cand, cnt, lane, t);
+        // 2) The kernel sum over this lane's hits only, in candidate order. Lanes walk their own hit lists
+        //    in step, so the wave runs as many iterations as its busiest lane instead of one per candidate
+        //    with most lanes idle.
         float acc = 0.f;
-        for (int jj = 0; jj < cnt; ++jj) {
-            float4 pj = cP[jj];
-            float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float hij = 0.5f * (hi + cH[jj]);
-            if (r2 < hij * hij && cand[jj] != t) {  // sph_step.wgsl:201,210
+        uint64_t rest = hit;
+        while (__ballot(rest != 0ull)) {
+            if (rest) {
+                const int jj = __builtin_ctzll(rest);
+                rest &= rest - 1ull;
+                const float4 pj = cP[jj];
+                const float mj = cM[jj];
                 if (F64) {
                     double ddx = (double)me.x - (double)pj.x, ddy = (double)me.y - (double)pj.y,
                            ddz = (double)me.z - (double)pj.z;
-                    double inv_h = 1.0 / (0.5 * ((double)hi + (double)cH[jj]));
+                    double inv_h = 1.0 / (0.5 * ((double)hi + (double)pj.w));
                     double q = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) * inv_h;
                     double sig = (8.0 / 3.14159265358979323846) * (inv_h * inv_h * inv_h);
                     double w = q < 0.5 ? sig * (6.0 * (q * q * q - q * q) + 1.0)
                                        : (q < 1.0 ? sig * 2.0 * (1.0 - q) * (1.0 - q) * (1.0 - q) : 0.0);
-                    rho64 += (double)pj.w * w;
+                    rho64 += (double)mj * w;
                 } else {
-                    float r = sqrtf(r2);
-                    float inv_h = 1.0f / hij;
-                    acc += pj.w * spline_w(r * inv_h, inv_h);
+                    float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    float hij = 0.5f * (hi + pj.w);
+                    float r = hw_sqrt(r2);
+                    float inv_h = hw_rcp(hij);
+                    acc += mj * spline_w(r * inv_h, inv_h);
                 }
             }
         }
@@ -187,9 +233,9 @@ __global__ __launch_bounds__(kEosThreads) void k_eos(const float* __restrict__ s
     }
 }
 
-// Five resident waves per SIMD: the f64 form needs 96 VGPRs at most then, without spilling.
+// Resident waves per SIMD: five for the plain form, four for the f64 form (up to 128 VGPRs, no spilling).
 template <bool F64>
-__global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
+__global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
                                               const float4* __restrict__ svel, const float* __restrict__ srho,
                                               const float* __restrict__ spr, const float* __restrict__ scs, uint32_t n,
                                               uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
@@ -204,13 +250,15 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
                                               const uint32_t* __restrict__ iroot, Boundary bnd) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
-    __shared__ float4 cP[kWave];  // x, y, z, mass
+    __shared__ float4 cP[kWave];  // x, y, z, h
     __shared__ float4 cV[kWave];  // vx, vy, vz, rho
-    __shared__ float4 cS[kWave];  // h, P/rho^2, c
+    __shared__ float4 cS[kWave];  // mass, P/rho^2, c
     __shared__ double cR64[F64 ? kWave : 1];
     __shared__ double cPr64[F64 ? kWave : 1];
     __shared__ double cC64[F64 ? kWave : 1];
+    __shared__ uint32_t own[kWave];
     const int lane = threadIdx.x;
+    own[lane] = kWave;
     unsigned long long tested_all = 0;
     WalkQueue queue(scal + kQueueForce, scal[kGroupsSph]);
     uint32_t gi;
@@ -239,30 +287,35 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
                    counters ? counters + 11 : nullptr, [&](int cnt) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
-            cP[lane] = spos[j];
+            const float4 p = spos[j];
+            cP[lane] = make_float4(p.x, p.y, p.z, sh[j]);
             float4 v = svel[j];
             cV[lane] = make_float4(v.x, v.y, v.z, srho[j]);
-            cS[lane] = make_float4(sh[j], spr[j], scs[j], 0.f);
+            cS[lane] = make_float4(p.w, spr[j], scs[j], 0.f);
             if (F64) {
                 cR64[lane] = srho64[j];
                 cPr64[lane] = spr64[j];
                 cC64[lane] = scs64[j];
             }
+            note_owner(own, j, grp.x, grp.y, lane);
         }
         __syncthreads();
-        for (int jj = 0; jj < cnt; ++jj) {
+        // 1) The radius test against every candidate, one bit per pair (plain: the exact test of
+        //    sph_step.wgsl:262-272; mixed: an f32 pre-filter with a 1e-4 margin, a superset of the exact f64
+        //    test below). 2) The pair terms over this lane's hits only, in candidate order.
+        uint64_t rest = (F64 ? pair_mask(cP, cnt, me, hi, false, 1.0001f) : pair_mask(cP, cnt, me, hi, true)) &
+                        ~self_bit(own, cand, cnt, lane, t);
+        while (__ballot(rest != 0ull)) {
+            if (!rest) continue;
+            const int jj = __builtin_ctzll(rest);
+            rest &= rest - 1ull;
+            const float4 pj = cP[jj];
+            const float4 sj = cS[jj];
             if (F64) {
-                float4 pj = cP[jj];
-                float4 sj = cS[jj];
-                // f32 pre-filter with a 1e-4 margin (a superset of the exact f64 radius test), so the f64
-                // arithmetic runs only on the ~1 in 4 candidates that are true neighbours.
-                float fx = me.x - pj.x, fy = me.y - pj.y, fz = me.z - pj.z;
-                float fh = 0.5f * (hi + sj.x);
-                if (!(fx * fx + fy * fy + fz * fz < 1.0001f * fh * fh) || cand[jj] == t) continue;
                 double dx = (double)me.x - (double)pj.x, dy = (double)me.y - (double)pj.y,
                        dz = (double)me.z - (double)pj.z;
                 double r2 = dx * dx + dy * dy + dz * dz;
-                double hij = 0.5 * ((double)hi + (double)sj.x);
+                double hij = 0.5 * ((double)hi + (double)pj.w);
                 if (r2 < hij * hij && r2 > 1.0e-18) {
                     float4 vj = cV[jj];
                     double inv_r = 1.0 / sqrt(r2);
@@ -283,40 +336,36 @@ __global__ __launch_bounds__(64, 5) void k_force(const float4* __restrict__ spos
                     double dw = q < 0.5 ? sig * (18.0 * q * q - 12.0 * q) : sig * -6.0 * (1.0 - q) * (1.0 - q);
                     double gs = dw * inv_r;
                     double gx = dx * gs, gy = dy * gs, gz = dz * gs;
-                    double mj = (double)pj.w;
+                    double mj = (double)sj.x;
                     ax64 -= gx * (coeff * mj);
                     ay64 -= gy * (coeff * mj);
                     az64 -= gz * (coeff * mj);
                     de64 += 0.5 * mj * coeff * (dvx * gx + dvy * gy + dvz * gz);
                 }
-                continue;
-            }
-            float4 pj = cP[jj];
-            float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;  // dpos = pos_i - pos_j
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float4 sj = cS[jj];
-            float hij = 0.5f * (hi + sj.x);
-            if (r2 < hij * hij && r2 > 1.0e-18f && cand[jj] != t) {  // r < hij && r > 1e-9 (:272)
+            } else {
+                float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;  // dpos = pos_i - pos_j
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float hij = 0.5f * (hi + pj.w);
                 float4 vj = cV[jj];
-                float r = sqrtf(r2);
-                float inv_h = 1.0f / hij;
+                float r = hw_sqrt(r2);
+                float inv_h = hw_rcp(hij);
                 float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
                 float vr = dvx * dx + dvy * dy + dvz * dz;
                 float pij = 0.f;
                 if (vr < 0.f) {  // Monaghan viscosity (:279-284)
-                    float mu = hij * vr / (r2 + 0.01f * hij * hij);
+                    float mu = hij * vr * hw_rcp(r2 + 0.01f * hij * hij);
                     float cbar = 0.5f * (ci + sj.z);
                     float rbar = 0.5f * (rhoi + vj.w);
-                    pij = (-av_alpha * cbar * mu + av_beta * mu * mu) / rbar;
+                    pij = (-av_alpha * cbar * mu + av_beta * mu * mu) * hw_rcp(rbar);
                 }
                 float coeff = pri + sj.y + pij;
-                float gs = spline_dw(r * inv_h, inv_h) / r;
+                float gs = spline_dw(r * inv_h, inv_h) * hw_rcp(r);
                 float gx = dx * gs, gy = dy * gs, gz = dz * gs;
-                float f = -coeff * pj.w;
+                float f = -coeff * sj.x;
                 ax += gx * f;
                 ay += gy * f;
                 az += gz * f;
-                de += 0.5f * pj.w * coeff * (dvx * gx + dvy * gy + dvz * gz);
+                de += 0.5f * sj.x * coeff * (dvx * gx + dvy * gy + dvz * gz);
             }
         }
         tested += cnt;
