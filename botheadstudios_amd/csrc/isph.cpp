@@ -109,7 +109,8 @@ void free_all(isph_ctx* c) {
     Dev& d = c->d;
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
-                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.ggroups, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.ggroups, d.chead,
+                    d.cnext, d.ccnt, d.cpool, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -169,6 +170,15 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.cross, cap);
     dalloc(&d.groups, cap);
     dalloc(&d.ggroups, cap);
+    {
+        // expected ~0.25 chunks per particle, plus up to one part-used block per resident walk wave
+        const size_t blocks = ((size_t)cap / 2 + kChunksPerBlock - 1) / kChunksPerBlock + kWalkWaves;
+        dalloc(&d.chead, cap);
+        dalloc(&d.cnext, blocks * kChunksPerBlock);
+        dalloc(&d.ccnt, blocks * kChunksPerBlock);
+        dalloc(&d.cpool, blocks * kChunksPerBlock * kWave);
+        d.cblocks = (uint32_t)blocks;
+    }
     dalloc(&d.gflag, cap);
     dalloc(&d.gscan, cap / kWave + 16);
     HIP_OK(hipMemset(d.gflag, 0, (size_t)cap * sizeof(uint32_t)));

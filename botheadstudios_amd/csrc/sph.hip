@@ -45,6 +45,58 @@ __device__ __forceinline__ float eos_point(uint32_t t, double rho, float h, floa
     return h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
 }
 
+// Candidate-list cache: the density walk stores each group's candidate batches, the force pass replays them.
+struct CandCache {
+    uint32_t* head;
+    uint32_t* next;
+    uint32_t* count;
+    uint32_t* pool;
+    uint32_t blocks;
+};
+
+// Wave-uniform writer state: the current block of chunks (taken with one atomic per kChunksPerBlock
+// chunks), and the group's first and last chunk. A group whose chunks run out is marked for the walk.
+struct CacheWriter {
+    uint32_t blk = kNoChunk, used = kChunksPerBlock, first = kNoChunk, last = kNoChunk;
+    bool ok = true;
+    __device__ __forceinline__ void begin() {
+        first = last = kNoChunk;
+        ok = true;
+    }
+    __device__ __forceinline__ void put(const CandCache& cc, uint32_t* scal, const uint32_t* cand, int cnt,
+                                        int lane) {
+        if (!ok) return;
+        if (used == kChunksPerBlock) {
+            uint32_t b = 0;
+            if (lane == 0) b = atomicAdd(&scal[kPoolHead], 1u);
+            b = __builtin_amdgcn_readfirstlane(b);
+            if (b >= cc.blocks) {
+                ok = false;
+                return;
+            }
+            blk = b;
+            used = 0;
+        }
+        const uint32_t id = blk * kChunksPerBlock + used++;
+        if (lane < cnt) cc.pool[(size_t)id * kWave + lane] = cand[lane];
+        if (lane == 0) {
+            cc.count[id] = (uint32_t)cnt;
+            if (last == kNoChunk) first = id;
+            else cc.next[last] = id;
+        }
+        last = id;
+    }
+    __device__ __forceinline__ void end(const CandCache& cc, uint32_t gi, int lane) {
+        if (lane != 0) return;
+        if (!ok) {
+            cc.head[gi] = kChunkOverflow;
+            return;
+        }
+        if (last != kNoChunk) cc.next[last] = kNoChunk;
+        cc.head[gi] = first;
+    }
+};
+
 // The pair test of one lane against the candidates in LDS (x, y, z, h per entry): bit jj is set when
 // r < (h_i + h_j) / 2 (sph_step.wgsl:201,210 / :262-272); `not_coincident` adds the force loop's r > 1e-9.
 // `margin` widens the radius test (the mixed form's f32 pre-filter keeps a superset of its exact f64 test).
@@ -87,7 +139,7 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
                                                 unsigned long long* counters, const uint2* __restrict__ groups,
                                                 const uint2* __restrict__ nrange, uint32_t expand,
                                                 const uint32_t* __restrict__ sinst,
-                                                const uint32_t* __restrict__ iroot) {
+                                                const uint32_t* __restrict__ iroot, CandCache cc) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, h
@@ -96,6 +148,7 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
     const int lane = threadIdx.x;
     own[lane] = kWave;
     unsigned long long tested_all = 0;
+    CacheWriter cw;
     WalkQueue queue(scal + kQueueDensity, scal[kGroupsSph]);
     uint32_t gi;
     while (queue.next(gi)) {
@@ -116,8 +169,10 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
     float rho = 0.f;
     double rho64 = 0.0;
     unsigned long long tested = 0;
+    cw.begin();
     neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                    counters ? counters + 8 : nullptr, [&](int cnt) {
+        cw.put(cc, scal, cand, cnt, lane);
         if (lane < cnt) {
             uint32_t j = cand[lane];
             const float4 p = spos[j];
@@ -162,6 +217,7 @@ __global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos,
         rho += acc;
         tested += cnt;
     }, sinst ? iroot[sinst[grp.x]] : 0u);
+    cw.end(cc, gi, lane);
     const bool live = valid && me.w > 0.f;  // retired (mass 0) particles keep their density (:185-189)
     double rho_i;
     if (live) {
@@ -247,7 +303,9 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                                               const uint2* __restrict__ groups, uint32_t* scal,
                                               const uint2* __restrict__ nrange, uint32_t expand,
                                               double* __restrict__ acc64, const uint32_t* __restrict__ sinst,
-                                              const uint32_t* __restrict__ iroot, Boundary bnd) {
+                                              const uint32_t* __restrict__ iroot, Boundary bnd,
+                                              const uint32_t* __restrict__ chead, const uint32_t* __restrict__ cnext,
+                                              const uint32_t* __restrict__ ccnt, const uint32_t* __restrict__ cpool) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, h
@@ -283,8 +341,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     double ax64 = 0.0, ay64 = 0.0, az64 = 0.0, de64 = 0.0;
     const double rhoi64 = F64 ? srho64[tc] : 0.0, pri64 = F64 ? spr64[tc] : 0.0, ci64 = F64 ? scs64[tc] : 0.0;
     unsigned long long tested = 0;
-    neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
-                   counters ? counters + 11 : nullptr, [&](int cnt) {
+    auto flush = [&](int cnt) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
             const float4 p = spos[j];
@@ -369,7 +426,23 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             }
         }
         tested += cnt;
-    }, sinst ? iroot[sinst[grp.x]] : 0u);
+    };
+    // The density walk of this evaluation stored the group's candidates batch by batch: replay them in the
+    // same order (so the sums are the walk's, bit for bit), or walk when the list did not fit.
+    const uint32_t head = chead ? chead[gi] : kChunkOverflow;
+    if (head == kChunkOverflow) {
+        neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
+                       counters ? counters + 11 : nullptr, flush, sinst ? iroot[sinst[grp.x]] : 0u);
+    } else {
+        for (uint32_t id = head; id != kNoChunk;) {
+            const uint32_t cnt = ccnt[id], nx = cnext[id];
+            if ((uint32_t)lane < cnt) cand[lane] = cpool[(size_t)id * kWave + lane];
+            __syncthreads();
+            flush((int)cnt);
+            __syncthreads();
+            id = nx;
+        }
+    }
     if (F64) {
         ax = (float)ax64;
         ay = (float)ay64;
@@ -685,6 +758,8 @@ __global__ __launch_bounds__(256) void k_diag_pe(uint32_t n, const float4* posh,
 
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+static CandCache cand_cache(const Dev& d) { return CandCache{d.chead, d.cnext, d.ccnt, d.cpool, d.cblocks}; }
+
 static Boundary boundary(const Dev& d, const StepParams& p) {
     return Boundary{p.bulk_cr, p.bulk_vm, d.ext, p.n_ext, p.soft2};
 }
@@ -694,11 +769,11 @@ void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     if (p.mixed)
         hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR, d.nB,
                            d.nC, d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
-                           (uint32_t)p.sph_expand, d.sinst(), d.iroot);
+                           (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d));
     else
         hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR,
                            d.nB, d.nC, d.vals, d.srho, d.rho, (double*)nullptr, d.scal, d.counters, d.groups,
-                           d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot);
+                           d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d));
     launch_eos(d, s);
 }
 void launch_eos(const Dev& d, hipStream_t s) {
@@ -712,13 +787,14 @@ void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
                            d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
                            d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
-                           (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot, boundary(d, p));
+                           (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext,
+                           d.ccnt, d.cpool);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
                            d.scs, d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
                            p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand,
-                           (double*)nullptr, d.sinst(), d.iroot, boundary(d, p));
+                           (double*)nullptr, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext, d.ccnt, d.cpool);
 }
 void launch_kick_drift(const Dev& d, const StepParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,

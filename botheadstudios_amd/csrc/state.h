@@ -58,6 +58,14 @@ struct Dev {
     uint32_t* cross = nullptr;  // internal nodes whose range crosses an upsweep chunk boundary
     uint2* groups = nullptr;    // SPH walk groups (first sorted particle, count), Morton order
     uint2* ggroups = nullptr;   // gravity walk groups (up to 64 x particles-per-lane), Morton order
+    // Candidate lists of the SPH walk groups, written by the density walk and read back by the force pass
+    // (same groups, same candidates, same order): per group the first chunk, per chunk its count, the next
+    // chunk and 64 candidate slots. Chunks come in blocks of kChunksPerBlock taken by one atomic.
+    uint32_t* chead = nullptr;
+    uint32_t* cnext = nullptr;
+    uint32_t* ccnt = nullptr;
+    uint32_t* cpool = nullptr;
+    uint32_t cblocks = 0;
     uint32_t* gflag = nullptr;  // group marks at their first particle (all zero between builds)
     uint32_t* gscan = nullptr;  // per-64-particle mark counts / offsets
     uint4* nG = nullptr;   // gravity walk record, 2 x 16 B: (com.xyz, mass), (MAC radius bits, left, right, 0)
@@ -148,6 +156,10 @@ static_assert(kQueues == 1 || kQueues == 8, "one queue or one per XCD");
 static_assert(kQueueStride >= 1 && kQueueStride * 8 < 512, "queue area overflow");
 constexpr int kQueueArea = 512;  // per walk: kQueues heads, then the exhausted-range mask
 constexpr int kGroupsSph = 8, kGroupsGrav = 12;
+constexpr int kPoolHead = 20;                 // scalar slot: next free candidate-chunk block
+constexpr uint32_t kChunksPerBlock = 32;
+constexpr uint32_t kNoChunk = 0xffffffffu;
+constexpr uint32_t kChunkOverflow = 0xfffffffeu;  // the group's list did not fit: the force pass walks
 constexpr int kQueueGrav = 512, kQueueDensity = 1024, kQueueForce = 1536;
 constexpr int kScalars = 2048;
 
