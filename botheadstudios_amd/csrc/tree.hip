@@ -4,10 +4,11 @@
 // all-equal-code ranges follows Karras 2012 instead of the shader's midpoint, see k_karras) and
 // :186-241 (single-pass climb). The reference engine replaced its climb with 32+ level-synchronous sweeps
 // because WGSL cannot order the sibling reads (bh_gravity.wgsl:192-199, gpu_gravity.rs:484-519). Here the
-// climb runs in two launches whose arrivals are ordered with workgroup-scope acquire/release only (a
-// device-scope fence writes back the XCD-local L2 on every climber): pass 1 climbs inside chunks of 1024
-// leaves, pass 2 inside chunks of 65536 leaves, pass 3 (one workgroup) finishes the top of the tree. Each node is computed from its
-// two children in a fixed left/right order, so the result does not depend on arrival order.
+// climb runs in three launches whose arrivals are ordered with workgroup-scope acquire/release only (a
+// device-scope fence writes back the XCD-local L2 on every climber): pass 1 climbs inside chunks of 4096
+// leaves (four per thread), pass 2 inside chunks of 262144 leaves, pass 3 (one workgroup) finishes the top of
+// the tree. Each node is computed from its two children in a fixed left/right order, so the result does not
+// depend on arrival order.
 #include "state.h"
 
 namespace isph {
@@ -15,8 +16,9 @@ namespace isph {
 // Upsweep levels: internal nodes whose leaf range lies inside one kUpChunk-leaf chunk are finished by that
 // chunk's workgroup (pass 1); nodes crossing those chunks but inside one kUpChunk2-leaf chunk by that
 // chunk's workgroup (pass 2); the few crossing kUpChunk2 chunks by a single workgroup (pass 3).
-constexpr uint32_t kUpChunk = 1024;
-constexpr uint32_t kUpChunk2 = 65536;
+constexpr uint32_t kUpThreads = 1024;
+constexpr uint32_t kUpChunk = 4 * kUpThreads;
+constexpr uint32_t kUpChunk2 = 262144;
 
 __device__ __forceinline__ int clz_key(uint32_t x) { return __clz(x); }
 __device__ __forceinline__ int clz_key(uint64_t x) { return __clzll(x); }
@@ -178,51 +180,53 @@ __device__ __forceinline__ bool crosses(const TreeView& tv, uint32_t node, uint3
     return r.x / chunk != r.y / chunk;
 }
 
-// Pass 1: one thread per leaf (bucket moments, bh_gravity.wgsl:200-223), climbing (:224-240) through the
-// internal nodes whose range stays inside this workgroup's chunk of kUpChunk leaves.
-__global__ __launch_bounds__(kUpChunk) void k_upsweep_chunk(const float4* __restrict__ spos,
-                                                           const float* __restrict__ sh, uint32_t n, uint32_t K,
-                                                           uint32_t L, TreeView tv, float inv_theta, int quad) {
-    uint32_t c = blockIdx.x * kUpChunk + threadIdx.x;
-    if (c >= L) return;
-    uint32_t li = (L - 1) + c;
-    uint32_t start = c * K, end = min(start + K, n);
-    float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f, hmax = 0.0f;
-    float3 lo = make_float3(1.0e30f, 1.0e30f, 1.0e30f), hi = make_float3(-1.0e30f, -1.0e30f, -1.0e30f);
-    for (uint32_t s = start; s < end; ++s) {
-        float4 b = spos[s];
-        m += b.w;
-        mx += b.x * b.w;
-        my += b.y * b.w;
-        mz += b.z * b.w;
-        lo = make_float3(fminf(lo.x, b.x), fminf(lo.y, b.y), fminf(lo.z, b.z));
-        hi = make_float3(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z));
-        if (sh) hmax = fmaxf(hmax, sh[s]);  // gravity-only builds carry no smoothing lengths
-    }
-    float3 com;
-    if (end - start == 1) com = make_float3(spos[start].x, spos[start].y, spos[start].z);
-    else com = m > 0.0f ? make_float3(mx / m, my / m, mz / m) : make_float3(0.0f, 0.0f, 0.0f);
-    NodeMom lm;
-    lm.q0 = make_float4(0, 0, 0, 0);
-    lm.q1 = make_float2(0, 0);
-    if (quad && end - start > 1 && m > 0.0f) {
+// Pass 1: one thread per leaf, four leaves per thread (bucket moments, bh_gravity.wgsl:200-223), climbing
+// (:224-240) through the internal nodes whose range stays inside this workgroup's chunk of kUpChunk leaves.
+__global__ __launch_bounds__(kUpThreads) void k_upsweep_chunk(const float4* __restrict__ spos,
+                                                             const float* __restrict__ sh, uint32_t n, uint32_t K,
+                                                             uint32_t L, TreeView tv, float inv_theta, int quad) {
+    for (uint32_t part = 0; part < kUpChunk / kUpThreads; ++part) {
+        uint32_t c = blockIdx.x * kUpChunk + part * kUpThreads + threadIdx.x;
+        if (c >= L) return;
+        uint32_t li = (L - 1) + c;
+        uint32_t start = c * K, end = min(start + K, n);
+        float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f, hmax = 0.0f;
+        float3 lo = make_float3(1.0e30f, 1.0e30f, 1.0e30f), hi = make_float3(-1.0e30f, -1.0e30f, -1.0e30f);
         for (uint32_t s = start; s < end; ++s) {
             float4 b = spos[s];
-            add_shifted_quad(lm.q0, lm.q1, make_float4(0, 0, 0, 0), make_float2(0, 0), b.w / m, b.x - com.x,
-                             b.y - com.y, b.z - com.z);
+            m += b.w;
+            mx += b.x * b.w;
+            my += b.y * b.w;
+            mz += b.z * b.w;
+            lo = make_float3(fminf(lo.x, b.x), fminf(lo.y, b.y), fminf(lo.z, b.z));
+            hi = make_float3(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z));
+            if (sh) hmax = fmaxf(hmax, sh[s]);  // gravity-only builds carry no smoothing lengths
         }
-    }
-    lm.a = make_float4(com.x, com.y, com.z, m);
-    lm.b = make_float4(lo.x, lo.y, lo.z, hmax);
-    lm.c = make_float4(hi.x, hi.y, hi.z, mac_radius(lo, hi, com, inv_theta));
-    store_node(tv, li, lm, quad, 0u, 0u);
+        float3 com;
+        if (end - start == 1) com = make_float3(spos[start].x, spos[start].y, spos[start].z);
+        else com = m > 0.0f ? make_float3(mx / m, my / m, mz / m) : make_float3(0.0f, 0.0f, 0.0f);
+        NodeMom lm;
+        lm.q0 = make_float4(0, 0, 0, 0);
+        lm.q1 = make_float2(0, 0);
+        if (quad && end - start > 1 && m > 0.0f) {
+            for (uint32_t s = start; s < end; ++s) {
+                float4 b = spos[s];
+                add_shifted_quad(lm.q0, lm.q1, make_float4(0, 0, 0, 0), make_float2(0, 0), b.w / m, b.x - com.x,
+                                 b.y - com.y, b.z - com.z);
+            }
+        }
+        lm.a = make_float4(com.x, com.y, com.z, m);
+        lm.b = make_float4(lo.x, lo.y, lo.z, hmax);
+        lm.c = make_float4(hi.x, hi.y, hi.z, mac_radius(lo, hi, com, inv_theta));
+        store_node(tv, li, lm, quad, 0u, 0u);
 
-    uint32_t node = tv.parent[li];
-    while (node != kNoParent && !crosses(tv, node)) {
-        if (!second_arrival(tv.ready, node)) break;  // first child: the sibling's climber finishes it
-        NodeMom nm = merge_children(tv, node, inv_theta, quad);
-        store_node(tv, node, nm, quad, tv.left[node], tv.right[node]);
-        node = tv.parent[node];
+        uint32_t node = tv.parent[li];
+        while (node != kNoParent && !crosses(tv, node)) {
+            if (!second_arrival(tv.ready, node)) break;  // first child: the sibling's climber finishes it
+            NodeMom nm = merge_children(tv, node, inv_theta, quad);
+            store_node(tv, node, nm, quad, tv.left[node], tv.right[node]);
+            node = tv.parent[node];
+        }
     }
 }
 
@@ -367,7 +371,7 @@ void launch_karras(const Dev& d, hipStream_t s) {
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s) {
     TreeView tv{d.left, d.right, d.parent, d.nrange, d.ready, d.nG, d.nA, d.nB, d.nC, d.nQ0, d.nQ1};
     float inv_theta = 1.0f / p.theta;
-    hipLaunchKernelGGL(k_upsweep_chunk, dim3((d.L + kUpChunk - 1) / kUpChunk), dim3(kUpChunk), 0, s, d.spos, d.sh,
+    hipLaunchKernelGGL(k_upsweep_chunk, dim3((d.L + kUpChunk - 1) / kUpChunk), dim3(kUpThreads), 0, s, d.spos, d.sh,
                        d.n, d.K, d.L, tv, inv_theta, p.quad);
     if (d.L > kUpChunk)
         hipLaunchKernelGGL(k_upsweep_cross, dim3((d.L + kUpChunk2 - 1) / kUpChunk2), dim3(1024), 0, s, d.cross, d.scal,
