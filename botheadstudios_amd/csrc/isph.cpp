@@ -151,7 +151,7 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.vals, cap);
     dalloc(&d.keys_alt, cap);
     dalloc(&d.vals_alt, cap);
-    dalloc(&d.hist, 256 * (size_t)sort_blocks);
+    dalloc(&d.hist, 256 * ((size_t)sort_blocks + 1));  // counts per (digit, block), then the 256 digit totals
     d.nsort_blocks = sort_blocks;
     dalloc(&d.spos, cap);
     dalloc(&d.svel, cap);
@@ -476,7 +476,7 @@ struct EvalBufs {
     void alloc(uint32_t cap, uint32_t sort_blocks) {
         const size_t nodes = 2 * (size_t)cap;
         dalloc(&keys, cap); dalloc(&vals, cap); dalloc(&keys_alt, cap); dalloc(&vals_alt, cap);
-        dalloc(&hist, 256 * (size_t)sort_blocks);
+        dalloc(&hist, 256 * ((size_t)sort_blocks + 1));
         dalloc(&spos, cap); dalloc(&svel, cap); dalloc(&sgrav, cap); dalloc(&acc, cap);
         dalloc(&sh, cap); dalloc(&srho, cap); dalloc(&spr, cap); dalloc(&scs, cap); dalloc(&smat, cap);
         dalloc(&srho64, cap); dalloc(&spr64, cap); dalloc(&scs64, cap);

@@ -148,6 +148,40 @@ __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, uint32_t to
     }
 }
 
+// Digit-major histogram scan for the sort passes: one wave per digit scans that digit's per-block counts
+// into exclusive offsets within the digit (4 consecutive counts per lane, a wave shuffle scan) and stores
+// the digit total after the histogram (hist[256 * nb + digit]); each scatter block adds the digit bases.
+__global__ __launch_bounds__(256) void k_radix_colscan(uint32_t* hist, uint32_t nb) {
+    const uint32_t d = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    uint32_t* col = hist + (size_t)d * nb;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 4 * kWave) {
+        uint32_t v[4], s = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t idx = base + 4 * lane + k;
+            v[k] = idx < nb ? col[idx] : 0u;
+            s += v[k];
+        }
+        uint32_t incl = s;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        uint32_t run = carry + incl - s;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t idx = base + 4 * lane + k;
+            if (idx < nb) col[idx] = run;
+            run += v[k];
+        }
+        carry += __shfl(incl, kWave - 1);
+    }
+    if (lane == 0) hist[256u * nb + d] = carry;
+}
+
 // Stable scatter: within a tile, elements are ranked in index order per digit with wave64 match masks.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                                 const uint32_t* __restrict__ vin,
@@ -158,7 +192,19 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* 
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[kWaves][256];
     int tid = threadIdx.x, w = tid / kWave;
-    run[tid] = hist[tid * nb + blockIdx.x];
+    // digit bases: exclusive scan of the digit totals
+    const uint32_t tot = hist[256u * nb + tid];
+    run[tid] = tot;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t t = tid >= o ? run[tid - o] : 0u;
+        __syncthreads();
+        run[tid] += t;
+        __syncthreads();
+    }
+    const uint32_t dbase = run[tid] - tot;
+    __syncthreads();
+    run[tid] = dbase + hist[tid * nb + blockIdx.x];
 #pragma unroll
     for (int k = 0; k < kWaves; ++k) wcnt[k][tid] = 0;
     __syncthreads();
@@ -241,7 +287,7 @@ void launch_radix_sort(const Dev& d, hipStream_t s) {
     uint32_t* vout = d.vals_alt;
     for (int shift = 0; shift < 32; shift += 8) {  // 4 passes: even count, result back in keys/vals
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, kin, d.n, shift, d.hist, nb);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, d.hist, 256u * nb);
+        hipLaunchKernelGGL(k_radix_colscan, dim3(256 / (256 / kWave)), dim3(256), 0, s, d.hist, nb);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, kin, vin, kout, vout, d.n, shift,
                            d.hist, nb);
         std::swap(kin, kout);
@@ -252,7 +298,7 @@ void launch_radix_pass(const Dev& d, const uint32_t* kin, const uint32_t* vin, u
                        int shift, hipStream_t s) {
     uint32_t nb = cdiv(d.n, kSortTile);
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, kin, d.n, shift, d.hist, nb);
-    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, d.hist, 256u * nb);
+    hipLaunchKernelGGL(k_radix_colscan, dim3(256 / (256 / kWave)), dim3(256), 0, s, d.hist, nb);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, kin, vin, kout, vout, d.n, shift, d.hist,
                        nb);
 }
@@ -261,7 +307,7 @@ void launch_radix_pass_n(uint32_t n, uint32_t* hist, const uint32_t* kin, const 
     if (n == 0) return;
     uint32_t nb = cdiv(n, kSortTile);
     hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, kin, n, shift, hist, nb);
-    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, hist, 256u * nb);
+    hipLaunchKernelGGL(k_radix_colscan, dim3(256 / (256 / kWave)), dim3(256), 0, s, hist, nb);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, kin, vin, kout, vout, n, shift, hist, nb);
 }
 void launch_exclusive_scan(uint32_t* data, uint32_t count, hipStream_t s) {
