@@ -233,7 +233,7 @@ __global__ __launch_bounds__(kDirTile) void k_vjp_gravity_direct(uint32_t n, con
             const float4 pj = tp[k];
             const float4 aj = ta[k];
             const float dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;
-            const float r2 = dx * dx + dy * dy + dz * dz + soft2;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
             const float inv = __builtin_amdgcn_rsqf(r2);
             const float inv2 = inv * inv;
             const float s = inv * inv2;
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(64) void k_vjp_gravity_tree(const float4* __restric
                 float gx = 0.f, gy = 0.f, gz = 0.f;
                 if (lane < cnt && xp.w > 0.f && gI[pidx] != src_idx) {
                     const float dx = src.x - xp.x, dy = src.y - xp.y, dz = src.z - xp.z;
-                    const float r2 = dx * dx + dy * dy + dz * dz + soft2;
+                    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
                     const float inv = __builtin_amdgcn_rsqf(r2);
                     const float inv2 = inv * inv;
                     const float s = inv * inv2;

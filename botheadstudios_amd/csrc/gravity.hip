@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     float dx = c.x - me[k].x, dy = c.y - me[k].y, dz = c.z - me[k].z;
-                    float r2 = dx * dx + dy * dy + dz * dz + soft2;
+                    float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
                     float inv = __builtin_amdgcn_rsqf(r2);
                     float inv2 = inv * inv;
                     float w = c.w * inv * inv2;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
 #pragma unroll
                 for (int k = 0; k < P; ++k) {
                     float dx = c.x - me[k].x, dy = c.y - me[k].y, dz = c.z - me[k].z;
-                    float r2 = dx * dx + dy * dy + dz * dz + soft2;
+                    float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
                     float inv = __builtin_amdgcn_rsqf(r2);
                     float w = (idx == t[k]) ? 0.f : c.w * inv * inv * inv;  // s != t (bh_gravity.wgsl:362)
                     bx[k] += dx * w;
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void k_gravity_direct(const float4* __restrict
         for (uint32_t jj = 0; jj < cnt; ++jj) {
             float4 c = tile[jj];
             float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
-            float r2 = dx * dx + dy * dy + dz * dz + soft2;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
             float inv = __builtin_amdgcn_rsqf(r2);
             float w = (base + jj == i) ? 0.f : c.w * inv * inv * inv;
             bx += dx * w;

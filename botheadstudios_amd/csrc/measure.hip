@@ -133,14 +133,21 @@ __global__ __launch_bounds__(kMT) void k_meas_live(const uint2* __restrict__ chu
     }
 }
 
-__global__ void k_meas_view1(uint32_t ninst, const uint32_t* __restrict__ ichunk, const double* __restrict__ part,
-                             double* __restrict__ view, uint32_t* __restrict__ cross) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ninst) return;
+// One block per instance: chunk partials summed in a fixed order (strided per thread, then a tree).
+__global__ __launch_bounds__(kMT) void k_meas_view1(const uint32_t* __restrict__ ichunk,
+                                                    const double* __restrict__ part, double* __restrict__ view,
+                                                    uint32_t* __restrict__ cross) {
+    __shared__ double red[kMT];
+    const uint32_t k = blockIdx.x;
     double f[9];
+#pragma unroll
     for (int j = 0; j < 9; ++j) f[j] = 0.0;
-    for (uint32_t c = ichunk[k]; c < ichunk[k + 1]; ++c)
+    for (uint32_t c = ichunk[k] + threadIdx.x; c < ichunk[k + 1]; c += kMT)
+#pragma unroll
         for (int j = 0; j < 9; ++j) f[j] += part[(size_t)c * kPartFields + j];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) f[j] = block_sum(red, f[j]);
+    if (threadIdx.x != 0) return;
     double* vw = view + (size_t)k * kViewFields;
     for (int j = 0; j < kViewFields; ++j) vw[j] = 0.0;
     const double n_live = f[0], mt = f[1];
@@ -220,14 +227,40 @@ __global__ __launch_bounds__(kMT) void k_meas_scan(const uint2* __restrict__ chu
     }
 }
 
-__global__ void k_meas_coff(uint32_t ninst, const uint32_t* __restrict__ ichunk, double* __restrict__ coff) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= ninst) return;
+// Exclusive scan of one instance's chunk totals (one block per instance), left to right in one thread so
+// that each chunk's offset is exactly the previous offset plus the previous total: the cumulative mass
+// stays non-decreasing across chunk boundaries. The totals are staged through LDS so the serial chain
+// runs on LDS reads rather than on global-memory latency.
+constexpr uint32_t kCoffTile = 4096;
+__global__ __launch_bounds__(kMT) void k_meas_coff(const uint32_t* __restrict__ ichunk, double* __restrict__ coff) {
+    __shared__ double buf[kCoffTile];
+    const uint32_t c0 = ichunk[blockIdx.x], c1 = ichunk[blockIdx.x + 1];
     double run = 0.0;
-    for (uint32_t c = ichunk[k]; c < ichunk[k + 1]; ++c) {
-        const double x = coff[c];
-        coff[c] = run;
-        run += x;
+    for (uint32_t base = c0; base < c1; base += kCoffTile) {
+        const uint32_t len = min(kCoffTile, c1 - base);
+        for (uint32_t j = threadIdx.x; j < len; j += kMT) buf[j] = coff[base + j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t j = 0;
+            for (; j + 8 <= len; j += 8) {
+                double x[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) x[q] = buf[j + q];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    buf[j + q] = run;
+                    run += x[q];
+                }
+            }
+            for (; j < len; ++j) {
+                const double x = buf[j];
+                buf[j] = run;
+                run += x;
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < len; j += kMT) coff[base + j] = buf[j];
+        __syncthreads();
     }
 }
 
@@ -340,41 +373,80 @@ __global__ __launch_bounds__(kMT) void k_meas_classify(const uint2* __restrict__
     }
 }
 
-__global__ __launch_bounds__(256) void k_meas_view3(uint32_t ninst, uint32_t nchunks,
-                                                    const uint32_t* __restrict__ ichunk,
+// Classification sums per instance (one block per instance, fixed order) and its clump-finder parameters.
+__global__ __launch_bounds__(kMT) void k_meas_view3(const uint32_t* __restrict__ ichunk,
                                                     const double* __restrict__ part, double* __restrict__ view,
-                                                    uint32_t* __restrict__ mbase, uint32_t* __restrict__ moff,
-                                                    double* __restrict__ iparam, uint32_t* __restrict__ word,
-                                                    double clump_soft, double link_factor) {
-    for (uint32_t k = threadIdx.x; k < ninst; k += blockDim.x) {
-        double f[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (uint32_t c = ichunk[k]; c < ichunk[k + 1]; ++c)
-            for (int j = 0; j < 7; ++j) f[j] += part[(size_t)c * kPartFields + 9 + j];
-        double* vw = view + (size_t)k * kViewFields;
-        vw[kVDisk0] = f[0];
-        vw[kVDiskO] = f[1];
-        vw[kVEsc] = f[2];
-        vw[kVRem0] = f[3];
-        vw[kVRemO] = f[4];
-        vw[kVEsc0] = f[5];
-        vw[kVNDisk] = f[6];
-        double* pr = iparam + (size_t)k * kParamFields;
-        pr[kPLink] = link_factor * vw[kVMeanH];  // gpu_sph.rs:803: 2 x mean h of the whole field
-        pr[kPSoft2] = clump_soft * clump_soft;
-        pr[kPG] = kG64;
-        for (int a = 0; a < 3; ++a) pr[kPCPos + a] = vw[kVCom + a];
-        pr[kPCMass] = vw[kVMRem];
-        pr[kPCRad] = vw[kVRRem];
-    }
-    if (threadIdx.x == 0) {
-        uint32_t mb = 0;
-        for (uint32_t c = 0; c < nchunks; ++c) {
-            mbase[c] = mb;
-            mb += (uint32_t)part[(size_t)c * kPartFields + 15];
+                                                    double* __restrict__ iparam, double clump_soft,
+                                                    double link_factor) {
+    __shared__ double red[kMT];
+    const uint32_t k = blockIdx.x;
+    double f[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) f[j] = 0.0;
+    for (uint32_t c = ichunk[k] + threadIdx.x; c < ichunk[k + 1]; c += kMT)
+#pragma unroll
+        for (int j = 0; j < 7; ++j) f[j] += part[(size_t)c * kPartFields + 9 + j];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) f[j] = block_sum(red, f[j]);
+    if (threadIdx.x != 0) return;
+    double* vw = view + (size_t)k * kViewFields;
+    vw[kVDisk0] = f[0];
+    vw[kVDiskO] = f[1];
+    vw[kVEsc] = f[2];
+    vw[kVRem0] = f[3];
+    vw[kVRemO] = f[4];
+    vw[kVEsc0] = f[5];
+    vw[kVNDisk] = f[6];
+    double* pr = iparam + (size_t)k * kParamFields;
+    pr[kPLink] = link_factor * vw[kVMeanH];  // gpu_sph.rs:803: 2 x mean h of the whole field
+    pr[kPSoft2] = clump_soft * clump_soft;
+    pr[kPG] = kG64;
+    for (int a = 0; a < 3; ++a) pr[kPCPos + a] = vw[kVCom + a];
+    pr[kPCMass] = vw[kVMRem];
+    pr[kPCRad] = vw[kVRRem];
+}
+
+// Member base of every chunk (exclusive scan of the per-chunk disk counts, integers, so the order of the
+// additions does not matter), the first member of each instance and the member count. One block of
+// 1024 threads, 1024 chunks per tile, wave scans plus a carry.
+constexpr int kMbT = 1024;
+__global__ __launch_bounds__(kMbT) void k_meas_mbase(uint32_t ninst, uint32_t nchunks,
+                                                     const uint32_t* __restrict__ ichunk,
+                                                     const double* __restrict__ part, uint32_t* __restrict__ mbase,
+                                                     uint32_t* __restrict__ moff, uint32_t* __restrict__ word) {
+    __shared__ uint32_t wsum[kMbT / 64];
+    __shared__ uint32_t carry_s;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nchunks; base += kMbT) {
+        const uint32_t c = base + threadIdx.x;
+        const uint32_t x = c < nchunks ? (uint32_t)part[(size_t)c * kPartFields + 15] : 0u;
+        uint32_t inc = x;  // inclusive wave scan
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o);
+            if (lane >= o) inc += t;
         }
-        for (uint32_t k = 0; k < ninst; ++k) moff[k] = ichunk[k] < nchunks ? mbase[ichunk[k]] : mb;
-        moff[ninst] = mb;
-        word[0] = mb;
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t r = carry;
+            for (int q = 0; q < kMbT / 64; ++q) {
+                const uint32_t t = wsum[q];
+                wsum[q] = r;
+                r += t;
+            }
+            carry_s = r;
+        }
+        __syncthreads();
+        if (c < nchunks) mbase[c] = wsum[w] + inc - x;
+        carry = carry_s;
+        __syncthreads();
+    }
+    for (uint32_t k = threadIdx.x; k < ninst; k += kMbT) moff[k] = ichunk[k] < nchunks ? mbase[ichunk[k]] : carry;
+    if (threadIdx.x == 0) {
+        moff[ninst] = carry;
+        word[0] = carry;
     }
 }
 
@@ -753,17 +825,44 @@ __global__ __launch_bounds__(256) void k_clump_gather(uint32_t count, const uint
 
 // impact-run's clump counts (main.rs:838-866): clumps of >= 2 members, the self-bound ones, and the mass of
 // the largest that accretes (positive f64 bits order as integers).
+// Clumps arrive in instance order (a clump's id is its first member slot and member slots are grouped by
+// instance), so a wave's clumps nearly always share one instance: each wave folds the counts and the
+// largest mass of every instance it holds and one lane per instance updates the counters.
 __global__ __launch_bounds__(256) void k_clump_summary(uint32_t nc, const double* __restrict__ crec,
                                                        unsigned long long* __restrict__ csum) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const double* rec = crec + (size_t)c * kClumpFields;
-    if (rec[kCN] < 2.0) return;
-    unsigned long long* cs = csum + 3 * (size_t)rec[kCInst];
-    atomicAdd(cs, 1ull);
-    if (rec[kCBound] == 0.0) return;
-    atomicAdd(cs + 1, 1ull);
-    if (rec[kCOutside] != 0.0) atomicMax(cs + 2, (unsigned long long)__double_as_longlong(rec[kCMass]));
+    const int lane = threadIdx.x & 63;
+    uint32_t inst = 0xffffffffu;
+    bool bound = false;
+    unsigned long long mbits = 0ull;
+    if (c < nc) {
+        const double* rec = crec + (size_t)c * kClumpFields;
+        if (rec[kCN] >= 2.0) {
+            inst = (uint32_t)rec[kCInst];
+            bound = rec[kCBound] != 0.0;
+            if (bound && rec[kCOutside] != 0.0) mbits = (unsigned long long)__double_as_longlong(rec[kCMass]);
+        }
+    }
+    unsigned long long pending = __ballot(inst != 0xffffffffu);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t li = __shfl(inst, leader);
+        const bool mine = inst == li;
+        const unsigned long long mm = __ballot(mine), bb = __ballot(mine && bound);
+        unsigned long long v = mine ? mbits : 0ull;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long t = __shfl_xor(v, o);
+            v = t > v ? t : v;
+        }
+        if (lane == leader) {
+            unsigned long long* cs = csum + 3 * (size_t)li;
+            atomicAdd(cs, (unsigned long long)__popcll(mm));
+            if (bb) atomicAdd(cs + 1, (unsigned long long)__popcll(bb));
+            if (v) atomicMax(cs + 2, v);
+        }
+        pending &= ~mm;
+    }
 }
 
 __global__ void k_clear_u64(unsigned long long* p, uint32_t n) {
@@ -779,8 +878,7 @@ __global__ void k_clear_words(uint32_t* word) {
 void launch_meas_live(const Dev& d, Meas& m, uint32_t ninst, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_words, dim3(1), dim3(64), 0, s, m.word);
     hipLaunchKernelGGL(k_meas_live, dim3(m.nchunks), dim3(kMT), 0, s, m.chunks, d.posh, d.velu, d.mass, m.part);
-    hipLaunchKernelGGL(k_meas_view1, dim3(cdiv_m(ninst, 64)), dim3(64), 0, s, ninst, m.ichunk, m.part, m.view,
-                       m.cross);
+    hipLaunchKernelGGL(k_meas_view1, dim3(ninst), dim3(kMT), 0, s, m.ichunk, m.part, m.view, m.cross);
 }
 
 void launch_meas_remnant(const Dev& d, Meas& m, uint32_t ninst, hipStream_t s) {
@@ -807,7 +905,7 @@ void launch_meas_remnant(const Dev& d, Meas& m, uint32_t ninst, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_meas_scan, dim3(m.nchunks), dim3(kMT), 0, s, m.chunks, m.cinst, vs, d.posh, d.velu, d.mass,
                        m.view, m.scum, m.coff);
-    hipLaunchKernelGGL(k_meas_coff, dim3(cdiv_m(ninst, 64)), dim3(64), 0, s, ninst, m.ichunk, m.coff);
+    hipLaunchKernelGGL(k_meas_coff, dim3(ninst), dim3(kMT), 0, s, m.ichunk, m.coff);
     hipLaunchKernelGGL(k_meas_cross, dim3(cdiv_m(n, 256)), dim3(256), 0, s, n, inst, ioff, m.ichunk, vs, d.posh,
                        d.velu, d.mass, m.view, m.scum, m.coff, m.cross);
     hipLaunchKernelGGL(k_meas_view2, dim3(cdiv_m(ninst, 64)), dim3(64), 0, s, ninst, ioff, m.ichunk, vs, d.posh,
@@ -821,8 +919,10 @@ void launch_meas_classify(const Dev& d, Meas& m, uint32_t ninst, double clump_so
     const uint32_t* ioff = ninst > 1 ? d.ioff : nullptr;
     hipLaunchKernelGGL(k_meas_classify, dim3(m.nchunks), dim3(kMT), 0, s, m.chunks, m.cinst, d.posh, d.velu, d.mass,
                        d.prov, m.view, m.part, m.slot);
-    hipLaunchKernelGGL(k_meas_view3, dim3(1), dim3(256), 0, s, ninst, m.nchunks, m.ichunk, m.part, m.view, m.mbase,
-                       m.moff, m.iparam, m.word, clump_soft, link_factor);
+    hipLaunchKernelGGL(k_meas_view3, dim3(ninst), dim3(kMT), 0, s, m.ichunk, m.part, m.view, m.iparam, clump_soft,
+                       link_factor);
+    hipLaunchKernelGGL(k_meas_mbase, dim3(1), dim3(kMbT), 0, s, ninst, m.nchunks, m.ichunk, m.part, m.mbase, m.moff,
+                       m.word);
     hipLaunchKernelGGL(k_meas_gather, dim3(cdiv_m(n, 256)), dim3(256), 0, s, n, inst, ioff, m.ichunk, d.posh, d.velu,
                        d.mass, d.rho, m.slot, m.mbase, m.mp, m.mv, m.minst, m.midx);
 }
