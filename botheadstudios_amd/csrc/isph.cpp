@@ -109,7 +109,7 @@ void free_all(isph_ctx* c) {
     Dev& d = c->d;
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
-                    d.lcode, d.left, d.right, d.parent, d.ready, d.nrange, d.cross, d.groups, d.ggroups, d.chead,
+                    d.lcode, d.left, d.right, d.parent, d.cslot, d.nrange, d.nlevel, d.carry, d.ncarry, d.groups, d.ggroups, d.chead,
                     d.cnext, d.ccnt, d.cpool, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
@@ -165,9 +165,12 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.left, cap);
     dalloc(&d.right, cap);
     dalloc(&d.parent, nodes);
-    dalloc(&d.ready, cap);
+    dalloc(&d.cslot, cap);
     dalloc(&d.nrange, cap);
-    dalloc(&d.cross, cap);
+    dalloc(&d.nlevel, cap + 4);
+    d.carry_blocks = (cap + kUpLeaves - 1) / kUpLeaves;
+    dalloc(&d.carry, 2 * (size_t)d.carry_blocks * kUpCarry);
+    dalloc(&d.ncarry, 2 * (size_t)d.carry_blocks);
     dalloc(&d.groups, cap);
     dalloc(&d.ggroups, cap);
     {
@@ -467,7 +470,8 @@ struct EvalBufs {
     float *sh = nullptr, *srho = nullptr, *spr = nullptr, *scs = nullptr;
     uint32_t* smat = nullptr;
     double *srho64 = nullptr, *spr64 = nullptr, *scs64 = nullptr;
-    uint32_t *lcode = nullptr, *left = nullptr, *right = nullptr, *parent = nullptr, *ready = nullptr, *cross = nullptr;
+    uint32_t *lcode = nullptr, *left = nullptr, *right = nullptr, *parent = nullptr, *cslot = nullptr;
+    uint8_t* nlevel = nullptr;
     uint32_t *gscan = nullptr, *scal = nullptr;
     uint2 *nrange = nullptr, *groups = nullptr, *nLR = nullptr;
     uint4* nG = nullptr;
@@ -480,14 +484,14 @@ struct EvalBufs {
         dalloc(&spos, cap); dalloc(&svel, cap); dalloc(&sgrav, cap); dalloc(&acc, cap);
         dalloc(&sh, cap); dalloc(&srho, cap); dalloc(&spr, cap); dalloc(&scs, cap); dalloc(&smat, cap);
         dalloc(&srho64, cap); dalloc(&spr64, cap); dalloc(&scs64, cap);
-        dalloc(&lcode, cap); dalloc(&left, cap); dalloc(&right, cap); dalloc(&parent, nodes); dalloc(&ready, cap);
-        dalloc(&cross, cap); dalloc(&gscan, cap / kWave + 16); dalloc(&scal, kScalars);
+        dalloc(&lcode, cap); dalloc(&left, cap); dalloc(&right, cap); dalloc(&parent, nodes); dalloc(&cslot, cap);
+        dalloc(&nlevel, cap + 4); dalloc(&gscan, cap / kWave + 16); dalloc(&scal, kScalars);
         dalloc(&nrange, cap); dalloc(&groups, cap); dalloc(&nLR, cap);
         dalloc(&nG, 2 * nodes); dalloc(&nA, nodes); dalloc(&nB, nodes); dalloc(&nC, nodes);
     }
     void release() {
         void* ptrs[] = {keys, vals, keys_alt, vals_alt, hist, spos, svel, sgrav, acc, sh, srho, spr, scs, smat,
-                        srho64, spr64, scs64, lcode, left, right, parent, ready, cross, gscan, scal, nrange, groups,
+                        srho64, spr64, scs64, lcode, left, right, parent, cslot, nlevel, gscan, scal, nrange, groups,
                         nLR, nG, nA, nB, nC};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
@@ -501,7 +505,7 @@ struct EvalBufs {
         std::swap(sh, d.sh); std::swap(srho, d.srho); std::swap(spr, d.spr); std::swap(scs, d.scs);
         std::swap(smat, d.smat); std::swap(srho64, d.srho64); std::swap(spr64, d.spr64); std::swap(scs64, d.scs64);
         std::swap(lcode, d.lcode); std::swap(left, d.left); std::swap(right, d.right); std::swap(parent, d.parent);
-        std::swap(ready, d.ready); std::swap(cross, d.cross); std::swap(gscan, d.gscan); std::swap(scal, d.scal);
+        std::swap(cslot, d.cslot); std::swap(nlevel, d.nlevel); std::swap(gscan, d.gscan); std::swap(scal, d.scal);
         std::swap(nrange, d.nrange); std::swap(groups, d.groups); std::swap(nLR, d.nLR); std::swap(nG, d.nG);
         std::swap(nA, d.nA); std::swap(nB, d.nB); std::swap(nC, d.nC);
     }

@@ -7,6 +7,11 @@
 
 namespace isph {
 
+// Upsweep chunking: level-1 chunks of kUpLeaves leaves; each chunk carries at most kUpCarry ids of
+// higher-level nodes to the next level.
+constexpr uint32_t kUpLeaves = 512;
+constexpr uint32_t kUpCarry = 256;
+
 // All device arrays. "orig" arrays are in upload order; "sorted" arrays in Morton order of the last
 // evaluation (order[t] = upload index of sorted slot t).
 struct Dev {
@@ -53,9 +58,12 @@ struct Dev {
     uint32_t* left = nullptr;
     uint32_t* right = nullptr;
     uint32_t* parent = nullptr;
-    uint32_t* ready = nullptr;
+    uint32_t* cslot = nullptr;  // upsweep: LDS slot of each internal node within its level's chunk
     uint2* nrange = nullptr;    // [first, last] leaf range of each internal node
-    uint32_t* cross = nullptr;  // internal nodes whose range crosses an upsweep chunk boundary
+    uint8_t* nlevel = nullptr;  // upsweep level of each internal node
+    uint32_t* carry = nullptr;   // upsweep: higher-level node ids carried per chunk (two alternating lists)
+    uint32_t* ncarry = nullptr;  // upsweep: their counts
+    uint32_t carry_blocks = 0;   // chunks per list (level 1: kUpLeaves leaves each)
     uint2* groups = nullptr;    // SPH walk groups (first sorted particle, count), Morton order
     uint2* ggroups = nullptr;   // gravity walk groups (up to 64 x particles-per-lane), Morton order
     // Candidate lists of the SPH walk groups, written by the density walk and read back by the force pass

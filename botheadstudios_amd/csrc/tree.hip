@@ -4,21 +4,28 @@
 // all-equal-code ranges follows Karras 2012 instead of the shader's midpoint, see k_karras) and
 // :186-241 (single-pass climb). The reference engine replaced its climb with 32+ level-synchronous sweeps
 // because WGSL cannot order the sibling reads (bh_gravity.wgsl:192-199, gpu_gravity.rs:484-519). Here the
-// climb runs in three launches whose arrivals are ordered with workgroup-scope acquire/release only (a
-// device-scope fence writes back the XCD-local L2 on every climber): pass 1 climbs inside chunks of 4096
-// leaves (four per thread), pass 2 inside chunks of 262144 leaves, pass 3 (one workgroup) finishes the top of
-// the tree. Each node is computed from its two children in a fixed left/right order, so the result does not
-// depend on arrival order.
+// climb runs level by level over chunks of leaves, every arrival inside one workgroup's LDS: level 1 takes
+// chunks of 512 leaves (one leaf per thread), level k >= 2 chunks of 512 * 8^(k-1) leaves and the internal
+// nodes whose range lies inside such a chunk but crosses a level-(k-1) chunk boundary. A node's children
+// are either of its own level (they arrive through LDS) or complete from a lower level (read from HBM once).
+// Each node is computed from its two children in a fixed left/right order, so the result does not depend
+// on arrival order.
 #include "state.h"
 
 namespace isph {
 
-// Upsweep levels: internal nodes whose leaf range lies inside one kUpChunk-leaf chunk are finished by that
-// chunk's workgroup (pass 1); nodes crossing those chunks but inside one kUpChunk2-leaf chunk by that
-// chunk's workgroup (pass 2); the few crossing kUpChunk2 chunks by a single workgroup (pass 3).
-constexpr uint32_t kUpThreads = 1024;
-constexpr uint32_t kUpChunk = 4 * kUpThreads;
-constexpr uint32_t kUpChunk2 = 262144;
+// Upsweep levels: an internal node's level is the smallest k whose chunk of kLv1 * 8^(k-1) leaves holds its
+// whole range. The nodes of one level-k chunk (k >= 2) all contain one of its seven inner level-(k-1)
+// boundaries; the nodes containing one boundary are nested (one root path, at most 97 nodes for a 64-bit
+// key with the 32-bit index tie-break), so a chunk has at most 7 * 97 < kSlots of them.
+// Each workgroup also hands the next level the nodes of its chunk that belong to a higher level: such a
+// node's id lies in the chunk and its range leaves it, so it contains the chunk's left or right boundary,
+// and there are at most 2 * 97 < kCarry of them.
+constexpr uint32_t kLv1 = kUpLeaves;
+constexpr uint32_t kLvShift = 3;  // chunk growth per level: x8
+constexpr uint32_t kSlots = 768;  // LDS slots per level-k workgroup
+constexpr uint32_t kCarry = kUpCarry;  // carried higher-level ids per workgroup
+constexpr uint32_t kNoSlot = 0xffffffffu;
 
 __device__ __forceinline__ int clz_key(uint32_t x) { return __clz(x); }
 __device__ __forceinline__ int clz_key(uint64_t x) { return __clzll(x); }
@@ -32,20 +39,18 @@ __device__ __forceinline__ int delta_codes(const Key* __restrict__ codes, int L,
     return clz_key(ci ^ cj);
 }
 
-// One thread per internal node (bh_gravity.wgsl:134-184). Also clears the climb counters and the root's
-// parent (cs_tree_reset, :122-131, restricted to what is read later). Keys are the 30-bit leaf codes, or
+// One thread per internal node (bh_gravity.wgsl:134-184), with the node's upsweep level. Also sets the
+// root's parent (cs_tree_reset, :122-131, restricted to what is read later). Keys are the 30-bit leaf codes, or
 // (instance << 30 | code) for an ensemble, where each instance's leaves then form one subtree.
 template <class Key>
 __global__ __launch_bounds__(256) void k_karras(const Key* __restrict__ codes, uint32_t L,
                                                 uint32_t* __restrict__ left, uint32_t* __restrict__ right,
-                                                uint32_t* __restrict__ parent, uint32_t* __restrict__ ready,
-                                                uint2* __restrict__ nrange, uint2* __restrict__ nlr,
-                                                uint32_t* __restrict__ cross, uint32_t* scal) {
+                                                uint32_t* __restrict__ parent, uint2* __restrict__ nrange,
+                                                uint2* __restrict__ nlr, uint8_t* __restrict__ nlevel) {
     uint32_t ii = blockIdx.x * blockDim.x + threadIdx.x;
     if (ii == 0) parent[0] = kNoParent;
     if (ii + 1 >= L) return;  // L-1 internal nodes
     int i = (int)ii, Li = (int)L;
-    ready[ii] = 0;
     int d = delta_codes(codes, Li, i, i + 1) > delta_codes(codes, Li, i, i - 1) ? 1 : -1;
     int dmin = delta_codes(codes, Li, i, i - d);
     int lmax = 2;
@@ -78,7 +83,9 @@ __global__ __launch_bounds__(256) void k_karras(const Key* __restrict__ codes, u
     parent[ri] = ii;
     nrange[ii] = make_uint2((uint32_t)first, (uint32_t)last);
     nlr[ii] = make_uint2(li, ri);
-    if ((uint32_t)first / kUpChunk != (uint32_t)last / kUpChunk) cross[atomicAdd(&scal[7], 1u)] = ii;
+    uint32_t level = 1;
+    for (uint64_t span = kLv1; (uint64_t)first / span != (uint64_t)last / span; span <<= kLvShift) ++level;
+    nlevel[ii] = (uint8_t)level;
 }
 
 __device__ __forceinline__ float mac_radius(float3 lo, float3 hi, float3 com, float inv_theta) {
@@ -118,7 +125,6 @@ struct TreeView {
     const uint32_t* right;
     const uint32_t* parent;
     const uint2* nrange;
-    uint32_t* ready;
     uint4* nG;
     float4* nA;
     float4* nB;
@@ -141,13 +147,20 @@ __device__ __forceinline__ void store_node(const TreeView& tv, uint32_t i, const
     }
 }
 
-// Merge the two children of `node` (bh_gravity.wgsl:229-238: mass-weighted COM, unioned AABB; plus the
-// max smoothing length, the MAC radius and the shifted quadrupoles).
-__device__ __forceinline__ NodeMom merge_children(const TreeView& tv, uint32_t node, float inv_theta, int quad) {
-    uint32_t l = tv.left[node], r = tv.right[node];
-    float4 al = tv.nA[l], ar = tv.nA[r];
-    float4 bl = tv.nB[l], br = tv.nB[r];
-    float4 cl = tv.nC[l], cr = tv.nC[r];
+__device__ __forceinline__ NodeMom load_mom(const TreeView& tv, uint32_t i, bool quad) {
+    NodeMom m;
+    m.a = tv.nA[i];
+    m.b = tv.nB[i];
+    m.c = tv.nC[i];
+    m.q0 = quad ? tv.nQ0[i] : make_float4(0, 0, 0, 0);
+    m.q1 = quad ? tv.nQ1[i] : make_float2(0, 0);
+    return m;
+}
+
+// Parent moments from the left and right child (bh_gravity.wgsl:229-238: mass-weighted COM, unioned AABB;
+// plus the max smoothing length, the MAC radius and the shifted quadrupoles).
+__device__ __forceinline__ NodeMom merge_moms(const NodeMom& L_, const NodeMom& R_, float inv_theta, bool quad) {
+    const float4 al = L_.a, ar = R_.a, bl = L_.b, br = R_.b, cl = L_.c, cr = R_.c;
     NodeMom m;
     float mm = al.w + ar.w;
     float3 cm = mm > 0.0f ? make_float3((al.x * al.w + ar.x * ar.w) / mm, (al.y * al.w + ar.y * ar.w) / mm,
@@ -162,34 +175,105 @@ __device__ __forceinline__ NodeMom merge_children(const TreeView& tv, uint32_t n
     m.q1 = make_float2(0, 0);
     if (quad && mm > 0.0f) {
         const float fl = al.w / mm, fr = ar.w / mm;
-        add_shifted_quad(m.q0, m.q1, tv.nQ0[l], tv.nQ1[l], fl, al.x - cm.x, al.y - cm.y, al.z - cm.z);
-        add_shifted_quad(m.q0, m.q1, tv.nQ0[r], tv.nQ1[r], fr, ar.x - cm.x, ar.y - cm.y, ar.z - cm.z);
+        add_shifted_quad(m.q0, m.q1, L_.q0, L_.q1, fl, al.x - cm.x, al.y - cm.y, al.z - cm.z);
+        add_shifted_quad(m.q0, m.q1, R_.q0, R_.q1, fr, ar.x - cm.x, ar.y - cm.y, ar.z - cm.z);
     }
     return m;
 }
 
-// Arrival at an internal node with workgroup-scope acquire/release: the second arriver merges and
-// returns true. Every arrival at a given node comes from the same workgroup (the chunk's in pass 1, the
-// single workgroup of pass 2), so no device-wide cache write-back is needed.
-__device__ __forceinline__ bool second_arrival(uint32_t* ready, uint32_t node) {
-    return __hip_atomic_fetch_add(&ready[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
-}
+// LDS state of one workgroup's climb: per slot (an internal node of this level) the moments of its two
+// children as they arrive, an arrival counter, the node, its children and the slot of its parent
+// (kNoSlot when the parent belongs to a higher level; bit 31 = right child).
+template <bool QUAD>
+struct Climb {
+    float4 *A, *B, *C, *Q0;
+    float2* Q1;
+    uint32_t *cnt, *node, *par;
+    uint2* child;
 
-__device__ __forceinline__ bool crosses(const TreeView& tv, uint32_t node, uint32_t chunk = kUpChunk) {
-    uint2 r = tv.nrange[node];
-    return r.x / chunk != r.y / chunk;
-}
+    __device__ __forceinline__ void put(uint32_t s, uint32_t side, const NodeMom& m) const {
+        const uint32_t e = 2 * s + side;
+        A[e] = m.a;
+        B[e] = m.b;
+        C[e] = m.c;
+        if (QUAD) {
+            Q0[e] = m.q0;
+            Q1[e] = m.q1;
+        }
+    }
+    __device__ __forceinline__ NodeMom get(uint32_t e) const {
+        NodeMom m;
+        m.a = A[e];
+        m.b = B[e];
+        m.c = C[e];
+        m.q0 = QUAD ? Q0[e] : make_float4(0, 0, 0, 0);
+        m.q1 = QUAD ? Q1[e] : make_float2(0, 0);
+        return m;
+    }
+    // Deliver child moments `m` to slot s; the second arrival merges, writes the node to HBM and carries
+    // on to the parent slot. Arrivals are LDS atomics with workgroup-scope acquire/release.
+    __device__ __forceinline__ void deliver(uint32_t s, uint32_t side, NodeMom m, const TreeView& tv,
+                                           float inv_theta) const {
+        while (true) {
+            put(s, side, m);
+            if (__hip_atomic_fetch_add(&cnt[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
+            m = merge_moms(get(2 * s), get(2 * s + 1), inv_theta, QUAD);
+            const uint2 ch = child[s];
+            store_node(tv, node[s], m, QUAD, ch.x, ch.y);
+            const uint32_t p = par[s];
+            if (p == kNoSlot) return;
+            s = p & 0x7fffffffu;
+            side = p >> 31;
+        }
+    }
+};
 
-// Pass 1: one thread per leaf, four leaves per thread (bucket moments, bh_gravity.wgsl:200-223), climbing
-// (:224-240) through the internal nodes whose range stays inside this workgroup's chunk of kUpChunk leaves.
-__global__ __launch_bounds__(kUpThreads) void k_upsweep_chunk(const float4* __restrict__ spos,
-                                                             const float* __restrict__ sh, uint32_t n, uint32_t K,
-                                                             uint32_t L, TreeView tv, float inv_theta, int quad) {
-    for (uint32_t part = 0; part < kUpChunk / kUpThreads; ++part) {
-        uint32_t c = blockIdx.x * kUpChunk + part * kUpThreads + threadIdx.x;
-        if (c >= L) return;
-        uint32_t li = (L - 1) + c;
-        uint32_t start = c * K, end = min(start + K, n);
+#define ISPH_CLIMB_LDS(QUAD, N)                                              \
+    __shared__ float4 sA[2 * (N)], sB[2 * (N)], sC[2 * (N)];                 \
+    __shared__ float4 sQ0[QUAD ? 2 * (N) : 1];                               \
+    __shared__ float2 sQ1[QUAD ? 2 * (N) : 1];                               \
+    __shared__ uint32_t sCnt[N], sNode[N], sPar[N];                          \
+    __shared__ uint2 sChild[N];                                              \
+    const Climb<QUAD> cl{sA, sB, sC, sQ0, sQ1, sCnt, sNode, sPar, sChild}
+
+// Level 1: one thread per leaf of a kLv1-leaf chunk (bucket moments, bh_gravity.wgsl:200-223), then the
+// climb (:224-240) through the chunk's level-1 nodes, whose ids are the chunk's leaf indices (a Karras
+// node's id is one end of its range), so slot = id - first leaf of the chunk.
+template <bool QUAD>
+__global__ __launch_bounds__(kLv1) void k_upsweep_lv1(const float4* __restrict__ spos, const float* __restrict__ sh,
+                                                      uint32_t n, uint32_t K, uint32_t L, TreeView tv,
+                                                      const uint2* __restrict__ nlr,
+                                                      const uint8_t* __restrict__ nlevel, float inv_theta,
+                                                      uint32_t* __restrict__ carry, uint32_t* __restrict__ ncarry) {
+    ISPH_CLIMB_LDS(QUAD, kLv1);
+    __shared__ uint32_t nup;
+    const uint32_t c0 = blockIdx.x * kLv1, t = threadIdx.x;
+    if (t == 0) nup = 0u;
+    __syncthreads();
+    auto slot_of_parent = [&](uint32_t child) {
+        const uint32_t p = tv.parent[child];
+        if (p == kNoParent || nlevel[p] != 1) return kNoSlot;
+        return (p - c0) | (nlr[p].x == child ? 0u : 0x80000000u);
+    };
+    // slot metadata of internal node c0 + t
+    const uint32_t id = c0 + t;
+    sCnt[t] = 0u;
+    if (id + 1 < L) {
+        if (nlevel[id] == 1) {
+            sNode[t] = id;
+            sChild[t] = nlr[id];
+            sPar[t] = slot_of_parent(id);
+        } else {
+            const uint32_t e = atomicAdd(&nup, 1u);
+            if (e < kCarry) carry[(size_t)blockIdx.x * kCarry + e] = id;
+        }
+    }
+    // leaf c0 + t
+    const uint32_t c = c0 + t;
+    uint32_t target = kNoSlot;
+    NodeMom lm;
+    if (c < L) {
+        const uint32_t start = c * K, end = min(start + K, n);
         float m = 0.0f, mx = 0.0f, my = 0.0f, mz = 0.0f, hmax = 0.0f;
         float3 lo = make_float3(1.0e30f, 1.0e30f, 1.0e30f), hi = make_float3(-1.0e30f, -1.0e30f, -1.0e30f);
         for (uint32_t s = start; s < end; ++s) {
@@ -205,10 +289,9 @@ __global__ __launch_bounds__(kUpThreads) void k_upsweep_chunk(const float4* __re
         float3 com;
         if (end - start == 1) com = make_float3(spos[start].x, spos[start].y, spos[start].z);
         else com = m > 0.0f ? make_float3(mx / m, my / m, mz / m) : make_float3(0.0f, 0.0f, 0.0f);
-        NodeMom lm;
         lm.q0 = make_float4(0, 0, 0, 0);
         lm.q1 = make_float2(0, 0);
-        if (quad && end - start > 1 && m > 0.0f) {
+        if (QUAD && end - start > 1 && m > 0.0f) {
             for (uint32_t s = start; s < end; ++s) {
                 float4 b = spos[s];
                 add_shifted_quad(lm.q0, lm.q1, make_float4(0, 0, 0, 0), make_float2(0, 0), b.w / m, b.x - com.x,
@@ -218,42 +301,73 @@ __global__ __launch_bounds__(kUpThreads) void k_upsweep_chunk(const float4* __re
         lm.a = make_float4(com.x, com.y, com.z, m);
         lm.b = make_float4(lo.x, lo.y, lo.z, hmax);
         lm.c = make_float4(hi.x, hi.y, hi.z, mac_radius(lo, hi, com, inv_theta));
-        store_node(tv, li, lm, quad, 0u, 0u);
-
-        uint32_t node = tv.parent[li];
-        while (node != kNoParent && !crosses(tv, node)) {
-            if (!second_arrival(tv.ready, node)) break;  // first child: the sibling's climber finishes it
-            NodeMom nm = merge_children(tv, node, inv_theta, quad);
-            store_node(tv, node, nm, quad, tv.left[node], tv.right[node]);
-            node = tv.parent[node];
-        }
+        const uint32_t li = (L - 1) + c;
+        store_node(tv, li, lm, QUAD, 0u, 0u);
+        target = slot_of_parent(li);
     }
+    __syncthreads();
+    if (t == 0) ncarry[blockIdx.x] = min(nup, kCarry);
+    if (target != kNoSlot) cl.deliver(target & 0x7fffffffu, target >> 31, lm, tv, inv_theta);
 }
 
-// Passes 2 and 3: a workgroup finishes the crossing internal nodes of its kUpChunk2-leaf chunk (pass 2,
-// nodes crossing kUpChunk but not kUpChunk2 chunks) or of the whole tree (pass 3, one workgroup, nodes
-// crossing kUpChunk2 chunks). Children finished by an earlier pass count as complete: each crossing node
-// receives one arrival per complete child from the thread that owns it, and the climb continues upward
-// through nodes of the same pass.
-__global__ __launch_bounds__(1024) void k_upsweep_cross(const uint32_t* __restrict__ cross, const uint32_t* scal,
-                                                        uint32_t L, TreeView tv, float inv_theta, int quad, int top) {
-    const uint32_t count = scal[7];
-    const uint32_t lo = top ? kUpChunk2 : kUpChunk;  // children not crossing `lo` chunks are complete
-    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) {
-        const uint32_t node = cross[k];
-        const bool big = crosses(tv, node, kUpChunk2);
-        if (top ? !big : (big || tv.nrange[node].x / kUpChunk2 != blockIdx.x)) continue;
-        const uint32_t ch[2] = {tv.left[node], tv.right[node]};
-        for (int side = 0; side < 2; ++side) {
-            const uint32_t child = ch[side];
-            if (child < L - 1 && crosses(tv, child, lo)) continue;  // arrives by its own climb in this pass
-            uint32_t at = node;
-            while (at != kNoParent && (top || !crosses(tv, at, kUpChunk2))) {
-                if (!second_arrival(tv.ready, at)) break;
-                NodeMom nm = merge_children(tv, at, inv_theta, quad);
-                store_node(tv, at, nm, quad, tv.left[at], tv.right[at]);
-                at = tv.parent[at];
+// Level k >= 2: the workgroup of a level-k chunk takes its level-k nodes from the ids that its eight
+// level-(k-1) chunks carried up, gives each an LDS slot (`cslot` maps node -> slot for the parent links),
+// carries the rest on, reads the children that a lower level completed from HBM and climbs.
+template <bool QUAD>
+__global__ __launch_bounds__(kSlots) void k_upsweep_lvk(uint32_t L, uint32_t level, uint32_t nbelow, TreeView tv,
+                                                        const uint2* __restrict__ nlr,
+                                                        const uint8_t* __restrict__ nlevel,
+                                                        uint32_t* __restrict__ cslot,
+                                                        const uint32_t* __restrict__ carry_in,
+                                                        const uint32_t* __restrict__ ncarry_in,
+                                                        uint32_t* __restrict__ carry_out,
+                                                        uint32_t* __restrict__ ncarry_out, float inv_theta) {
+    ISPH_CLIMB_LDS(QUAD, kSlots);
+    __shared__ uint32_t nslots, nup;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        nslots = 0u;
+        nup = 0u;
+    }
+    __syncthreads();
+    for (uint32_t x = t; x < (kCarry << kLvShift); x += blockDim.x) {
+        const uint32_t below = (blockIdx.x << kLvShift) + x / kCarry, e = x % kCarry;
+        if (below >= nbelow || e >= ncarry_in[below]) continue;
+        const uint32_t id = carry_in[(size_t)below * kCarry + e];
+        if (nlevel[id] == level) {
+            const uint32_t s = atomicAdd(&nslots, 1u);
+            if (s < kSlots) {
+                sNode[s] = id;
+                cslot[id] = s;
             }
+        } else {
+            const uint32_t o = atomicAdd(&nup, 1u);
+            if (o < kCarry) carry_out[(size_t)blockIdx.x * kCarry + o] = id;
+        }
+    }
+    __syncthreads();
+    if (t == 0) ncarry_out[blockIdx.x] = min(nup, kCarry);
+    const uint32_t ns = min(nslots, kSlots);
+    for (uint32_t s = t; s < ns; s += blockDim.x) {
+        const uint32_t id = sNode[s];
+        sCnt[s] = 0u;
+        sChild[s] = nlr[id];
+        const uint32_t p = tv.parent[id];
+        uint32_t ps = kNoSlot;
+        if (p != kNoParent && nlevel[p] == level) {
+            const uint32_t cs = cslot[p];
+            if (cs < ns) ps = cs | (nlr[p].x == id ? 0u : 0x80000000u);
+        }
+        sPar[s] = ps;
+    }
+    __syncthreads();
+    for (uint32_t s = t; s < ns; s += blockDim.x) {
+        const uint2 ch = sChild[s];
+#pragma unroll
+        for (uint32_t side = 0; side < 2; ++side) {
+            const uint32_t c = side ? ch.y : ch.x;
+            if (c + 1 < L && nlevel[c] == level) continue;  // arrives by its own climb
+            cl.deliver(s, side, load_mom(tv, c, QUAD), tv, inv_theta);
         }
     }
 }
@@ -363,21 +477,35 @@ void launch_karras(const Dev& d, hipStream_t s) {
     uint32_t threads = d.L > 1 ? d.L - 1 : 1;
     if (d.ninst > 1)
         hipLaunchKernelGGL(k_karras<uint64_t>, dim3((threads + 255) / 256), dim3(256), 0, s, d.lkey, d.L, d.left,
-                           d.right, d.parent, d.ready, d.nrange, d.nLR, d.cross, d.scal);
+                           d.right, d.parent, d.nrange, d.nLR, d.nlevel);
     else
         hipLaunchKernelGGL(k_karras<uint32_t>, dim3((threads + 255) / 256), dim3(256), 0, s, d.lcode, d.L, d.left,
-                           d.right, d.parent, d.ready, d.nrange, d.nLR, d.cross, d.scal);
+                           d.right, d.parent, d.nrange, d.nLR, d.nlevel);
 }
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s) {
-    TreeView tv{d.left, d.right, d.parent, d.nrange, d.ready, d.nG, d.nA, d.nB, d.nC, d.nQ0, d.nQ1};
-    float inv_theta = 1.0f / p.theta;
-    hipLaunchKernelGGL(k_upsweep_chunk, dim3((d.L + kUpChunk - 1) / kUpChunk), dim3(kUpThreads), 0, s, d.spos, d.sh,
-                       d.n, d.K, d.L, tv, inv_theta, p.quad);
-    if (d.L > kUpChunk)
-        hipLaunchKernelGGL(k_upsweep_cross, dim3((d.L + kUpChunk2 - 1) / kUpChunk2), dim3(1024), 0, s, d.cross, d.scal,
-                           d.L, tv, inv_theta, p.quad, 0);
-    if (d.L > kUpChunk2)
-        hipLaunchKernelGGL(k_upsweep_cross, dim3(1), dim3(1024), 0, s, d.cross, d.scal, d.L, tv, inv_theta, p.quad, 1);
+    TreeView tv{d.left, d.right, d.parent, d.nrange, d.nG, d.nA, d.nB, d.nC, d.nQ0, d.nQ1};
+    const float inv_theta = 1.0f / p.theta;
+    // carried ids: level-1 workgroups write list 0, level k reads list (k % 2) ^ 1 ... alternating halves
+    uint32_t* lists[2] = {d.carry, d.carry + (size_t)d.carry_blocks * kCarry};
+    uint32_t* counts[2] = {d.ncarry, d.ncarry + d.carry_blocks};
+    auto go = [&](auto quad) {
+        constexpr bool Q = decltype(quad)::value;
+        uint32_t nbelow = (d.L + kLv1 - 1) / kLv1;
+        hipLaunchKernelGGL(k_upsweep_lv1<Q>, dim3(nbelow), dim3(kLv1), 0, s, d.spos, d.sh, d.n, d.K, d.L, tv, d.nLR,
+                           d.nlevel, inv_theta, lists[0], counts[0]);
+        uint32_t level = 2, in = 0;
+        for (uint64_t span = (uint64_t)kLv1 << kLvShift; (span >> kLvShift) < d.L; span <<= kLvShift, ++level) {
+            const uint32_t blocks = (uint32_t)((d.L + span - 1) / span);
+            hipLaunchKernelGGL(k_upsweep_lvk<Q>, dim3(blocks), dim3(kSlots), 0, s, d.L, level, nbelow, tv, d.nLR,
+                               d.nlevel, d.cslot, lists[in], counts[in], lists[in ^ 1], counts[in ^ 1], inv_theta);
+            nbelow = blocks;
+            in ^= 1;
+        }
+    };
+    if (p.quad)
+        go(std::true_type{});
+    else
+        go(std::false_type{});
 }
 
 }  // namespace isph
