@@ -31,6 +31,7 @@ __global__ __launch_bounds__(1024) void k_reset_scalars(uint32_t* scal, int mode
 __global__ __launch_bounds__(256) void k_bbox(const float4* __restrict__ pos, uint32_t n, uint32_t* scal) {
     __shared__ uint32_t red[6][4];
     uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+#pragma unroll 4
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float4 p = pos[i];
         uint32_t k0 = bbox_key(p.x), k1 = bbox_key(p.y), k2 = bbox_key(p.z);
@@ -273,7 +274,8 @@ void launch_reset_scalars(const Dev& d, hipStream_t s, int mode) {
     hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(1024), 0, s, d.scal, mode);
 }
 void launch_bbox(const Dev& d, const float4* pos, hipStream_t s) {
-    uint32_t blocks = std::min<uint32_t>(cdiv(d.n, 256), 2048);
+    // few blocks: each ends in six same-address atomics, which serialise in L2
+    uint32_t blocks = std::min<uint32_t>(cdiv(d.n, 256), 256);
     hipLaunchKernelGGL(k_bbox, dim3(std::max<uint32_t>(blocks, 1)), dim3(256), 0, s, pos, d.n, d.scal);
 }
 void launch_morton(const Dev& d, const float4* pos, hipStream_t s) {
