@@ -23,8 +23,12 @@ constexpr int kStackCap = ISPH_GRAVITY_STACK;
 constexpr int kStackMargin = 128;  // > maximum Karras depth (30 code bits + 32 index tie-break bits)
 constexpr int kBuf = 128;
 
-template <int P, bool QUAD, bool COMP>
-__global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ spos, uint32_t n, uint32_t K,
+// Monopole walks put leaf particles into the cell list: with softening, a particle's own term has d = 0 and
+// adds exactly zero, so the reference's s != t test (bh_gravity.wgsl:362) is only needed without softening
+// (SELF), where the list also carries each entry's particle index. One list keeps the workgroup's LDS at
+// ~4 KB, so the resident waves per SIMD are set by registers alone.
+template <int P, bool QUAD, bool COMP, bool SELF>
+__global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(const float4* __restrict__ spos, uint32_t n, uint32_t K,
                                                      uint32_t L, const uint4* __restrict__ nG,
                                                      const float4* __restrict__ nQ0, const float2* __restrict__ nQ1,
                                                      float soft2, float4* __restrict__ sgrav,
@@ -32,12 +36,14 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                                                      uint32_t* scal, uint32_t count_slot,
                                                      const uint32_t* __restrict__ sinst,
                                                      const uint32_t* __restrict__ iroot) {
+    constexpr bool UNI = !QUAD;
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 cA[kBuf];
     __shared__ float4 cQ0[QUAD ? kBuf : 1];
     __shared__ float2 cQ1[QUAD ? kBuf : 1];
-    __shared__ float4 pA[kBuf];
-    __shared__ uint32_t pI[kBuf];
+    __shared__ uint32_t cI[(UNI && SELF) ? kBuf : 1];
+    __shared__ float4 pA[UNI ? 1 : kBuf];
+    __shared__ uint32_t pI[UNI ? 1 : kBuf];
 
     const int lane = threadIdx.x;
     const uint32_t leaf0 = L - 1;
@@ -91,6 +97,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
             for (int k = 0; k < P; ++k) bx[k] = by[k] = bz[k] = 0.f;
             for (int j = 0; j < cnt; ++j) {
                 const float4 c = cA[j];
+                const uint32_t idx = (UNI && SELF) ? cI[j] : 0u;
                 float4 q0;
                 float2 q1;
                 if (QUAD) {
@@ -116,6 +123,7 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                         by[k] += w * (dy * s - qy * inv2);
                         bz[k] += w * (dz * s - qz * inv2);
                     } else {
+                        if (UNI && SELF) w = (idx == t[k]) ? 0.f : w;  // s != t (bh_gravity.wgsl:362)
                         bx[k] += dx * w;
                         by[k] += dy * w;
                         bz[k] += dz * w;
@@ -124,7 +132,6 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
             }
 #pragma unroll
             for (int k = 0; k < P; ++k) add(k, bx[k], by[k], bz[k]);
-            ncell += cnt;
         };
         auto flush_parts = [&](int cnt) {
             float bx[P], by[P], bz[P];
@@ -146,13 +153,13 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
             }
 #pragma unroll
             for (int k = 0; k < P; ++k) add(k, bx[k], by[k], bz[k]);
-            npart += cnt;
         };
         // Drop the first 64 entries of a list after they were evaluated.
         auto shift_cells = [&](int cnt) {
             __syncthreads();
             if (lane + kWave < cnt) {
                 cA[lane] = cA[lane + kWave];
+                if (UNI && SELF) cI[lane] = cI[lane + kWave];
                 if (QUAD) {
                     cQ0[lane] = cQ0[lane + kWave];
                     cQ1[lane] = cQ1[lane + kWave];
@@ -223,13 +230,21 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
             if (accept) {
                 int pos = nc + prefix_count(ma);
                 cA[pos] = A;
+                if (UNI && SELF) cI[pos] = 0xffffffffu;
                 if (QUAD) {
                     cQ0[pos] = nQ0[node];
                     cQ1[pos] = nQ1[node];
                 }
             }
             nc += __popcll(ma);
+            ncell += __popcll(ma);
             const uint64_t ml = __ballot(leaf);
+            if (UNI && ml && nc >= kWave) {  // room for a full batch of leaf particles
+                __syncthreads();
+                flush_cells(kWave);
+                shift_cells(nc);
+                nc -= kWave;
+            }
             if (ml) {
                 const uint32_t c = leaf ? node - leaf0 : 0u;
                 const uint32_t start = c * K;
@@ -241,17 +256,35 @@ __global__ __launch_bounds__(64) void k_gravity_tree(const float4* __restrict__ 
                     if (present) {
                         // Two explicit stores: a conditional on the float4 lvalues would select between a
                         // register value and a global address, and forces A into private memory.
-                        int pos = np + prefix_count(mp);
-                        if (K == 1) pA[pos] = A;
-                        else pA[pos] = spos[start + kk];
-                        pI[pos] = start + kk;
+                        if (UNI) {
+                            int pos = nc + prefix_count(mp);
+                            if (K == 1) cA[pos] = A;
+                            else cA[pos] = spos[start + kk];
+                            if (SELF) cI[pos] = start + kk;
+                        } else {
+                            int pos = np + prefix_count(mp);
+                            if (K == 1) pA[pos] = A;
+                            else pA[pos] = spos[start + kk];
+                            pI[pos] = start + kk;
+                        }
                     }
-                    np += __popcll(mp);
-                    if (np >= kWave) {
-                        __syncthreads();
-                        flush_parts(kWave);
-                        shift_parts(np);
-                        np -= kWave;
+                    npart += __popcll(mp);
+                    if (UNI) {
+                        nc += __popcll(mp);
+                        if (nc >= kWave) {
+                            __syncthreads();
+                            flush_cells(kWave);
+                            shift_cells(nc);
+                            nc -= kWave;
+                        }
+                    } else {
+                        np += __popcll(mp);
+                        if (np >= kWave) {
+                            __syncthreads();
+                            flush_parts(kWave);
+                            shift_parts(np);
+                            np -= kWave;
+                        }
                     }
                 }
             }
@@ -354,16 +387,22 @@ void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s) {
                            d.sgrav, d.counters, own ? d.ggroups : d.groups, d.scal,
                            (uint32_t)(own ? kGroupsGrav : kGroupsSph), d.sinst(), d.iroot);
     };
-    const int sel = (p.grav_ppl == 2 ? 4 : 0) | (p.quad ? 2 : 0) | (p.mixed ? 1 : 0);
+    // the monopole walks exclude a particle's own term explicitly only without softening
+    const bool self = !p.quad && !(p.soft2 > 0.0f);
+    const int sel = (p.grav_ppl == 2 ? 8 : 0) | (p.quad ? 4 : 0) | (p.mixed ? 2 : 0) | (self ? 1 : 0);
     switch (sel) {
-        case 0: go(k_gravity_tree<1, false, false>); break;
-        case 1: go(k_gravity_tree<1, false, true>); break;
-        case 2: go(k_gravity_tree<1, true, false>); break;
-        case 3: go(k_gravity_tree<1, true, true>); break;
-        case 4: go(k_gravity_tree<2, false, false>); break;
-        case 5: go(k_gravity_tree<2, false, true>); break;
-        case 6: go(k_gravity_tree<2, true, false>); break;
-        default: go(k_gravity_tree<2, true, true>); break;
+        case 0: go(k_gravity_tree<1, false, false, false>); break;
+        case 1: go(k_gravity_tree<1, false, false, true>); break;
+        case 2: go(k_gravity_tree<1, false, true, false>); break;
+        case 3: go(k_gravity_tree<1, false, true, true>); break;
+        case 4: go(k_gravity_tree<1, true, false, false>); break;
+        case 6: go(k_gravity_tree<1, true, true, false>); break;
+        case 8: go(k_gravity_tree<2, false, false, false>); break;
+        case 9: go(k_gravity_tree<2, false, false, true>); break;
+        case 10: go(k_gravity_tree<2, false, true, false>); break;
+        case 11: go(k_gravity_tree<2, false, true, true>); break;
+        case 12: go(k_gravity_tree<2, true, false, false>); break;
+        default: go(k_gravity_tree<2, true, true, false>); break;
     }
 }
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s) {

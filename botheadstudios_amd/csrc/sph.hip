@@ -130,7 +130,7 @@ __device__ __forceinline__ uint64_t self_bit(const uint32_t* own, const uint32_t
 }
 
 template <bool F64>
-__global__ __launch_bounds__(64) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
+__global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                 uint32_t n, uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
                                                 const float4* __restrict__ nB,
                                                 const float4* __restrict__ nC, const uint32_t* __restrict__ order,
