@@ -522,9 +522,8 @@ __device__ __forceinline__ double spline_d2w(double q, double inv_h, double sig4
 
 __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                   const float4* __restrict__ svel, uint32_t n, uint32_t K,
-                                                  uint32_t L, const uint2* __restrict__ nlr,
+                                                  uint32_t L, const SphNode* __restrict__ nS,
                                                   const uint2* __restrict__ nrange, uint32_t expand,
-                                                  const float4* __restrict__ nB, const float4* __restrict__ nC,
                                                   const double* __restrict__ srho64,
                                                   const double* __restrict__ spr64,
                                                   const double* __restrict__ scs64, float av_alpha, float av_beta,
@@ -555,7 +554,7 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
         const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
         const float hg = wave_max(hi);
         double xb[3] = {0, 0, 0}, vbar[3] = {0, 0, 0}, rb = 0, qb = 0, cb = 0, mb = 0;
-        neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
+        neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
                        [&](int cnt) {
             if (lane < cnt) {
                 uint32_t j = cand[lane];
@@ -708,9 +707,8 @@ __global__ __launch_bounds__(kEosAdjThreads) void k_vjp_eos(uint32_t n, const fl
 // sum_j rbar_j W(r). Retired particles keep their density: their rbar does not propagate.
 __global__ __launch_bounds__(64) void k_vjp_density(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                     uint32_t n, uint32_t K, uint32_t L,
-                                                    const uint2* __restrict__ nlr, const uint2* __restrict__ nrange,
-                                                    uint32_t expand, const float4* __restrict__ nB,
-                                                    const float4* __restrict__ nC, const double* __restrict__ srb,
+                                                    const SphNode* __restrict__ nS, const uint2* __restrict__ nrange,
+                                                    uint32_t expand, const double* __restrict__ srb,
                                                     const uint2* __restrict__ groups,
                                                     const uint32_t* __restrict__ scal, double4* sxb) {
     __shared__ uint32_t stk[kSphStack];
@@ -734,7 +732,7 @@ __global__ __launch_bounds__(64) void k_vjp_density(const float4* __restrict__ s
         const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
         const float hg = wave_max(hi);
         double xb[3] = {0, 0, 0}, mb = 0;
-        neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
+        neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
                        [&](int cnt) {
             if (lane < cnt) {
                 uint32_t j = cand[lane];
@@ -883,13 +881,13 @@ void launch_vjp_eval(const Dev& d, const StepParams& p, int gravity_mode, const 
                            a.nbar, d.spos, a.sxb);
     }
     hipLaunchKernelGGL(k_vjp_force, dim3(std::min<uint32_t>(kWalkWaves, n)), dim3(kWave), 0, s, d.spos, d.sh,
-                       d.svel, n, d.K, d.L, d.nLR, d.nrange, (uint32_t)p.sph_expand, d.nB, d.nC, d.srho64,
+                       d.svel, n, d.K, d.L, d.nS, d.nrange, (uint32_t)p.sph_expand, d.srho64,
                        d.spr64, d.scs64, p.av_alpha, p.av_beta, a.sa, d.groups, d.scal, a.sxb, a.svb, a.srb,
                        a.sqb, a.scb);
     hipLaunchKernelGGL(k_vjp_eos, dim3(std::min<uint32_t>(1024, cdiv_u(n, kEosAdjThreads))), dim3(kEosAdjThreads),
                        0, s, n, d.svel, d.smat, d.eos, d.srho64, a.sqb, a.scb, a.srb, a.svb, a.pbar);
     hipLaunchKernelGGL(k_vjp_density, dim3(std::min<uint32_t>(kWalkWaves, n)), dim3(kWave), 0, s, d.spos, d.sh, n,
-                       d.K, d.L, d.nLR, d.nrange, (uint32_t)p.sph_expand, d.nB, d.nC, a.srb, d.groups, d.scal,
+                       d.K, d.L, d.nS, d.nrange, (uint32_t)p.sph_expand, a.srb, d.groups, d.scal,
                        a.sxb);
     hipLaunchKernelGGL(k_vjp_scatter, dim3(cdiv_u(n, 256)), dim3(256), 0, s, n, d.vals, a.sxb, a.svb, a.xb, a.vb,
                        a.ub, a.mb);

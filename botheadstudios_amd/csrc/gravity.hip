@@ -28,8 +28,8 @@ constexpr int kBuf = 128;
 // (SELF), where the list also carries each entry's particle index. One list keeps the workgroup's LDS at
 // ~4 KB, so the resident waves per SIMD are set by registers alone.
 template <int P, bool QUAD, bool COMP, bool SELF>
-__global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(const float4* __restrict__ spos, uint32_t n, uint32_t K,
-                                                     uint32_t L, const uint4* __restrict__ nG,
+__global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(const float4* __restrict__ spos,
+                                                                                uint32_t n, uint32_t K, uint32_t L, const uint4* __restrict__ nG,
                                                      const float4* __restrict__ nQ0, const float2* __restrict__ nQ1,
                                                      float soft2, float4* __restrict__ sgrav,
                                                      unsigned long long* counters, const uint2* __restrict__ groups,
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
     const uint32_t leaf0 = L - 1;
     unsigned long long tot_cell = 0, tot_part = 0, rounds = 0, popped = 0;
     int max_sp = 1;
-    WalkQueue queue(scal + kQueueGrav, scal[count_slot]);
+    WalkQueue queue(scal + kQueueGrav, scal[count_slot], 2);
     uint32_t gi;
     while (queue.next(gi)) {
         const uint2 grp = groups[gi];

@@ -110,7 +110,7 @@ void free_all(isph_ctx* c) {
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
                     d.lcode, d.left, d.right, d.parent, d.cslot, d.nrange, d.nlevel, d.carry, d.ncarry, d.groups, d.ggroups, d.chead,
-                    d.cnext, d.ccnt, d.cpool, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nLR, d.nA, d.nB, d.nC, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.cnext, d.ccnt, d.cpool, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nS, d.nA, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -186,10 +186,9 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.gscan, cap / kWave + 16);
     HIP_OK(hipMemset(d.gflag, 0, (size_t)cap * sizeof(uint32_t)));
     dalloc(&d.nG, 2 * nodes);
-    dalloc(&d.nLR, cap);
+
     dalloc(&d.nA, nodes);
-    dalloc(&d.nB, nodes);
-    dalloc(&d.nC, nodes);
+    dalloc(&d.nS, nodes);
     dalloc(&d.nQ0, nodes);
     dalloc(&d.nQ1, nodes);
     dalloc(&d.scal, kScalars);
@@ -473,9 +472,10 @@ struct EvalBufs {
     uint32_t *lcode = nullptr, *left = nullptr, *right = nullptr, *parent = nullptr, *cslot = nullptr;
     uint8_t* nlevel = nullptr;
     uint32_t *gscan = nullptr, *scal = nullptr;
-    uint2 *nrange = nullptr, *groups = nullptr, *nLR = nullptr;
+    uint2 *nrange = nullptr, *groups = nullptr;
+    SphNode* nS = nullptr;
     uint4* nG = nullptr;
-    float4 *nA = nullptr, *nB = nullptr, *nC = nullptr;
+    float4* nA = nullptr;
 
     void alloc(uint32_t cap, uint32_t sort_blocks) {
         const size_t nodes = 2 * (size_t)cap;
@@ -486,13 +486,13 @@ struct EvalBufs {
         dalloc(&srho64, cap); dalloc(&spr64, cap); dalloc(&scs64, cap);
         dalloc(&lcode, cap); dalloc(&left, cap); dalloc(&right, cap); dalloc(&parent, nodes); dalloc(&cslot, cap);
         dalloc(&nlevel, cap + 4); dalloc(&gscan, cap / kWave + 16); dalloc(&scal, kScalars);
-        dalloc(&nrange, cap); dalloc(&groups, cap); dalloc(&nLR, cap);
-        dalloc(&nG, 2 * nodes); dalloc(&nA, nodes); dalloc(&nB, nodes); dalloc(&nC, nodes);
+        dalloc(&nrange, cap); dalloc(&groups, cap);
+        dalloc(&nG, 2 * nodes); dalloc(&nA, nodes); dalloc(&nS, nodes);
     }
     void release() {
         void* ptrs[] = {keys, vals, keys_alt, vals_alt, hist, spos, svel, sgrav, acc, sh, srho, spr, scs, smat,
                         srho64, spr64, scs64, lcode, left, right, parent, cslot, nlevel, gscan, scal, nrange, groups,
-                        nLR, nG, nA, nB, nC};
+                        nS, nG, nA};
         for (void* q : ptrs)
             if (q) (void)hipFree(q);
         *this = EvalBufs{};
@@ -506,8 +506,8 @@ struct EvalBufs {
         std::swap(smat, d.smat); std::swap(srho64, d.srho64); std::swap(spr64, d.spr64); std::swap(scs64, d.scs64);
         std::swap(lcode, d.lcode); std::swap(left, d.left); std::swap(right, d.right); std::swap(parent, d.parent);
         std::swap(cslot, d.cslot); std::swap(nlevel, d.nlevel); std::swap(gscan, d.gscan); std::swap(scal, d.scal);
-        std::swap(nrange, d.nrange); std::swap(groups, d.groups); std::swap(nLR, d.nLR); std::swap(nG, d.nG);
-        std::swap(nA, d.nA); std::swap(nB, d.nB); std::swap(nC, d.nC);
+        std::swap(nrange, d.nrange); std::swap(groups, d.groups); std::swap(nS, d.nS); std::swap(nG, d.nG);
+        std::swap(nA, d.nA);
     }
 };
 
@@ -1590,8 +1590,10 @@ int isph_probe_tree(isph_t* h, uint32_t* left, uint32_t* right, uint32_t* parent
         if (right && L > 1) HIP_OK(hipMemcpy(right, h->d.right, (L - 1) * 4, hipMemcpyDeviceToHost));
         if (parent) HIP_OK(hipMemcpy(parent, h->d.parent, nodes * 4, hipMemcpyDeviceToHost));
         if (com4) HIP_OK(hipMemcpy(com4, h->d.nA, nodes * 16, hipMemcpyDeviceToHost));
-        if (bmin4) HIP_OK(hipMemcpy(bmin4, h->d.nB, nodes * 16, hipMemcpyDeviceToHost));
-        if (bmax4) HIP_OK(hipMemcpy(bmax4, h->d.nC, nodes * 16, hipMemcpyDeviceToHost));
+        if (bmin4)
+            HIP_OK(hipMemcpy2D(bmin4, 16, &h->d.nS->lo, sizeof(SphNode), 16, nodes, hipMemcpyDeviceToHost));
+        if (bmax4)
+            HIP_OK(hipMemcpy2D(bmax4, 16, &h->d.nS->hi, sizeof(SphNode), 16, nodes, hipMemcpyDeviceToHost));
         h->reuse_valid = false;
         return (int)L;
     });

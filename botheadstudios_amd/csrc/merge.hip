@@ -16,9 +16,8 @@ namespace isph {
 __global__ __launch_bounds__(64) void k_merge_pick(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                    const float4* __restrict__ svel, const uint32_t* __restrict__ smat,
                                                    const float* __restrict__ scs, uint32_t n, uint32_t K, uint32_t L,
-                                                   const uint2* __restrict__ nlr, const uint2* __restrict__ nrange,
-                                                   uint32_t expand, const float4* __restrict__ nB,
-                                                   const float4* __restrict__ nC, const uint32_t* __restrict__ order,
+                                                   const SphNode* __restrict__ nS, const uint2* __restrict__ nrange,
+                                                   uint32_t expand, const uint32_t* __restrict__ order,
                                                    const uint2* __restrict__ groups, const uint32_t* __restrict__ scal,
                                                    const uint32_t* __restrict__ sinst,
                                                    const uint32_t* __restrict__ iroot, uint32_t* __restrict__ target) {
@@ -44,7 +43,7 @@ __global__ __launch_bounds__(64) void k_merge_pick(const float4* __restrict__ sp
         const float hg = wave_max(hi);
         uint32_t best = kNoMerge;
         float best_mass = me.w;
-        neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
+        neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
                        [&](int cnt) {
             for (int jj = 0; jj < cnt; ++jj) {
                 const uint32_t j = cand[jj];
@@ -72,9 +71,8 @@ __global__ __launch_bounds__(64) void k_merge_pick(const float4* __restrict__ sp
 
 __global__ __launch_bounds__(64) void k_merge_apply(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                     const float4* __restrict__ svel, uint32_t n, uint32_t K,
-                                                    uint32_t L, const uint2* __restrict__ nlr,
+                                                    uint32_t L, const SphNode* __restrict__ nS,
                                                     const uint2* __restrict__ nrange, uint32_t expand,
-                                                    const float4* __restrict__ nB, const float4* __restrict__ nC,
                                                     const uint32_t* __restrict__ order,
                                                     const uint2* __restrict__ groups,
                                                     const uint32_t* __restrict__ scal,
@@ -114,7 +112,7 @@ __global__ __launch_bounds__(64) void k_merge_apply(const float4* __restrict__ s
             ke += 0.5 * m * v * v;
         }
         uint32_t absorbed = 0;
-        neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
+        neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg, nullptr,
                        [&](int cnt) {
             for (int jj = 0; jj < cnt; ++jj) {
                 const uint32_t i = cand[jj];
@@ -170,10 +168,10 @@ __global__ __launch_bounds__(256) void k_merge_retire(uint32_t n, const uint32_t
 void launch_merge(const Dev& d, const StepParams& p, hipStream_t s) {
     const uint32_t waves = std::min<uint32_t>(kWalkWaves, d.n);
     hipLaunchKernelGGL(k_merge_pick, dim3(waves), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.smat, d.scs, d.n, d.K,
-                       d.L, d.nLR, d.nrange, (uint32_t)p.sph_expand, d.nB, d.nC, d.vals, d.groups, d.scal, d.sinst(),
+                       d.L, d.nS, d.nrange, (uint32_t)p.sph_expand, d.vals, d.groups, d.scal, d.sinst(),
                        d.iroot, d.mtarget);
-    hipLaunchKernelGGL(k_merge_apply, dim3(waves), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.n, d.K, d.L, d.nLR,
-                       d.nrange, (uint32_t)p.sph_expand, d.nB, d.nC, d.vals, d.groups, d.scal, d.sinst(), d.iroot,
+    hipLaunchKernelGGL(k_merge_apply, dim3(waves), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.n, d.K, d.L, d.nS,
+                       d.nrange, (uint32_t)p.sph_expand, d.vals, d.groups, d.scal, d.sinst(), d.iroot,
                        d.mtarget, d.posh, d.velu, d.mass, d.pos64, d.vel64);
     hipLaunchKernelGGL(k_merge_retire, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.mtarget, d.velu, d.mass,
                        d.vel64);

@@ -54,10 +54,11 @@ struct CandCache {
     uint32_t blocks;
 };
 
-// Wave-uniform writer state: the current block of chunks (taken with one atomic per kChunksPerBlock
-// chunks), and the group's first and last chunk. A group whose chunks run out is marked for the walk.
+// Wave-uniform writer state: the current block of chunks and the group's first and last chunk. A wave
+// starts in block `blockIdx.x` (the first kWalkWaves blocks are one per wave) and takes further blocks
+// with one atomic per kChunksPerBlock chunks. A group whose chunks run out is marked for the walk.
 struct CacheWriter {
-    uint32_t blk = kNoChunk, used = kChunksPerBlock, first = kNoChunk, last = kNoChunk;
+    uint32_t blk = blockIdx.x, used = 0, first = kNoChunk, last = kNoChunk;
     bool ok = true;
     __device__ __forceinline__ void begin() {
         first = last = kNoChunk;
@@ -68,7 +69,7 @@ struct CacheWriter {
         if (!ok) return;
         if (used == kChunksPerBlock) {
             uint32_t b = 0;
-            if (lane == 0) b = atomicAdd(&scal[kPoolHead], 1u);
+            if (lane == 0) b = kWalkWaves + atomicAdd(&scal[kPoolHead], 1u);
             b = __builtin_amdgcn_readfirstlane(b);
             if (b >= cc.blocks) {
                 ok = false;
@@ -130,10 +131,9 @@ __device__ __forceinline__ uint64_t self_bit(const uint32_t* own, const uint32_t
 }
 
 template <bool F64>
-__global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __restrict__ spos, const float* __restrict__ sh,
-                                                uint32_t n, uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
-                                                const float4* __restrict__ nB,
-                                                const float4* __restrict__ nC, const uint32_t* __restrict__ order,
+__global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __restrict__ spos,
+                                                             const float* __restrict__ sh, uint32_t n, uint32_t K, uint32_t L, const SphNode* __restrict__ nS,
+                                                const uint32_t* __restrict__ order,
                                                 float* __restrict__ srho, float* __restrict__ rho_orig,
                                                 double* __restrict__ srho64, uint32_t* scal,
                                                 unsigned long long* counters, const uint2* __restrict__ groups,
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     own[lane] = kWave;
     unsigned long long tested_all = 0;
     CacheWriter cw;
-    WalkQueue queue(scal + kQueueDensity, scal[kGroupsSph]);
+    WalkQueue queue(scal + kQueueDensity, scal[kGroupsSph], 2);
     uint32_t gi;
     while (queue.next(gi)) {
     const uint2 grp = groups[gi];
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     double rho64 = 0.0;
     unsigned long long tested = 0;
     cw.begin();
-    neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
+    neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                    counters ? counters + 8 : nullptr, [&](int cnt) {
         cw.put(cc, scal, cand, cnt, lane);
         if (lane < cnt) {
@@ -294,9 +294,8 @@ template <bool F64>
 __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
                                               const float4* __restrict__ svel, const float* __restrict__ srho,
                                               const float* __restrict__ spr, const float* __restrict__ scs, uint32_t n,
-                                              uint32_t K, uint32_t L, const uint2* __restrict__ nlr,
-                                              const float4* __restrict__ nB,
-                                              const float4* __restrict__ nC, const uint32_t* __restrict__ order,
+                                              uint32_t K, uint32_t L, const SphNode* __restrict__ nS,
+                                              const uint32_t* __restrict__ order,
                                               const float4* __restrict__ sgrav, float4* __restrict__ acc_orig,
                                               const double* __restrict__ srho64, const double* __restrict__ spr64,
                                               const double* __restrict__ scs64, float av_alpha, float av_beta, unsigned long long* counters,
@@ -431,7 +430,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     // same order (so the sums are the walk's, bit for bit), or walk when the list did not fit.
     const uint32_t head = chead ? chead[gi] : kChunkOverflow;
     if (head == kChunkOverflow) {
-        neighbour_walk(stk, cand, n, K, L, nlr, nrange, expand, nB, nC, gx0, gy0, gz0, gx1, gy1, gz1, hg,
+        neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                        counters ? counters + 11 : nullptr, flush, sinst ? iroot[sinst[grp.x]] : 0u);
     } else {
         for (uint32_t id = head; id != kNoChunk;) {
@@ -767,12 +766,12 @@ static Boundary boundary(const Dev& d, const StepParams& p) {
 void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
     if (p.mixed)
-        hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR, d.nB,
-                           d.nC, d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
+        hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nS,
+                           d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
                            (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d));
     else
-        hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nLR,
-                           d.nB, d.nC, d.vals, d.srho, d.rho, (double*)nullptr, d.scal, d.counters, d.groups,
+        hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nS,
+                           d.vals, d.srho, d.rho, (double*)nullptr, d.scal, d.counters, d.groups,
                            d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d));
     launch_eos(d, s);
 }
@@ -785,13 +784,13 @@ void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
     if (p.mixed)
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
-                           d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
+                           d.n, d.K, d.L, d.nS, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
                            d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
                            (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext,
                            d.ccnt, d.cpool);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
-                           d.scs, d.n, d.K, d.L, d.nLR, d.nB, d.nC, d.vals, d.sgrav, d.acc,
+                           d.scs, d.n, d.K, d.L, d.nS, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
                            p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand,
                            (double*)nullptr, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext, d.ccnt, d.cpool);

@@ -12,6 +12,14 @@ namespace isph {
 constexpr uint32_t kUpLeaves = 512;
 constexpr uint32_t kUpCarry = 256;
 
+// SPH walk record of a node, 48 B (one cache line for most nodes): the box with the largest smoothing
+// length, then for an internal node its children and leaf range.
+struct SphNode {
+    float4 lo;  // bmin.xyz, max h
+    float4 hi;  // bmax.xyz, MAC radius size/theta + |centre - com|
+    uint4 lr;   // left child, right child, first leaf, last leaf
+};
+
 // All device arrays. "orig" arrays are in upload order; "sorted" arrays in Morton order of the last
 // evaluation (order[t] = upload index of sorted slot t).
 struct Dev {
@@ -77,10 +85,8 @@ struct Dev {
     uint32_t* gflag = nullptr;  // group marks at their first particle (all zero between builds)
     uint32_t* gscan = nullptr;  // per-64-particle mark counts / offsets
     uint4* nG = nullptr;   // gravity walk record, 2 x 16 B: (com.xyz, mass), (MAC radius bits, left, right, 0)
-    uint2* nLR = nullptr;  // children of internal nodes, for the SPH walks
-    float4* nA = nullptr;  // com.xyz, mass
-    float4* nB = nullptr;  // bmin.xyz, max h
-    float4* nC = nullptr;  // bmax.xyz, MAC radius size/theta + |centre - com|
+    SphNode* nS = nullptr;  // SPH walk record: box, largest h, children and leaf range (one load per walk step)
+    float4* nA = nullptr;   // com.xyz, mass
     float4* nQ0 = nullptr; // quadrupole xx, yy, zz, xy
     float2* nQ1 = nullptr; // quadrupole xz, yz
 
@@ -177,27 +183,42 @@ constexpr int kScalars = 2048;
 // share its L2. The wave that drains a range sets its bit in the area's mask, and later waves skip that
 // range with one load instead of a failing atomic. next() returns false once every range is drained, an
 // exit every wave reaches.
+// The first `statics` groups of a wave's own range are assigned without atomics (wave j of the W waves
+// dispatched to that range takes lo + j, lo + j + W, ...); the range's counter hands out the rest.
 struct WalkQueue {
     uint32_t* area;
     uint32_t total;
-    uint32_t q, tries;
-    __device__ WalkQueue(uint32_t* a, uint32_t total_groups) : area(a), total(total_groups) {
+    uint32_t q, tries, taken, statics;
+    __device__ WalkQueue(uint32_t* a, uint32_t total_groups, uint32_t static_takes = 0)
+        : area(a), total(total_groups), statics(static_takes) {
         q = kQueues == 1 ? 0u : (blockIdx.x & 7u);
         tries = 0;
+        taken = 0;
     }
     __device__ __forceinline__ bool next(uint32_t& gi) {
         uint32_t* mask = area + kQueueStride * kQueues;
         while (tries < kQueues) {
             const uint32_t lo = (uint32_t)(((uint64_t)q * total) / kQueues);
             const uint32_t hi = (uint32_t)(((uint64_t)(q + 1) * total) / kQueues);
+            const uint32_t waves = kQueues == 1 ? gridDim.x : (gridDim.x + 7u - q) / 8u;  // this range's waves
+            const uint32_t first = lo + statics * waves;  // first group the counter hands out
+            if (taken < statics) {
+                const uint32_t g = lo + (kQueues == 1 ? blockIdx.x : blockIdx.x / 8u) + taken * waves;
+                ++taken;
+                if (g < hi) {
+                    gi = g;
+                    return true;
+                }
+                taken = statics;
+            }
             uint32_t g = hi;
             if (__lane_id() == 0) {
                 bool drained = false;
                 if (ISPH_QUEUE_MASK && kQueues > 1)
                     drained = (__hip_atomic_load(mask, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> q) & 1u;
                 if (!drained) {
-                    g = lo + atomicAdd(&area[q * kQueueStride], 1u);
-                    if (ISPH_QUEUE_MASK && kQueues > 1 && g == hi) atomicOr(mask, 1u << q);
+                    g = first + atomicAdd(&area[q * kQueueStride], 1u);
+                    if (ISPH_QUEUE_MASK && kQueues > 1 && g >= hi) atomicOr(mask, 1u << q);
                 }
             }
             g = __builtin_amdgcn_readfirstlane(g);

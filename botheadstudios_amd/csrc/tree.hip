@@ -46,7 +46,7 @@ template <class Key>
 __global__ __launch_bounds__(256) void k_karras(const Key* __restrict__ codes, uint32_t L,
                                                 uint32_t* __restrict__ left, uint32_t* __restrict__ right,
                                                 uint32_t* __restrict__ parent, uint2* __restrict__ nrange,
-                                                uint2* __restrict__ nlr, uint8_t* __restrict__ nlevel) {
+                                                SphNode* __restrict__ nS, uint8_t* __restrict__ nlevel) {
     uint32_t ii = blockIdx.x * blockDim.x + threadIdx.x;
     if (ii == 0) parent[0] = kNoParent;
     if (ii + 1 >= L) return;  // L-1 internal nodes
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void k_karras(const Key* __restrict__ codes, u
     parent[li] = ii;
     parent[ri] = ii;
     nrange[ii] = make_uint2((uint32_t)first, (uint32_t)last);
-    nlr[ii] = make_uint2(li, ri);
+    nS[ii].lr = make_uint4(li, ri, (uint32_t)first, (uint32_t)last);
     uint32_t level = 1;
     for (uint64_t span = kLv1; (uint64_t)first / span != (uint64_t)last / span; span <<= kLvShift) ++level;
     nlevel[ii] = (uint8_t)level;
@@ -127,8 +127,7 @@ struct TreeView {
     const uint2* nrange;
     uint4* nG;
     float4* nA;
-    float4* nB;
-    float4* nC;
+    SphNode* nS;
     float4* nQ0;
     float2* nQ1;
 };
@@ -136,8 +135,8 @@ struct TreeView {
 __device__ __forceinline__ void store_node(const TreeView& tv, uint32_t i, const NodeMom& m, int quad, uint32_t l,
                                            uint32_t r) {
     tv.nA[i] = m.a;
-    tv.nB[i] = m.b;
-    tv.nC[i] = m.c;
+    tv.nS[i].lo = m.b;
+    tv.nS[i].hi = m.c;
     tv.nG[2 * i] = make_uint4(__float_as_uint(m.a.x), __float_as_uint(m.a.y), __float_as_uint(m.a.z),
                               __float_as_uint(m.a.w));
     tv.nG[2 * i + 1] = make_uint4(__float_as_uint(m.c.w), l, r, 0u);
@@ -150,8 +149,8 @@ __device__ __forceinline__ void store_node(const TreeView& tv, uint32_t i, const
 __device__ __forceinline__ NodeMom load_mom(const TreeView& tv, uint32_t i, bool quad) {
     NodeMom m;
     m.a = tv.nA[i];
-    m.b = tv.nB[i];
-    m.c = tv.nC[i];
+    m.b = tv.nS[i].lo;
+    m.c = tv.nS[i].hi;
     m.q0 = quad ? tv.nQ0[i] : make_float4(0, 0, 0, 0);
     m.q1 = quad ? tv.nQ1[i] : make_float2(0, 0);
     return m;
@@ -242,7 +241,7 @@ struct Climb {
 template <bool QUAD>
 __global__ __launch_bounds__(kLv1) void k_upsweep_lv1(const float4* __restrict__ spos, const float* __restrict__ sh,
                                                       uint32_t n, uint32_t K, uint32_t L, TreeView tv,
-                                                      const uint2* __restrict__ nlr,
+                                                      const SphNode* __restrict__ nS,
                                                       const uint8_t* __restrict__ nlevel, float inv_theta,
                                                       uint32_t* __restrict__ carry, uint32_t* __restrict__ ncarry) {
     ISPH_CLIMB_LDS(QUAD, kLv1);
@@ -253,7 +252,7 @@ __global__ __launch_bounds__(kLv1) void k_upsweep_lv1(const float4* __restrict__
     auto slot_of_parent = [&](uint32_t child) {
         const uint32_t p = tv.parent[child];
         if (p == kNoParent || nlevel[p] != 1) return kNoSlot;
-        return (p - c0) | (nlr[p].x == child ? 0u : 0x80000000u);
+        return (p - c0) | (nS[p].lr.x == child ? 0u : 0x80000000u);
     };
     // slot metadata of internal node c0 + t
     const uint32_t id = c0 + t;
@@ -261,7 +260,8 @@ __global__ __launch_bounds__(kLv1) void k_upsweep_lv1(const float4* __restrict__
     if (id + 1 < L) {
         if (nlevel[id] == 1) {
             sNode[t] = id;
-            sChild[t] = nlr[id];
+            const uint4 c4 = nS[id].lr;
+            sChild[t] = make_uint2(c4.x, c4.y);
             sPar[t] = slot_of_parent(id);
         } else {
             const uint32_t e = atomicAdd(&nup, 1u);
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(kLv1) void k_upsweep_lv1(const float4* __restrict__
 // carries the rest on, reads the children that a lower level completed from HBM and climbs.
 template <bool QUAD>
 __global__ __launch_bounds__(kSlots) void k_upsweep_lvk(uint32_t L, uint32_t level, uint32_t nbelow, TreeView tv,
-                                                        const uint2* __restrict__ nlr,
+                                                        const SphNode* __restrict__ nS,
                                                         const uint8_t* __restrict__ nlevel,
                                                         uint32_t* __restrict__ cslot,
                                                         const uint32_t* __restrict__ carry_in,
@@ -351,12 +351,13 @@ __global__ __launch_bounds__(kSlots) void k_upsweep_lvk(uint32_t L, uint32_t lev
     for (uint32_t s = t; s < ns; s += blockDim.x) {
         const uint32_t id = sNode[s];
         sCnt[s] = 0u;
-        sChild[s] = nlr[id];
+        const uint4 c4 = nS[id].lr;
+        sChild[s] = make_uint2(c4.x, c4.y);
         const uint32_t p = tv.parent[id];
         uint32_t ps = kNoSlot;
         if (p != kNoParent && nlevel[p] == level) {
             const uint32_t cs = cslot[p];
-            if (cs < ns) ps = cs | (nlr[p].x == id ? 0u : 0x80000000u);
+            if (cs < ns) ps = cs | (nS[p].lr.x == id ? 0u : 0x80000000u);
         }
         sPar[s] = ps;
     }
@@ -379,8 +380,7 @@ __global__ __launch_bounds__(kSlots) void k_upsweep_lvk(uint32_t L, uint32_t lev
 // Instance context of the group construction: the length unit of each node's limit comes from its
 // instance's root (the whole tree for one system), and no group may span two instances.
 struct GroupCtx {
-    const float4* nB;
-    const float4* nC;
+    const SphNode* nS;
     const uint32_t* sinst;  // sorted instance ids, or nullptr for one system
     const uint32_t* iroot;
     const uint32_t* ioff;
@@ -395,7 +395,7 @@ struct GroupCtx {
             root = iroot[k];
             cnt = ioff[k + 1] - ioff[k];
         }
-        const float4 b0 = nB[root], c0 = nC[root];
+        const float4 b0 = nS[root].lo, c0 = nS[root].hi;
         const float ex0 = c0.x - b0.x, ey0 = c0.y - b0.y, ez0 = c0.z - b0.z;
         const float spacing = sqrtf(ex0 * ex0 + ey0 * ey0 + ez0 * ez0) / cbrtf((float)cnt);
         const float lim = extent * (b0.w > 0.0f ? b0.w : spacing);
@@ -420,7 +420,7 @@ __device__ __forceinline__ bool group_ok(uint32_t v, uint32_t K, uint32_t L, con
     if (leaf) return true;
     if (count > maxc) return false;
     if (g.sinst && g.sinst[start] != g.sinst[end - 1]) return false;
-    float4 b = g.nB[v], c = g.nC[v];
+    float4 b = g.nS[v].lo, c = g.nS[v].hi;
     float ex = c.x - b.x, ey = c.y - b.y, ez = c.z - b.z;
     return ex * ex + ey * ey + ez * ez <= g.lim2(start);
 }
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(256) void k_group_emit(uint32_t n, uint32_t* __rest
 void launch_groups(const Dev& d, const StepParams& p, uint32_t maxc, uint2* out, uint32_t* total, hipStream_t s) {
     uint32_t nodes = 2 * d.L - 1;
     uint32_t nb = (d.n + kWave - 1) / kWave;
-    GroupCtx g{d.nB, d.nC, d.sinst(), d.iroot, d.ioff, d.n, p.group_extent};
+    GroupCtx g{d.nS, d.sinst(), d.iroot, d.ioff, d.n, p.group_extent};
     hipLaunchKernelGGL(k_group_mark, dim3((nodes + 255) / 256), dim3(256), 0, s, d.K, d.L, d.parent, d.nrange, g,
                        maxc, d.gflag);
     hipLaunchKernelGGL(k_group_count, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.gflag, d.gscan);
@@ -477,13 +477,13 @@ void launch_karras(const Dev& d, hipStream_t s) {
     uint32_t threads = d.L > 1 ? d.L - 1 : 1;
     if (d.ninst > 1)
         hipLaunchKernelGGL(k_karras<uint64_t>, dim3((threads + 255) / 256), dim3(256), 0, s, d.lkey, d.L, d.left,
-                           d.right, d.parent, d.nrange, d.nLR, d.nlevel);
+                           d.right, d.parent, d.nrange, d.nS, d.nlevel);
     else
         hipLaunchKernelGGL(k_karras<uint32_t>, dim3((threads + 255) / 256), dim3(256), 0, s, d.lcode, d.L, d.left,
-                           d.right, d.parent, d.nrange, d.nLR, d.nlevel);
+                           d.right, d.parent, d.nrange, d.nS, d.nlevel);
 }
 void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s) {
-    TreeView tv{d.left, d.right, d.parent, d.nrange, d.nG, d.nA, d.nB, d.nC, d.nQ0, d.nQ1};
+    TreeView tv{d.left, d.right, d.parent, d.nrange, d.nG, d.nA, d.nS, d.nQ0, d.nQ1};
     const float inv_theta = 1.0f / p.theta;
     // carried ids: level-1 workgroups write list 0, level k reads list (k % 2) ^ 1 ... alternating halves
     uint32_t* lists[2] = {d.carry, d.carry + (size_t)d.carry_blocks * kCarry};
@@ -491,12 +491,12 @@ void launch_upsweep(const Dev& d, const StepParams& p, hipStream_t s) {
     auto go = [&](auto quad) {
         constexpr bool Q = decltype(quad)::value;
         uint32_t nbelow = (d.L + kLv1 - 1) / kLv1;
-        hipLaunchKernelGGL(k_upsweep_lv1<Q>, dim3(nbelow), dim3(kLv1), 0, s, d.spos, d.sh, d.n, d.K, d.L, tv, d.nLR,
+        hipLaunchKernelGGL(k_upsweep_lv1<Q>, dim3(nbelow), dim3(kLv1), 0, s, d.spos, d.sh, d.n, d.K, d.L, tv, d.nS,
                            d.nlevel, inv_theta, lists[0], counts[0]);
         uint32_t level = 2, in = 0;
         for (uint64_t span = (uint64_t)kLv1 << kLvShift; (span >> kLvShift) < d.L; span <<= kLvShift, ++level) {
             const uint32_t blocks = (uint32_t)((d.L + span - 1) / span);
-            hipLaunchKernelGGL(k_upsweep_lvk<Q>, dim3(blocks), dim3(kSlots), 0, s, d.L, level, nbelow, tv, d.nLR,
+            hipLaunchKernelGGL(k_upsweep_lvk<Q>, dim3(blocks), dim3(kSlots), 0, s, d.L, level, nbelow, tv, d.nS,
                                d.nlevel, d.cslot, lists[in], counts[in], lists[in ^ 1], counts[in ^ 1], inv_theta);
             nbelow = blocks;
             in ^= 1;

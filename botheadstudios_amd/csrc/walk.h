@@ -17,9 +17,7 @@ constexpr int kSphBuf = 128;               // candidate list (two flushes of 64)
 // evaluate them for the calling lane.
 template <class Flush>
 __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, uint32_t n, uint32_t K, uint32_t L,
-                                               const uint2* __restrict__ nlr, const uint2* __restrict__ nrange,
-                                               uint32_t expand,
-                                               const float4* __restrict__ nB, const float4* __restrict__ nC,
+                                               const SphNode* __restrict__ nS, uint32_t expand,
                                                float gx0, float gy0, float gz0, float gx1, float gy1, float gz1,
                                                float hgmax, unsigned long long* stats, Flush&& flush,
                                                uint32_t root = 0) {
@@ -46,15 +44,11 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
         const uint32_t node = has ? stk[sp + lane] : 0u;
         __syncthreads();
         bool keep = false;
-        uint32_t lc = 0, rc = 0;
-        if (has) {
-            const float4 B = nB[node];
-            const float4 C = nC[node];
-            if (node < leaf0) {  // child indices travel with the box: one memory round trip per round
-                const uint2 ch = nlr[node];
-                lc = ch.x;
-                rc = ch.y;
-            }
+        uint4 ch = make_uint4(0u, 0u, 0u, 0u);  // children and leaf range travel with the box: one memory
+        if (has) {                               // round trip per round
+            const float4 B = nS[node].lo;
+            const float4 C = nS[node].hi;
+            if (node < leaf0) ch = nS[node].lr;
             float dx = fmaxf(fmaxf(B.x - gx1, gx0 - C.x), 0.f);
             float dy = fmaxf(fmaxf(B.y - gy1, gy0 - C.y), 0.f);
             float dz = fmaxf(fmaxf(B.z - gz1, gz0 - C.z), 0.f);
@@ -69,8 +63,7 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
                 first = (node - leaf0) * K;
                 cnt = min(first + K, n) - first;
             } else {
-                const uint2 r = nrange[node];
-                const uint32_t pf = r.x * K, pe = min((r.y + 1) * K, n);
+                const uint32_t pf = ch.z * K, pe = min((ch.w + 1) * K, n);
                 if (pe - pf <= expand) {
                     first = pf;
                     cnt = pe - pf;
@@ -82,8 +75,8 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
         const uint64_t mo = __ballot(open);
         if (open) {
             int pos = sp + 2 * prefix_count(mo);
-            stk[pos] = lc;
-            stk[pos + 1] = rc;
+            stk[pos] = ch.x;
+            stk[pos + 1] = ch.y;
         }
         sp += 2 * __popcll(mo);
         max_sp = max(max_sp, sp);
