@@ -315,11 +315,12 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
         tot_part += npart * grp.y;
     }
     if (lane == 0 && counters) {
-        atomicAdd(&counters[QUAD ? 2 : 0], tot_cell);
-        atomicAdd(&counters[1], tot_part);
-        atomicAdd(&counters[5], rounds);
-        atomicAdd(&counters[6], popped);
-        atomicMax(&counters[7], (unsigned long long)max_sp);
+        unsigned long long* c = counters + (size_t)blockIdx.x * kCounterSlots;  // this wave's slots
+        c[QUAD ? 2 : 0] += tot_cell;
+        c[1] += tot_part;
+        c[5] += rounds;
+        c[6] += popped;
+        c[7] = max(c[7], (unsigned long long)max_sp);
     }
 }
 

@@ -194,7 +194,7 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.scal, kScalars);
     dalloc(&d.dt, 1);
     dalloc(&d.time, 1);
-    dalloc(&d.counters, 16);
+    dalloc(&d.counters, (size_t)kCounterSlots * kWalkWaves);
     dalloc(&d.red, 1024 * 16);
     dalloc(&d.eos, kMaxMaterials);
     dalloc(&d.ext, kMaxExtBodies);
@@ -202,7 +202,7 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     HIP_OK(hipMalloc(&d.aos, (size_t)cap * sizeof(isph_particle)));
     HIP_OK(hipHostMalloc((void**)&c->pinned, (size_t)cap * sizeof(isph_particle), hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&c->rb_event, hipEventDisableTiming));
-    HIP_OK(hipMemset(d.counters, 0, 16 * sizeof(unsigned long long)));
+    HIP_OK(hipMemset(d.counters, 0, (size_t)kCounterSlots * kWalkWaves * sizeof(unsigned long long)));
     HIP_OK(hipMemset(d.time, 0, sizeof(double)));
     HIP_OK(hipMemset(d.dt, 0, sizeof(float)));
     HIP_OK(hipMemset(d.eos, 0, kMaxMaterials * sizeof(Till)));
@@ -1496,13 +1496,26 @@ int isph_phase_times(isph_t* h, double* ms, size_t n) {
     });
 }
 
+// Totals of the per-wave statistics slots (sums; slots 7, 10 and 13 hold maxima).
+static void read_counters(isph_t* h, unsigned long long* cnt) {
+    std::vector<unsigned long long> all((size_t)kCounterSlots * kWalkWaves);
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipMemcpy(all.data(), h->d.counters, all.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < kCounterSlots; ++k) cnt[k] = 0;
+    for (uint32_t w = 0; w < kWalkWaves; ++w)
+        for (uint32_t k = 0; k < kCounterSlots; ++k) {
+            const unsigned long long v = all[(size_t)w * kCounterSlots + k];
+            cnt[k] = (k == 7 || k == 10 || k == 13) ? std::max(cnt[k], v) : cnt[k] + v;
+        }
+}
+
 int isph_kernel_stats(isph_t* h, double* out7) {
     // out7 holds 8 entries (see isph.h)
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));
         flush_timing(h);
-        unsigned long long cnt[16];
-        HIP_OK(hipMemcpy(cnt, h->d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+        unsigned long long cnt[kCounterSlots];
+        read_counters(h, cnt);
         out7[0] = h->grav_launches;
         out7[1] = h->phase_ms[ISPH_PHASE_GRAVITY];
         out7[2] = (double)cnt[0];
@@ -1519,8 +1532,8 @@ int isph_walk_stats(isph_t* h, double* out9) {
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));
         HIP_OK(hipStreamSynchronize(h->stream));
-        unsigned long long cnt[16];
-        HIP_OK(hipMemcpy(cnt, h->d.counters, sizeof(cnt), hipMemcpyDeviceToHost));
+        unsigned long long cnt[kCounterSlots];
+        read_counters(h, cnt);
         for (int k = 0; k < 9; ++k) out9[k] = (double)cnt[5 + k];
         return ISPH_OK;
     });
@@ -1532,7 +1545,8 @@ int isph_reset_stats(isph_t* h) {
         flush_timing(h);
         for (double& x : h->phase_ms) x = 0.0;
         h->grav_launches = 0;
-        HIP_OK(hipMemsetAsync(h->d.counters, 0, 16 * sizeof(unsigned long long), h->stream));
+        HIP_OK(hipMemsetAsync(h->d.counters, 0, (size_t)kCounterSlots * kWalkWaves * sizeof(unsigned long long),
+                              h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         return ISPH_OK;
     });

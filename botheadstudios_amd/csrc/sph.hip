@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     unsigned long long tested = 0;
     cw.begin();
     neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
-                   counters ? counters + 8 : nullptr, [&](int cnt) {
+                   counters ? counters + (size_t)blockIdx.x * kCounterSlots + 8 : nullptr, [&](int cnt) {
         cw.put(cc, scal, cand, cnt, lane);
         if (lane < cnt) {
             uint32_t j = cand[lane];
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     }
     tested_all += tested * grp.y;
     }
-    if (lane == 0 && counters) atomicAdd(&counters[3], tested_all);
+    if (lane == 0 && counters) counters[(size_t)blockIdx.x * kCounterSlots + 3] += tested_all;
 }
 
 // Per-particle EOS and the Courant signal minimum, after the density walk (or on the reused density when
@@ -431,7 +431,8 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     const uint32_t head = chead ? chead[gi] : kChunkOverflow;
     if (head == kChunkOverflow) {
         neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
-                       counters ? counters + 11 : nullptr, flush, sinst ? iroot[sinst[grp.x]] : 0u);
+                       counters ? counters + (size_t)blockIdx.x * kCounterSlots + 11 : nullptr, flush,
+                       sinst ? iroot[sinst[grp.x]] : 0u);
     } else {
         for (uint32_t id = head; id != kNoChunk;) {
             const uint32_t cnt = ccnt[id], nx = cnext[id];
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     }
     tested_all += tested * grp.y;
     }
-    if (lane == 0 && counters) atomicAdd(&counters[4], tested_all);
+    if (lane == 0 && counters) counters[(size_t)blockIdx.x * kCounterSlots + 4] += tested_all;
 }
 
 // ---- KDK (sph_step.wgsl:304-333; hydrostatic.rs:451-465), retired particles are not integrated ----------

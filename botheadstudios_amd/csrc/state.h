@@ -155,6 +155,9 @@ struct Boundary {
 
 constexpr uint32_t kGroupMax = 64;    // particles per SPH walk group (one per lane)
 constexpr uint32_t kWalkWaves = 8192; // resident waves of the work-queue walks
+// Statistics counters: kCounterSlots per walk wave (one-wave workgroup w owns counters[w * kCounterSlots ..]),
+// so the walks update them without atomics; the host sums them when asked.
+constexpr uint32_t kCounterSlots = 16;
 #ifndef ISPH_QUEUES
 #define ISPH_QUEUES 8
 #endif

@@ -105,9 +105,9 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
         __syncthreads();
     }
     if (lane == 0 && stats) {
-        atomicAdd(&stats[0], rounds);
-        atomicAdd(&stats[1], popped);
-        atomicMax(&stats[2], (unsigned long long)max_sp);
+        stats[0] += rounds;  // the calling wave's own slots
+        stats[1] += popped;
+        stats[2] = max(stats[2], (unsigned long long)max_sp);
     }
 }
 
