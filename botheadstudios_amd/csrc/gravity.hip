@@ -49,7 +49,7 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
     const uint32_t leaf0 = L - 1;
     unsigned long long tot_cell = 0, tot_part = 0, rounds = 0, popped = 0;
     int max_sp = 1;
-    WalkQueue queue(scal + kQueueGrav, scal[count_slot], 2);
+    WalkQueue queue(scal + kQueueGrav, scal[count_slot], 1);
     uint32_t gi;
     while (queue.next(gi)) {
         const uint2 grp = groups[gi];

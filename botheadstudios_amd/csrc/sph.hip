@@ -149,7 +149,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     own[lane] = kWave;
     unsigned long long tested_all = 0;
     CacheWriter cw;
-    WalkQueue queue(scal + kQueueDensity, scal[kGroupsSph], 2);
+    WalkQueue queue(scal + kQueueDensity, scal[kGroupsSph], 1);
     uint32_t gi;
     while (queue.next(gi)) {
     const uint2 grp = groups[gi];
