@@ -98,23 +98,30 @@ struct CacheWriter {
     }
 };
 
-// The pair test of one lane against the candidates in LDS (x, y, z, h per entry): bit jj is set when
+// The pair test of one lane against the candidates in LDS (x, y, z, h/2 per entry): bit jj is set when
 // r < (h_i + h_j) / 2 (sph_step.wgsl:201,210 / :262-272); `not_coincident` adds the force loop's r > 1e-9.
 // `margin` widens the radius test (the mixed form's f32 pre-filter keeps a superset of its exact f64 test).
-// The lane's own particle is removed afterwards with `self_slot`.
-__device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 me, float hi, bool not_coincident,
+// The lane's own particle is removed afterwards with `self_slot`. Halving is exact, so h_i/2 + h_j/2 is
+// bit for bit the reference's (h_i + h_j) * 0.5. Each half of the mask is built from its highest slot down,
+// one shift-and-insert per candidate.
+__device__ __forceinline__ bool pair_hit(float4 pj, float4 me, float qi, bool not_coincident, float margin) {
+    const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float hij = qi + pj.w;
+    bool ok = r2 < margin * (hij * hij);
+    if (not_coincident) ok = ok && r2 > 1.0e-18f;
+    return ok;
+}
+__device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 me, float qi, bool not_coincident,
                                               float margin = 1.0f) {
-    uint64_t hit = 0ull;
-    for (int jj = 0; jj < cnt; ++jj) {
-        const float4 pj = cP[jj];
-        const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const float hij = 0.5f * (hi + pj.w);
-        bool ok = r2 < margin * (hij * hij);
-        if (not_coincident) ok = ok && r2 > 1.0e-18f;
-        hit |= ok ? (1ull << jj) : 0ull;
-    }
-    return hit;
+    uint32_t hi = 0u, lo = 0u;
+#pragma unroll 4
+    for (int jj = min(cnt, kWave) - 1; jj >= 32; --jj)
+        hi = (hi << 1) | (pair_hit(cP[jj], me, qi, not_coincident, margin) ? 1u : 0u);
+#pragma unroll 4
+    for (int jj = min(cnt, 32) - 1; jj >= 0; --jj)
+        lo = (lo << 1) | (pair_hit(cP[jj], me, qi, not_coincident, margin) ? 1u : 0u);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // While loading candidate `cand[lane]`, record where a particle of the lane's own group sits in this batch
@@ -159,6 +166,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     __syncthreads();
     const float4 me = spos[tc];
     const float hi = sh[tc];
+    const float qi = 0.5f * hi;  // half smoothing length, as the candidates carry it
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
     const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
     const float hg = wave_max(hi);
@@ -176,14 +184,14 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
         if (lane < cnt) {
             uint32_t j = cand[lane];
             const float4 p = spos[j];
-            cP[lane] = make_float4(p.x, p.y, p.z, sh[j]);
+            cP[lane] = make_float4(p.x, p.y, p.z, 0.5f * sh[j]);
             cM[lane] = p.w;
             note_owner(own, j, grp.x, grp.y, lane);
         }
         __syncthreads();
         // 1) The radius test against every candidate (LDS broadcast reads): one bit per true pair, the
         //    lane's own particle excluded (sph_step.wgsl:210).
-        const uint64_t hit = pair_mask(cP, cnt, me, hi, false) & ~self_bit(own, cand, cnt, lane, t);
+        const uint64_t hit = pair_mask(cP, cnt, me, qi, false) & ~self_bit(own, cand, cnt, lane, t);
         // 2) The kernel sum over this lane's hits only, in candidate order. Lanes walk their own hit lists
         //    in step, so the wave runs as many iterations as its busiest lane instead of one per candidate
         //    with most lanes idle.
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
                 if (F64) {
                     double ddx = (double)me.x - (double)pj.x, ddy = (double)me.y - (double)pj.y,
                            ddz = (double)me.z - (double)pj.z;
-                    double inv_h = 1.0 / (0.5 * ((double)hi + (double)pj.w));
+                    double inv_h = 1.0 / ((double)qi + (double)pj.w);
                     double q = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) * inv_h;
                     double sig = (8.0 / 3.14159265358979323846) * (inv_h * inv_h * inv_h);
                     double w = q < 0.5 ? sig * (6.0 * (q * q * q - q * q) + 1.0)
@@ -207,7 +215,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
                 } else {
                     float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
                     float r2 = dx * dx + dy * dy + dz * dz;
-                    float hij = 0.5f * (hi + pj.w);
+                    float hij = qi + pj.w;
                     float r = hw_sqrt(r2);
                     float inv_h = hw_rcp(hij);
                     acc += mj * spline_w(r * inv_h, inv_h);
@@ -327,6 +335,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     __syncthreads();
     const float4 me = spos[tc];
     const float hi = sh[tc];
+    const float qi = 0.5f * hi;  // half smoothing length, as the candidates carry it
     const float4 vi = svel[tc];
     const float rhoi = srho[tc], pri = spr[tc], ci = scs[tc];
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
@@ -344,7 +353,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
         if (lane < cnt) {
             uint32_t j = cand[lane];
             const float4 p = spos[j];
-            cP[lane] = make_float4(p.x, p.y, p.z, sh[j]);
+            cP[lane] = make_float4(p.x, p.y, p.z, 0.5f * sh[j]);
             float4 v = svel[j];
             cV[lane] = make_float4(v.x, v.y, v.z, srho[j]);
             cS[lane] = make_float4(p.w, spr[j], scs[j], 0.f);
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
         // 1) The radius test against every candidate, one bit per pair (plain: the exact test of
         //    sph_step.wgsl:262-272; mixed: an f32 pre-filter with a 1e-4 margin, a superset of the exact f64
         //    test below). 2) The pair terms over this lane's hits only, in candidate order.
-        uint64_t rest = (F64 ? pair_mask(cP, cnt, me, hi, false, 1.0001f) : pair_mask(cP, cnt, me, hi, true)) &
+        uint64_t rest = (F64 ? pair_mask(cP, cnt, me, qi, false, 1.0001f) : pair_mask(cP, cnt, me, qi, true)) &
                         ~self_bit(own, cand, cnt, lane, t);
         while (__ballot(rest != 0ull)) {
             if (!rest) continue;
@@ -371,7 +380,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                 double dx = (double)me.x - (double)pj.x, dy = (double)me.y - (double)pj.y,
                        dz = (double)me.z - (double)pj.z;
                 double r2 = dx * dx + dy * dy + dz * dz;
-                double hij = 0.5 * ((double)hi + (double)pj.w);
+                double hij = (double)qi + (double)pj.w;
                 if (r2 < hij * hij && r2 > 1.0e-18) {
                     float4 vj = cV[jj];
                     double inv_r = 1.0 / sqrt(r2);
@@ -401,7 +410,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             } else {
                 float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;  // dpos = pos_i - pos_j
                 float r2 = dx * dx + dy * dy + dz * dz;
-                float hij = 0.5f * (hi + pj.w);
+                float hij = qi + pj.w;
                 float4 vj = cV[jj];
                 float r = hw_sqrt(r2);
                 float inv_h = hw_rcp(hij);
