@@ -139,7 +139,7 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     Dev& d = c->d;
     d.cap = cap;
     size_t nodes = 2 * (size_t)cap;
-    uint32_t sort_blocks = (cap + 4095) / 4096;
+    uint32_t sort_blocks = (cap + kSortTile - 1) / kSortTile;
     dalloc(&d.posh, cap);
     dalloc(&d.velu, cap);
     dalloc(&d.mass, cap);

@@ -8,8 +8,7 @@
 namespace isph {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // keys per block per pass
+constexpr int kSortItems = kSortTile / kSortThreads;  // keys per thread per pass
 
 // mode 0: everything before a tree build; mode 1: only what a force evaluation on the existing tree uses
 // (Courant minimum, force-walk queue heads).

@@ -10,6 +10,10 @@ namespace isph {
 // Upsweep chunking: level-1 chunks of kUpLeaves leaves; each chunk carries at most kUpCarry ids of
 // higher-level nodes to the next level.
 constexpr uint32_t kUpLeaves = 512;
+#ifndef ISPH_SORT_TILE
+#define ISPH_SORT_TILE 4096
+#endif
+constexpr int kSortTile = ISPH_SORT_TILE;  // radix sort: keys per workgroup per pass (256 threads)
 constexpr uint32_t kUpCarry = 256;
 
 // SPH walk record of a node, 48 B (one cache line for most nodes): the box with the largest smoothing
