@@ -190,6 +190,25 @@ __device__ __forceinline__ void till_p_c2(const Till& t, double rho_in, double u
 // library forms expand into refinement sequences several times longer.
 __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// f64 reciprocal and reciprocal square root: the hardware estimate refined by two Newton-Raphson steps
+// (each roughly doubles the correct bits of the ~2^-22 estimate), for the mixed form's pair terms. For
+// finite positive x the result is within a few ulp of the correctly rounded value; a compiled f64
+// division or sqrt costs about twice as many instructions.
+__device__ __forceinline__ double nr_rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+__device__ __forceinline__ double nr_rsq(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    double t = fma(-h * y, y, 0.5);
+    y = fma(y, t, y);
+    t = fma(-h * y, y, 0.5);
+    return fma(y, t, y);
+}
 
 __device__ __forceinline__ float spline_w(float q, float inv_h) {
     float sig = (8.0f / kPi) * inv_h * inv_h * inv_h;

@@ -206,8 +206,9 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
                 if (F64) {
                     double ddx = (double)me.x - (double)pj.x, ddy = (double)me.y - (double)pj.y,
                            ddz = (double)me.z - (double)pj.z;
-                    double inv_h = 1.0 / ((double)qi + (double)pj.w);
-                    double q = sqrt(ddx * ddx + ddy * ddy + ddz * ddz) * inv_h;
+                    double inv_h = nr_rcp((double)qi + (double)pj.w);
+                    const double rr = ddx * ddx + ddy * ddy + ddz * ddz;
+                    double q = (rr > 0.0 ? rr * nr_rsq(rr) : 0.0) * inv_h;
                     double sig = (8.0 / 3.14159265358979323846) * (inv_h * inv_h * inv_h);
                     double w = q < 0.5 ? sig * (6.0 * (q * q * q - q * q) + 1.0)
                                        : (q < 1.0 ? sig * 2.0 * (1.0 - q) * (1.0 - q) * (1.0 - q) : 0.0);
@@ -383,17 +384,18 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                 double hij = (double)qi + (double)pj.w;
                 if (r2 < hij * hij && r2 > 1.0e-18) {
                     float4 vj = cV[jj];
-                    double inv_r = 1.0 / sqrt(r2);
-                    double inv_h = 1.0 / hij;
+                    double inv_r = nr_rsq(r2);
+                    double inv_h = nr_rcp(hij);
                     double q = r2 * inv_r * inv_h;
                     double dvx = (double)vi.x - (double)vj.x, dvy = (double)vi.y - (double)vj.y,
                            dvz = (double)vi.z - (double)vj.z;
                     double vr = dvx * dx + dvy * dy + dvz * dz;
                     double pij = 0.0;
                     if (vr < 0.0) {
-                        double mu = hij * vr / (r2 + 0.01 * hij * hij);
+                        double mu = hij * vr * nr_rcp(r2 + 0.01 * hij * hij);
                         double cbar = 0.5 * (ci64 + cC64[jj]);
-                        pij = 2.0 * (-(double)av_alpha * cbar * mu + (double)av_beta * mu * mu) / (rhoi64 + cR64[jj]);
+                        pij = 2.0 * (-(double)av_alpha * cbar * mu + (double)av_beta * mu * mu) *
+                              nr_rcp(rhoi64 + cR64[jj]);
                     }
                     double coeff = pri64 + cPr64[jj] + pij;
                     double ih2 = inv_h * inv_h;
