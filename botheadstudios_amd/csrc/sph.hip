@@ -392,22 +392,24 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                     double vr = dvx * dx + dvy * dy + dvz * dz;
                     double pij = 0.0;
                     if (vr < 0.0) {
-                        double mu = hij * vr * nr_rcp(r2 + 0.01 * hij * hij);
-                        double cbar = 0.5 * (ci64 + cC64[jj]);
-                        pij = 2.0 * (-(double)av_alpha * cbar * mu + (double)av_beta * mu * mu) *
-                              nr_rcp(rhoi64 + cR64[jj]);
+                        // Monaghan viscosity with mu = hij vr / D, D = r^2 + 0.01 hij^2, and one reciprocal
+                        // for both divisions: pij = 2 hij vr (beta hij vr - alpha cbar D) / (D^2 (rho_i + rho_j))
+                        const double D = r2 + 0.01 * hij * hij;
+                        const double cbar = 0.5 * (ci64 + cC64[jj]);
+                        const double hv = hij * vr;
+                        pij = 2.0 * hv * ((double)av_beta * hv - (double)av_alpha * cbar * D) *
+                              nr_rcp(D * D * (rhoi64 + cR64[jj]));
                     }
                     double coeff = pri64 + cPr64[jj] + pij;
                     double ih2 = inv_h * inv_h;
                     double sig = (8.0 / 3.14159265358979323846) * (ih2 * ih2);
                     double dw = q < 0.5 ? sig * (18.0 * q * q - 12.0 * q) : sig * -6.0 * (1.0 - q) * (1.0 - q);
-                    double gs = dw * inv_r;
-                    double gx = dx * gs, gy = dy * gs, gz = dz * gs;
-                    double mj = (double)sj.x;
-                    ax64 -= gx * (coeff * mj);
-                    ay64 -= gy * (coeff * mj);
-                    az64 -= gz * (coeff * mj);
-                    de64 += 0.5 * mj * coeff * (dvx * gx + dvy * gy + dvz * gz);
+                    // grad W = dpos dw / r, so dv . grad W = vr dw / r: one scalar weight per pair
+                    const double f = coeff * (double)sj.x * (dw * inv_r);
+                    ax64 = fma(-dx, f, ax64);
+                    ay64 = fma(-dy, f, ay64);
+                    az64 = fma(-dz, f, az64);
+                    de64 = fma(0.5 * f, vr, de64);
                 }
             } else {
                 float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;  // dpos = pos_i - pos_j
