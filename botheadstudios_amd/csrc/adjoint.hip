@@ -532,11 +532,13 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
                                                   double* srb, double* sqb, double* scb) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
-    __shared__ float4 cP[kWave];   // x, y, z, m
-    __shared__ float4 cV[kWave];   // vx, vy, vz, h
+    __shared__ float4 cP[kWave];   // x, y, z, h/2
+    __shared__ float4 cV[kWave];   // vx, vy, vz, m
     __shared__ double cR[kWave], cQ[kWave], cC[kWave];
     __shared__ double4 cAb[kWave];
+    __shared__ uint32_t own[kWave];
     const int lane = threadIdx.x;
+    own[lane] = kWave;
     const double alpha = av_alpha, beta = av_beta;
     const uint32_t ngroups = scal[kGroupsSph];
     for (uint32_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
@@ -547,6 +549,7 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
         __syncthreads();
         const float4 me = spos[tc];
         const float hi = sh[tc];
+        const float qh = 0.5f * hi;  // half smoothing length, as the candidates carry it
         const float4 vi = svel[tc];
         const double rhoi = srho64[tc], qi = spr64[tc], ci = scs64[tc];
         const double4 abi = valid ? sa[tc] : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -558,31 +561,36 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
                        [&](int cnt) {
             if (lane < cnt) {
                 uint32_t j = cand[lane];
-                cP[lane] = spos[j];
-                float4 v = svel[j];
-                cV[lane] = make_float4(v.x, v.y, v.z, sh[j]);
+                const float4 p = spos[j];
+                cP[lane] = make_float4(p.x, p.y, p.z, 0.5f * sh[j]);
+                const float4 v = svel[j];
+                cV[lane] = make_float4(v.x, v.y, v.z, p.w);
                 cR[lane] = srho64[j];
                 cQ[lane] = spr64[j];
                 cC[lane] = scs64[j];
                 cAb[lane] = sa[j];
+                note_owner(own, j, grp.x, grp.y, lane);
             }
             __syncthreads();
-            for (int jj = 0; jj < cnt; ++jj) {
+            // the forward's pair set: its f32 pre-filter (1e-4 margin), then its exact f64 test per hit
+            uint64_t rest = valid ? pair_mask(cP, cnt, me, qh, false, 1.0001f) & ~self_bit(own, cand, cnt, lane, t)
+                                  : 0ull;
+            while (__ballot(rest != 0ull)) {
+                if (!rest) continue;
+                const int jj = __builtin_ctzll(rest);
+                rest &= rest - 1ull;
                 const float4 pj = cP[jj];
-                const float4 vj = cV[jj];
-                float fx = me.x - pj.x, fy = me.y - pj.y, fz = me.z - pj.z;
-                float fh = 0.5f * (hi + vj.w);
-                if (!(fx * fx + fy * fy + fz * fz < 1.0001f * fh * fh) || cand[jj] == t || !valid) continue;
                 const double d[3] = {(double)me.x - (double)pj.x, (double)me.y - (double)pj.y,
                                      (double)me.z - (double)pj.z};
                 const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                const double hij = 0.5 * ((double)hi + (double)vj.w);
+                const double hij = (double)qh + (double)pj.w;
                 if (!(r2 < hij * hij && r2 > 1.0e-18)) continue;
+                const float4 vj = cV[jj];
                 const double v[3] = {(double)vi.x - (double)vj.x, (double)vi.y - (double)vj.y,
                                      (double)vi.z - (double)vj.z};
-                const double inv_r = 1.0 / sqrt(r2);
+                const double inv_r = nr_rsq(r2);
                 const double r = r2 * inv_r;
-                const double inv_h = 1.0 / hij;
+                const double inv_h = nr_rcp(hij);
                 const double q = r * inv_h;
                 const double ih2 = inv_h * inv_h;
                 const double sig4 = (8.0 / 3.14159265358979323846) * (ih2 * ih2);
@@ -591,15 +599,15 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
                 const double dF = (spline_d2w(q, inv_h, sig4) - F) * inv_r;  // dF/dr
                 const double vr = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
                 const double rhoj = cR[jj], qj = cQ[jj], cj = cC[jj];
-                const double mj = pj.w, mi = me.w;
-                double Pi = 0.0, dPi_dmu = 0.0, mu = 0.0, den = 0.0;
-                const double rsum = rhoi + rhoj;
+                const double mj = vj.w, mi = me.w;
+                double Pi = 0.0, dPi_dmu = 0.0, mu = 0.0, rden = 0.0, rrs = 0.0;
                 if (vr < 0.0) {
-                    den = r2 + 0.01 * hij * hij;
-                    mu = hij * vr / den;
+                    rden = nr_rcp(r2 + 0.01 * hij * hij);
+                    rrs = nr_rcp(rhoi + rhoj);
+                    mu = hij * vr * rden;
                     const double cbar = 0.5 * (ci + cj);
-                    Pi = 2.0 * (-alpha * cbar * mu + beta * mu * mu) / rsum;
-                    dPi_dmu = 2.0 * (-alpha * cbar + 2.0 * beta * mu) / rsum;
+                    Pi = 2.0 * (-alpha * cbar * mu + beta * mu * mu) * rrs;
+                    dPi_dmu = 2.0 * (-alpha * cbar + 2.0 * beta * mu) * rrs;
                 }
                 const double C = qi + qj + Pi;
                 const double4 abj = cAb[jj];
@@ -610,20 +618,22 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
                 const double S = F * wvd;  // d loss / d C
                 // position: C (F' (d/r) (wv.d) + F wv) + S dPi/dd
                 double gd[3], gv[3];
+                const double cdf = C * dF * inv_r * wvd, cf = C * F, cfd = cf * D;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    gd[k] = C * (dF * d[k] * inv_r * wvd + F * wv[k]);
-                    gv[k] = C * F * D * d[k];
+                    gd[k] = cdf * d[k] + cf * wv[k];
+                    gv[k] = cfd * d[k];
                 }
                 if (vr < 0.0) {
-                    const double dmu_dvr = hij / den, dmu_dr2 = -mu / den;
+                    const double sdp = S * dPi_dmu;
+                    const double dmu_dvr = hij * rden, dmu_dr2 = -mu * rden;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        gd[k] += S * dPi_dmu * (dmu_dvr * v[k] + dmu_dr2 * 2.0 * d[k]);
-                        gv[k] += S * dPi_dmu * dmu_dvr * d[k];
+                        gd[k] += sdp * (dmu_dvr * v[k] + dmu_dr2 * 2.0 * d[k]);
+                        gv[k] += sdp * dmu_dvr * d[k];
                     }
-                    cb += S * (-alpha * mu / rsum);
-                    rb += S * (-Pi / rsum);
+                    cb += S * (-alpha * mu * rrs);
+                    rb += S * (-Pi * rrs);
                 }
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -631,8 +641,8 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
                     vbar[k] += gv[k];
                 }
                 qb += S;
-                mb += C * F * ((abj.x + 0.5 * abj.w * v[0]) * d[0] + (abj.y + 0.5 * abj.w * v[1]) * d[1] +
-                               (abj.z + 0.5 * abj.w * v[2]) * d[2]);
+                mb += cf * ((abj.x + 0.5 * abj.w * v[0]) * d[0] + (abj.y + 0.5 * abj.w * v[1]) * d[1] +
+                            (abj.z + 0.5 * abj.w * v[2]) * d[2]);
             }
         });
         if (valid) {
@@ -713,10 +723,12 @@ __global__ __launch_bounds__(64) void k_vjp_density(const float4* __restrict__ s
                                                     const uint32_t* __restrict__ scal, double4* sxb) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
-    __shared__ float4 cP[kWave];
-    __shared__ float cH[kWave];
+    __shared__ float4 cP[kWave];  // x, y, z, h/2
+    __shared__ float cM[kWave];
     __shared__ double cRb[kWave];
+    __shared__ uint32_t own[kWave];
     const int lane = threadIdx.x;
+    own[lane] = kWave;
     const uint32_t ngroups = scal[kGroupsSph];
     for (uint32_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
         const uint2 grp = groups[gi];
@@ -726,6 +738,7 @@ __global__ __launch_bounds__(64) void k_vjp_density(const float4* __restrict__ s
         __syncthreads();
         const float4 me = spos[tc];
         const float hi = sh[tc];
+        const float qh = 0.5f * hi;  // half smoothing length, as the candidates carry it
         const bool live = valid && me.w > 0.f;
         const double rbi = live ? srb[tc] : 0.0;
         const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
@@ -736,22 +749,27 @@ __global__ __launch_bounds__(64) void k_vjp_density(const float4* __restrict__ s
                        [&](int cnt) {
             if (lane < cnt) {
                 uint32_t j = cand[lane];
-                float4 pj = spos[j];
-                cP[lane] = pj;
-                cH[lane] = sh[j];
+                const float4 pj = spos[j];
+                cP[lane] = make_float4(pj.x, pj.y, pj.z, 0.5f * sh[j]);
+                cM[lane] = pj.w;
                 cRb[lane] = pj.w > 0.f ? srb[j] : 0.0;
+                note_owner(own, j, grp.x, grp.y, lane);
             }
             __syncthreads();
-            for (int jj = 0; jj < cnt; ++jj) {
+            // the forward's pair test (k_density), the lane's own particle excluded
+            uint64_t rest = valid ? pair_mask(cP, cnt, me, qh, false) & ~self_bit(own, cand, cnt, lane, t) : 0ull;
+            while (__ballot(rest != 0ull)) {
+                if (!rest) continue;
+                const int jj = __builtin_ctzll(rest);
+                rest &= rest - 1ull;
                 const float4 pj = cP[jj];
-                float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                float hij = 0.5f * (hi + cH[jj]);
-                if (!(r2 < hij * hij) || cand[jj] == t || !valid) continue;  // the forward's pair test
+                const double mj = cM[jj];
                 const double d[3] = {(double)me.x - (double)pj.x, (double)me.y - (double)pj.y,
                                      (double)me.z - (double)pj.z};
-                const double rr = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                const double inv_h = 1.0 / (0.5 * ((double)hi + (double)cH[jj]));
+                const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const double irr = r2 > 0.0 ? nr_rsq(r2) : 0.0;
+                const double rr = r2 * irr;
+                const double inv_h = nr_rcp((double)qh + (double)pj.w);
                 const double q = rr * inv_h;
                 const double sig = (8.0 / 3.14159265358979323846) * (inv_h * inv_h * inv_h);
                 const double w = q < 0.5 ? sig * (6.0 * (q * q * q - q * q) + 1.0)
@@ -759,7 +777,7 @@ __global__ __launch_bounds__(64) void k_vjp_density(const float4* __restrict__ s
                 const double dw = q < 0.5 ? sig * inv_h * (18.0 * q * q - 12.0 * q)
                                           : (q < 1.0 ? sig * inv_h * -6.0 * (1.0 - q) * (1.0 - q) : 0.0);
                 const double rbj = cRb[jj];
-                const double coef = (rbi * (double)pj.w + rbj * (double)me.w) * (rr > 0.0 ? dw / rr : 0.0);
+                const double coef = (rbi * mj + rbj * (double)me.w) * (dw * irr);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) xb[k] += coef * d[k];
                 mb += rbj * w;

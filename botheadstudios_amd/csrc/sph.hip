@@ -98,45 +98,6 @@ struct CacheWriter {
     }
 };
 
-// The pair test of one lane against the candidates in LDS (x, y, z, h/2 per entry): bit jj is set when
-// r < (h_i + h_j) / 2 (sph_step.wgsl:201,210 / :262-272); `not_coincident` adds the force loop's r > 1e-9.
-// `margin` widens the radius test (the mixed form's f32 pre-filter keeps a superset of its exact f64 test).
-// The lane's own particle is removed afterwards with `self_slot`. Halving is exact, so h_i/2 + h_j/2 is
-// bit for bit the reference's (h_i + h_j) * 0.5. Each half of the mask is built from its highest slot down,
-// one shift-and-insert per candidate.
-__device__ __forceinline__ bool pair_hit(float4 pj, float4 me, float qi, bool not_coincident, float margin) {
-    const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const float hij = qi + pj.w;
-    bool ok = r2 < margin * (hij * hij);
-    if (not_coincident) ok = ok && r2 > 1.0e-18f;
-    return ok;
-}
-__device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 me, float qi, bool not_coincident,
-                                              float margin = 1.0f) {
-    uint32_t hi = 0u, lo = 0u;
-#pragma unroll 4
-    for (int jj = min(cnt, kWave) - 1; jj >= 32; --jj)
-        hi = (hi << 1) | (pair_hit(cP[jj], me, qi, not_coincident, margin) ? 1u : 0u);
-#pragma unroll 4
-    for (int jj = min(cnt, 32) - 1; jj >= 0; --jj)
-        lo = (lo << 1) | (pair_hit(cP[jj], me, qi, not_coincident, margin) ? 1u : 0u);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// While loading candidate `cand[lane]`, record where a particle of the lane's own group sits in this batch
-// (own[d] = slot for group member d). A lane then finds its own particle with one LDS read and a check,
-// instead of comparing indices at every candidate. Stale entries from earlier batches fail the check.
-__device__ __forceinline__ void note_owner(uint32_t* own, uint32_t j, uint32_t g0, uint32_t gn, int lane) {
-    const uint32_t d = j - g0;
-    if (d < gn) own[d] = (uint32_t)lane;
-}
-__device__ __forceinline__ uint64_t self_bit(const uint32_t* own, const uint32_t* cand, int cnt, int lane,
-                                             uint32_t t) {
-    const uint32_t ms = own[lane];
-    return (ms < (uint32_t)cnt && cand[ms] == t) ? (1ull << ms) : 0ull;
-}
-
 template <bool F64>
 __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __restrict__ spos,
                                                              const float* __restrict__ sh, uint32_t n, uint32_t K, uint32_t L, const SphNode* __restrict__ nS,
