@@ -27,10 +27,15 @@ struct Adj {
     double* srb = nullptr;   // rho bar
     double* sqb = nullptr;   // (P / rho^2) bar
     double* scb = nullptr;   // sound-speed bar
-    double4* nbar = nullptr; // per tree node: (COM bar, mass bar)
+    double4* nbar = nullptr; // per tree node: (COM bar, mass bar), kNbarSlots copies (slot 0 holds the sum)
     double* pbar = nullptr;  // kMaxMaterials x 4: Tillotson a, b, A, B bars
     double* red = nullptr;   // loss reductions (16 doubles)
 };
+
+// The gravity adjoint's node sums arrive as f64 atomics from every group that accepted the node; the top
+// nodes take thousands each. Waves add into one of kNbarSlots copies (by workgroup id), which a reduction
+// folds into slot 0 before the push-down, so each address sees an eighth of the contention.
+constexpr uint32_t kNbarSlots = 8;
 
 // Loss: sum_i m_i sig(k_r (r_i / R - r_cut)) sig(k_j (j_i / sqrt(G M r_i) - 1)), with r_i, j_i the
 // distance and the specific angular momentum along `axis` about the centre of mass (total M), in f64.

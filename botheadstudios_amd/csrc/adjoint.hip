@@ -276,6 +276,7 @@ __global__ __launch_bounds__(64) void k_vjp_gravity_tree(const float4* __restric
                                                          const double4* __restrict__ sa,
                                                          const uint2* __restrict__ groups,
                                                          const uint32_t* __restrict__ scal, double4* nbar,
+                                                         uint32_t nbar_stride,
                                                          double4* sxb) {
     __shared__ uint32_t stk[kGStack];
     __shared__ float4 cA[kGBuf];
@@ -345,7 +346,8 @@ __global__ __launch_bounds__(64) void k_vjp_gravity_tree(const float4* __restric
             float sx, sy, sz, sm;
             rotate(cnt, src, 0xfffffffeu, sx, sy, sz, sm);
             if (lane < cnt) {
-                double* nb = reinterpret_cast<double*>(&nbar[node]);
+                double* nb = reinterpret_cast<double*>(
+                    &nbar[(size_t)(blockIdx.x % kNbarSlots) * nbar_stride + node]);
                 atomic_add_f64(nb + 0, kG64 * sx);
                 atomic_add_f64(nb + 1, kG64 * sy);
                 atomic_add_f64(nb + 2, kG64 * sz);
@@ -475,6 +477,22 @@ __global__ __launch_bounds__(64) void k_vjp_gravity_tree(const float4* __restric
             atomic_add_f64(pb + 2, kG64 * xbz);
         }
     }
+}
+
+// Folds the kNbarSlots partial node sums into slot 0.
+__global__ __launch_bounds__(256) void k_vjp_nbar_reduce(uint32_t nodes, double4* nbar) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nodes) return;
+    double4 s = nbar[t];
+#pragma unroll
+    for (uint32_t k = 1; k < kNbarSlots; ++k) {
+        const double4 b = nbar[(size_t)k * nodes + t];
+        s.x += b.x;
+        s.y += b.y;
+        s.z += b.z;
+        s.w += b.w;
+    }
+    nbar[t] = s;
 }
 
 // Node adjoints to particles: COM_n = sum m_k x_k / M_n and M_n = sum m_k over the particles under n.
@@ -892,9 +910,10 @@ void launch_vjp_eval(const Dev& d, const StepParams& p, int gravity_mode, const 
                            p.soft2, a.sxb);
     } else if (gravity_mode == 0) {  // tree, the forward's groups (one particle per lane)
         const uint32_t nodes = 2 * d.L - 1;
-        zero_f64(reinterpret_cast<double*>(a.nbar), 4 * (size_t)nodes, s);
+        zero_f64(reinterpret_cast<double*>(a.nbar), 4 * (size_t)kNbarSlots * nodes, s);
         hipLaunchKernelGGL(k_vjp_gravity_tree, dim3(std::min<uint32_t>(kWalkWaves, n)), dim3(kWave), 0, s, d.spos, n,
-                           d.K, d.L, d.nG, p.soft2, a.sa, d.groups, d.scal, a.nbar, a.sxb);
+                           d.K, d.L, d.nG, p.soft2, a.sa, d.groups, d.scal, a.nbar, nodes, a.sxb);
+        hipLaunchKernelGGL(k_vjp_nbar_reduce, dim3(cdiv_u(nodes, 256)), dim3(256), 0, s, nodes, a.nbar);
         hipLaunchKernelGGL(k_vjp_downsweep, dim3(cdiv_u(n, 256)), dim3(256), 0, s, n, d.K, d.L, d.parent, d.nA,
                            a.nbar, d.spos, a.sxb);
     }

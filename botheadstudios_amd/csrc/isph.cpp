@@ -457,7 +457,7 @@ void ensure_adj(isph_ctx* c) {
     dalloc(&a.srb, cap);
     dalloc(&a.sqb, cap);
     dalloc(&a.scb, cap);
-    dalloc(&a.nbar, 2 * cap);
+    dalloc(&a.nbar, kNbarSlots * 2 * (size_t)cap);
     dalloc(&a.pbar, kMaxMaterials * 4);
     dalloc(&a.red, 16);
 }
