@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--adjoint-n", type=int, default=262144)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20, help="checkpoint interval of the reverse pass")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint/forward cost measurement and print its object (for profiling it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-energy", action="store_true", help="skip the O(N^2) energy diagnostics")
     ap.add_argument("--variants", default="mixed,reuse",
@@ -323,6 +325,9 @@ def main():
         print(f"bench.py runs on one GPU: the particle step here is a single-GPU build (got --gpus {args.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.adjoint_only:
+        print(json.dumps({"metric": "adjoint/forward cost", "adjoint": run_adjoint(args)}), flush=True)
+        return
     from botheadstudios_amd import scene
 
     sc = scene.two_body_1m() if args.n == 1048576 else scene.two_body(args.n)
