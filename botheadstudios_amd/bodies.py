@@ -3,19 +3,23 @@ particle step, on the host in f64 as the engine builds them on the CPU.
 
   planet.rs:26-130         LayeredBody: layers, radius, temperature_at, enclosed / total mass
   hydrostatic.rs:32-38     fib_dir; :71-73 smoothing_for; :211-264 particalize; :483-492 relax_dt
+  hydrostatic.rs:57-68     Cap; :291-335 particalize_cap (a surface patch of the planet, the rest a bulk)
   eos.rs:52-96             Tillotson pressure and the central-difference sound speed (relax_dt needs it)
   gpu_sph.rs:98-113        SphEos::from_tillotson; :1010-1060 SphAssembly (materials deduped by rho0)
   gpu_sph.rs:264-292       build_far_apart_n; :419-441 far_apart_pair
   gpu_sph.rs:451-503       body_bulk (robust centre, mass and 98 %-mass radius)
   gpu_sph.rs:505-549       BodyPlacement, assemble_from_relaxed_n
   gpu_sph.rs:602-653       assemble_from_relaxed_at / assemble_from_relaxed_with
+  gpu_sph.rs:293-330       build_cap_relax; :551-591 assemble_cap_impact
+  accretion.rs:1418-1423   resolution_distance (the 1 % tidal hand-off distance)
   terra/world_def.rs:317-420  ImpactDef and its defaults (the Birth-of-the-Moon world, worlds/birth/world.json)
 
 The body definitions are the layer tables (material, outer radius, in-situ density, geotherm) of the
 engine's assets/bodies/{proto-earth,theia,earth,moon}.json. `birth_of_the_moon` and `moon_drop` run the
 engine's two whole-body collision scenes end to end on the device stepper: staging far apart, damped
 relaxation with artificial viscosity off, assembly on the collision geometry, the impact, and the on-device
-disk measurement.
+disk measurement. `moon_on_earth_cap` runs the engine's resolution-on-demand branch (lib.rs:1950-1996,
+3711-3745): only a cap of the Earth around the impact site is particles, the rest is the abstract bulk.
 """
 from dataclasses import dataclass, field
 
@@ -194,6 +198,47 @@ def particalize(matter, resolution):
                      rho=np.concatenate(rho), eos=eos, softening=0.25 * float(np.min(h)), materials=names)
 
 
+@dataclass
+class Cap:
+    """hydrostatic.rs:57-68: a particalized surface patch of a planet, the radius of the floor it rests on
+    and the mass of the un-resolved remainder (the bulk's Gauss gravity source)."""
+    body: HydroBody
+    r_core: float
+    bulk_mass: float
+
+
+def particalize_cap(matter, site_dir, cap_radius, resolution):
+    """HydroBody::particalize_cap (hydrostatic.rs:291-335): particalize the whole body finely enough that
+    the ball of `cap_radius` about the surface point along `site_dir` keeps about `resolution` particles
+    (that ball is about half inside the planet, volume 2/3 pi R^3; the whole-body count is clamped to
+    [resolution, 400 000]), then keep only the particles inside the ball. The floor sits at the deepest
+    kept particle; the bulk mass is the planet less the cap."""
+    r_surf = matter.radius()
+    d = np.asarray(site_dir, np.float64)
+    nd = float(np.linalg.norm(d))
+    site = d / nd * r_surf if nd > 0.0 else np.zeros(3)
+    v_planet = FOUR_THIRDS_PI * r_surf ** 3
+    v_cap = (2.0 / 3.0) * np.pi * cap_radius ** 3
+    full_n = int(min(max(np.ceil(resolution * (v_planet / max(v_cap, 1.0))), float(resolution)), 400000.0))
+    full = particalize(matter, full_n)
+    keep = np.nonzero(np.linalg.norm(full.pos - site, axis=1) < cap_radius)[0]
+    body_ = HydroBody(pos=full.pos[keep], vel=np.zeros((len(keep), 3)), mass=full.mass[keep], u=full.u[keep],
+                      h=full.h[keep], rho=full.rho[keep], eos=[full.eos[i] for i in keep],
+                      softening=full.softening, materials=[full.materials[i] for i in keep])
+    cap_mass = float(body_.mass.sum())
+    r_core = float(np.linalg.norm(body_.pos, axis=1).min()) if len(keep) else np.inf
+    return Cap(body=body_, r_core=r_core if np.isfinite(r_core) else r_surf,
+               bulk_mass=max(matter.total_mass() - cap_mass, 0.0))
+
+
+def resolution_distance(m1, r1, m2, tidal_fraction):
+    """accretion.rs:1418-1423: the separation at which body 2's tidal stress across body 1 reaches
+    `tidal_fraction` of body 1's surface gravity (0 for non-physical inputs)."""
+    if m1 <= 0.0 or r1 <= 0.0 or m2 <= 0.0 or tidal_fraction <= 0.0:
+        return 0.0
+    return r1 * np.cbrt(2.0 * (m2 / m1) / tidal_fraction)
+
+
 # ---- the SPH assembly (gpu_sph.rs) ---------------------------------------------------------------------
 def eos_record(t):
     """SphEos::from_tillotson (gpu_sph.rs:98-113)."""
@@ -259,6 +304,25 @@ def build_far_apart_n(bodies, separation):
     softening = np.float32(min(hb.softening for hb in hbs))
     relax_dt = np.float32(min(hb.relax_dt(0.2) for hb in hbs))
     return asm.particles, asm.eos_table(), float(softening), float(relax_dt), asm
+
+
+def build_cap_relax(cap_body, impactors, separation):
+    """gpu_sph.rs:293-330: the cap where it lies (prov 0) plus each (matter, resolution) impactor
+    particalized and placed on +x beyond it: x starts at the cap's largest radius and each impactor of
+    radius r sits at x += separation * 2 r + r, then x += r. Returns (particles, eos table, softening,
+    relax dt, the assembly), the last two the smallest over the cap and the impactors."""
+    asm = SphAssembly()
+    asm.add_body(cap_body, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    hbs = [particalize(m, res) for m, res in impactors]
+    x = max(0.0, float(np.linalg.norm(cap_body.pos, axis=1).max())) if len(cap_body.mass) else 0.0
+    for k, hb in enumerate(hbs):
+        r = max(0.0, float(np.linalg.norm(hb.pos, axis=1).max()))
+        x += separation * 2.0 * r + r
+        asm.add_body(hb, k + 1, (x, 0.0, 0.0), (0.0, 0.0, 0.0))
+        x += r
+    softening = float(np.float32(min([cap_body.softening] + [hb.softening for hb in hbs])))
+    relax_dt = float(np.float32(min([cap_body.relax_dt(0.2)] + [hb.relax_dt(0.2) for hb in hbs])))
+    return asm.particles, asm.eos_table(), softening, relax_dt, asm
 
 
 def far_apart_pair(target, impactor, separation_m):
@@ -365,6 +429,29 @@ def assemble_from_relaxed_with(d, particles):
     return assemble_from_relaxed_at(particles, d.target_spin_rad_s, (d0, b, 0.0), (-v_esc, 0.0, 0.0), 0.0)
 
 
+def assemble_cap_impact(relaxed_impactors, placements, impactor_eos, cap_body):
+    """gpu_sph.rs:551-591: the relaxed impactor(s) placed by `assemble_from_relaxed_n`, after a freshly
+    assembled cap (prov 0, where it lies). One material table: the cap's materials first, then the
+    impactor's (shifted, capped at 16 slots); every impactor particle becomes prov 1. Returns (particles,
+    eos table, softening = 0.25 x the smallest h, the impactor placement's shock-safe dt, the f64 rows of
+    the table)."""
+    imp, _, dt = assemble_from_relaxed_n(relaxed_impactors, placements)
+    asm = SphAssembly()
+    asm.add_body(cap_body, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    cap_mats = len(asm.eos)
+    eos = list(asm.eos)
+    for e in impactor_eos:
+        if len(eos) < MAX_MATERIALS:
+            eos.append(e)
+    imp["mat"] = np.minimum(imp["mat"].astype(np.int64) + cap_mats, MAX_MATERIALS - 1)
+    imp["prov"] = 1
+    out = np.concatenate([asm.particles, imp])
+    min_h = np.min(out["h"]) if len(out) else np.float32(np.inf)
+    table = np.array(eos, dtype=materials.EOS_DTYPE)
+    rows = np.array([[float(e[k]) for k in materials.EOS_DTYPE.names[:10]] for e in eos])
+    return out, table, float(np.float32(0.25) * min_h), dt, rows
+
+
 def whole_body_counts(target, impactor, n_target):
     """Equal particle mass across the bodies, the impactor's count from its real mass with the small-body
     floor of 50 (lib.rs:3420-3423)."""
@@ -462,3 +549,55 @@ def moon_drop(n_target=700, relax_steps=800, sim_seconds=6000.0, cfl=0.1, device
         stats = s.disk_stats_json(0)
     return {"assembled": assembled, "final": final, "sh_by_mat": sh_by_mat, "stats": stats, "v0": v0,
             "steps": steps, "counts": counts}
+
+
+def moon_on_earth_cap(cap_resolution=2000, relax_steps=2400, sim_seconds=3000.0, cfl=0.1, device=0):
+    """The Moon dropped on the modern Earth through the engine's resolution-on-demand branch
+    (lib.rs:1950-1996 and 3711-3745): a cap of radius min(2 r_moon, 0.55 r_earth) around the impact site
+    is particalized at `cap_resolution` (particalize_cap), the Moon whole at 1500 particles (the impactor
+    mass over 1500, floor 200); both relax seated on the bulk (Gauss gravity of the planet less the cap
+    plus the floor at the cap's inner edge; damping 0.94, artificial viscosity off, build_cap_relax's
+    dt). Assembly keeps the cap where it settled and puts the Moon on a radial fall from the lunar
+    distance at the hand-off distance max(1 % tidal distance, contact); the impact runs with the bulk at
+    the Courant dt to `sim_seconds`. Returns the cap plan, the read-backs and the disk JSON."""
+    from . import isph
+    e_def, m_def = body("earth"), body("moon")
+    r_e, r_m = e_def.radius(), m_def.radius()
+    m_e, m_m = e_def.total_mass(), m_def.total_mass()
+    site = np.array([0.0, 1.0, 0.0])
+    cap_radius = min(2.0 * r_m, 0.55 * r_e)
+    cap = particalize_cap(e_def, site, cap_radius, cap_resolution)
+    m_i = m_m / 1500.0
+    n_imp = int(max(_round_half_away(m_m / m_i), 200.0))
+    staged, eos, soft, relax_dt, asm = build_cap_relax(cap.body, [(m_def, n_imp)], 40.0)
+    sh_by_mat = np.array([materials.SPECIFIC_HEAT.get(nm, 1000.0) for nm in asm.names])
+    with isph.GpuSph(len(staged), device) as s:
+        s.upload(staged, eos, soft)
+        s.set_eos_f64(asm.eos_f64())
+        s.set_dt(relax_dt, 0.94)
+        s.set_av(0.0, 0.0)
+        s.set_bulk((0.0, 0.0, 0.0), cap.r_core, (0.0, 0.0, 0.0), cap.bulk_mass)
+        s.encode_relax(relax_steps)
+        relaxed = s.readback_sync()
+        cap_com = body_bulk(relaxed, 0)[0]
+        d0 = max(resolution_distance(m_e, r_e, m_m, 0.01), r_e + r_m)
+        v0 = np.sqrt(2.0 * G * (m_e + m_m) * (1.0 / d0 - 1.0 / 3.844e8))
+        pl = [BodyPlacement(tuple(cap_com), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
+              BodyPlacement(tuple(site * d0), tuple(-site * v0), (0.0, 0.0, 0.0))]
+        assembled, soft2, shock_dt = assemble_from_relaxed_n(relaxed, pl)
+        s.upload(assembled, eos, soft2)
+        s.set_eos_f64(asm.eos_f64())
+        s.set_dt(shock_dt, 1.0)
+        s.set_av(1.0, 2.0)
+        s.set_bulk((0.0, 0.0, 0.0), cap.r_core, (0.0, 0.0, 0.0), cap.bulk_mass)
+        t0, _ = s.sim_time()
+        steps = 0
+        while s.sim_time()[0] - t0 < sim_seconds:
+            s.step_cfl(cfl, 20)
+            steps += 20
+        final = s.readback_sync()
+        s.disk_view()
+        stats = s.disk_stats_json(0)
+    return {"cap": cap, "staged": staged, "relaxed": relaxed, "assembled": assembled, "final": final,
+            "sh_by_mat": sh_by_mat, "stats": stats, "v0": v0, "d0": d0, "steps": steps,
+            "counts": (len(cap.body.mass), n_imp), "eos_names": list(asm.names)}

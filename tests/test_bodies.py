@@ -248,3 +248,107 @@ def test_body_bulk_of_a_uniform_ball_and_with_strays():
     rec["pos"][:5, 0] += np.float32(20 * R)
     c2, _, r2 = B.body_bulk(rec, 0)
     assert np.linalg.norm(c2 - c) < 0.01 * R and abs(r2 - r) < 0.01 * R
+
+
+def test_particalize_cap_resolves_only_the_impact_region_of_the_real_planet():
+    """hydrostatic.rs:1890-1942: a cap of 0.35 R about the surface point (site +x, 800 particles) is
+    localized, a small fraction of the planet, mass-conserving with its bulk, floored at its inner edge,
+    reaching the surface, with a workable count and the planet's own near-surface material."""
+    earth = B.body("earth")
+    r_surf = earth.radius()
+    cap_radius = 0.35 * r_surf
+    cap = B.particalize_cap(earth, (1.0, 0.0, 0.0), cap_radius, 800)
+    site_pt = np.array([r_surf, 0.0, 0.0])
+    assert np.all(np.linalg.norm(cap.body.pos - site_pt, axis=1) <= cap_radius + 1.0)
+    cap_mass = cap.body.mass.sum()
+    assert cap_mass < 0.2 * earth.total_mass()
+    assert abs(cap_mass + cap.bulk_mass - earth.total_mass()) / earth.total_mass() < 1e-9
+    assert r_surf - cap_radius - 0.1 * r_surf < cap.r_core < r_surf
+    assert np.linalg.norm(cap.body.pos, axis=1).max() > r_surf - 0.05 * r_surf
+    assert len(cap.body.mass) > 200
+    assert any(t[0] < 3500.0 for t in cap.body.eos)
+
+
+def test_particalize_cap_is_the_filtered_whole_body_seed():
+    """The cap is exactly the particles of the whole-body seed (count from the cap's volume share) inside
+    the ball, in seed order, with their own mass, h, u and material; the floor is the deepest of them."""
+    moon = B.body("moon")
+    r = moon.radius()
+    cap = B.particalize_cap(moon, (0.0, 0.0, 2.0), 0.5 * r, 300)
+    full_n = int(min(max(np.ceil(300 * ((4.0 / 3.0) * r ** 3) / ((2.0 / 3.0) * (0.5 * r) ** 3)), 300), 400000))
+    full = B.particalize(moon, full_n)
+    inside = np.linalg.norm(full.pos - np.array([0.0, 0.0, r]), axis=1) < 0.5 * r
+    np.testing.assert_array_equal(cap.body.pos, full.pos[inside])
+    np.testing.assert_array_equal(cap.body.u, full.u[inside])
+    np.testing.assert_array_equal(cap.body.h, full.h[inside])
+    assert cap.r_core == np.linalg.norm(full.pos[inside], axis=1).min()
+    assert np.all(cap.body.vel == 0.0)
+    # a zero site direction leaves the centre as the site: the ball about the centre
+    core = B.particalize_cap(moon, (0.0, 0.0, 0.0), 0.2 * r, 100)
+    assert np.all(np.linalg.norm(core.body.pos, axis=1) < 0.2 * r)
+
+
+def test_resolution_distance_scales_like_tides():
+    """accretion.rs:1431-1456: 1 % tidal stress of Theia on the proto-Earth at about 17,700 km, outside
+    contact; a heavier companion or a stricter threshold reaches further; no companion, never."""
+    m_earth, r_earth, m_theia = 5.435e24, 6.161e6, 6.477e23
+    d = B.resolution_distance(m_earth, r_earth, m_theia, 0.01)
+    assert 17.0e6 < d < 19.0e6 and d > 9.551e6
+    assert B.resolution_distance(m_earth, r_earth, 2.0 * m_theia, 0.01) > d
+    assert B.resolution_distance(m_earth, r_earth, m_theia, 0.001) > d
+    assert B.resolution_distance(m_earth, r_earth, 0.0, 0.01) == 0.0
+    # the closed form: r1 (2 (m2/m1) / f)^(1/3)
+    assert d == pytest.approx(r_earth * (2.0 * m_theia / m_earth / 0.01) ** (1.0 / 3.0), rel=1e-15)
+
+
+def test_build_cap_relax_keeps_the_cap_and_stages_the_impactors_beyond_it():
+    """gpu_sph.rs:293-330: the cap (prov 0) where it lies; impactor k (prov k+1) shifted along +x from the
+    cap's outer radius by separation x its diameter plus its radius, then past its own radius; softening
+    and relaxation dt the smallest of the parts."""
+    earth, moon = B.body("earth"), B.body("moon")
+    cap = B.particalize_cap(earth, (0.0, 1.0, 0.0), 2.0 * moon.radius(), 400)
+    parts, eos, soft, relax_dt, asm = B.build_cap_relax(cap.body, [(moon, 300), (moon, 120)], 40.0)
+    n0 = len(cap.body.mass)
+    assert np.all(parts["prov"][:n0] == 0)
+    np.testing.assert_array_equal(parts["pos"][:n0], cap.body.pos.astype(np.float32))
+    x = np.linalg.norm(cap.body.pos, axis=1).max()
+    start = n0
+    for k, res in enumerate((300, 120)):
+        hb = B.particalize(moon, res)
+        r = np.linalg.norm(hb.pos, axis=1).max()
+        x += 40.0 * 2.0 * r + r
+        sel = parts[start:start + len(hb.mass)]
+        assert np.all(sel["prov"] == k + 1)
+        np.testing.assert_array_equal(sel["pos"], (hb.pos + np.array([x, 0.0, 0.0])).astype(np.float32))
+        x += r
+        start += len(hb.mass)
+    assert start == len(parts)
+    hbs = [B.particalize(moon, 300), B.particalize(moon, 120)]
+    assert soft == float(np.float32(min([cap.body.softening] + [h.softening for h in hbs])))
+    assert relax_dt == float(np.float32(min([cap.body.relax_dt(0.2)] + [h.relax_dt(0.2) for h in hbs])))
+    # one material table for all: the moon's iron and peridotite dedupe against the earth's by rho0
+    assert len(eos) == len({float(e["rho0"]) for e in eos})
+
+
+def test_assemble_cap_impact_merges_the_tables_and_marks_the_impactor():
+    """gpu_sph.rs:551-591: cap materials first, the impactor's shifted after them, every impactor particle
+    prov 1, softening 0.25 x the smallest h, the placement's shock-safe dt."""
+    earth, moon = B.body("earth"), B.body("moon")
+    cap = B.particalize_cap(earth, (0.0, 1.0, 0.0), 2.0 * moon.radius(), 300)
+    staged, imp_eos, _, _, _ = B.build_far_apart_n([(moon, 200)], 40.0)
+    pl = [B.BodyPlacement((0.0, 1.0e7, 0.0), (0.0, -3000.0, 0.0), (0.0, 0.0, 0.0))]
+    out, table, soft, dt, rows = B.assemble_cap_impact(staged, pl, imp_eos, cap.body)
+    n0 = len(cap.body.mass)
+    cap_asm = B.SphAssembly()
+    cap_asm.add_body(cap.body, 0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    nc = len(cap_asm.eos)
+    assert len(table) == nc + len(imp_eos)
+    np.testing.assert_array_equal(table[nc:], imp_eos)
+    assert np.all(out["prov"][:n0] == 0) and np.all(out["prov"][n0:] == 1)
+    np.testing.assert_array_equal(out["mat"][n0:], staged["mat"] + nc)
+    np.testing.assert_array_equal(out["pos"][:n0], cap.body.pos.astype(np.float32))
+    imp, _, dt_n = B.assemble_from_relaxed_n(staged, pl)
+    np.testing.assert_array_equal(out["pos"][n0:], imp["pos"])
+    assert dt == dt_n
+    assert soft == float(np.float32(0.25) * np.min(out["h"]))
+    assert rows.shape == (len(table), 10) and rows[0][0] == float(table[0]["rho0"])
