@@ -590,9 +590,8 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
                 note_owner(own, j, grp.x, grp.y, lane);
             }
             __syncthreads();
-            // the forward's pair set: its f32 pre-filter (1e-4 margin), then its exact f64 test per hit
-            uint64_t rest = valid ? pair_mask(cP, cnt, me, qh, false, 1.0001f) & ~self_bit(own, cand, cnt, lane, t)
-                                  : 0ull;
+            // the forward's pair set: its f32 radius test, coincident pairs skipped per hit
+            uint64_t rest = valid ? pair_mask(cP, cnt, me, qh) & ~self_bit(own, cand, cnt, lane, t) : 0ull;
             while (__ballot(rest != 0ull)) {
                 if (!rest) continue;
                 const int jj = __builtin_ctzll(rest);
@@ -602,7 +601,7 @@ __global__ __launch_bounds__(64) void k_vjp_force(const float4* __restrict__ spo
                                      (double)me.z - (double)pj.z};
                 const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 const double hij = (double)qh + (double)pj.w;
-                if (!(r2 < hij * hij && r2 > 1.0e-18)) continue;
+                if (!(r2 > 1.0e-18)) continue;
                 const float4 vj = cV[jj];
                 const double v[3] = {(double)vi.x - (double)vj.x, (double)vi.y - (double)vj.y,
                                      (double)vi.z - (double)vj.z};
@@ -775,7 +774,7 @@ __global__ __launch_bounds__(64) void k_vjp_density(const float4* __restrict__ s
             }
             __syncthreads();
             // the forward's pair test (k_density), the lane's own particle excluded
-            uint64_t rest = valid ? pair_mask(cP, cnt, me, qh, false) & ~self_bit(own, cand, cnt, lane, t) : 0ull;
+            uint64_t rest = valid ? pair_mask(cP, cnt, me, qh) & ~self_bit(own, cand, cnt, lane, t) : 0ull;
             while (__ballot(rest != 0ull)) {
                 if (!rest) continue;
                 const int jj = __builtin_ctzll(rest);

@@ -110,7 +110,7 @@ void free_all(isph_ctx* c) {
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
                     d.lcode, d.left, d.right, d.parent, d.cslot, d.nrange, d.nlevel, d.carry, d.ncarry, d.groups, d.ggroups, d.chead,
-                    d.cnext, d.ccnt, d.cpool, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nS, d.nA, d.nQ0, d.nQ1, d.scal, d.dt,
+                    d.cnext, d.ccnt, d.cpool, d.cmask, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nS, d.nA, d.nQ0, d.nQ1, d.scal, d.dt,
                     d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -180,6 +180,7 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
         dalloc(&d.cnext, blocks * kChunksPerBlock);
         dalloc(&d.ccnt, blocks * kChunksPerBlock);
         dalloc(&d.cpool, blocks * kChunksPerBlock * kWave);
+        dalloc(&d.cmask, blocks * kChunksPerBlock * kWave);
         d.cblocks = (uint32_t)blocks;
     }
     dalloc(&d.gflag, cap);

@@ -51,6 +51,7 @@ struct CandCache {
     uint32_t* next;
     uint32_t* count;
     uint32_t* pool;
+    uint64_t* mask;
     uint32_t blocks;
 };
 
@@ -152,7 +153,9 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
         __syncthreads();
         // 1) The radius test against every candidate (LDS broadcast reads): one bit per true pair, the
         //    lane's own particle excluded (sph_step.wgsl:210).
-        const uint64_t hit = pair_mask(cP, cnt, me, qi, false) & ~self_bit(own, cand, cnt, lane, t);
+        const uint64_t hit = pair_mask(cP, cnt, me, qi) & ~self_bit(own, cand, cnt, lane, t);
+        // the force pass of this evaluation replays the chunk with this mask (its pair set is the same)
+        if (cw.ok) cc.mask[(size_t)cw.last * kWave + lane] = hit;
         // 2) The kernel sum over this lane's hits only, in candidate order. Lanes walk their own hit lists
         //    in step, so the wave runs as many iterations as its busiest lane instead of one per candidate
         //    with most lanes idle.
@@ -274,7 +277,8 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                                               double* __restrict__ acc64, const uint32_t* __restrict__ sinst,
                                               const uint32_t* __restrict__ iroot, Boundary bnd,
                                               const uint32_t* __restrict__ chead, const uint32_t* __restrict__ cnext,
-                                              const uint32_t* __restrict__ ccnt, const uint32_t* __restrict__ cpool) {
+                                              const uint32_t* __restrict__ ccnt, const uint32_t* __restrict__ cpool,
+                                              const uint64_t* __restrict__ cmask) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, h
@@ -311,7 +315,8 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     double ax64 = 0.0, ay64 = 0.0, az64 = 0.0, de64 = 0.0;
     const double rhoi64 = F64 ? srho64[tc] : 0.0, pri64 = F64 ? spr64[tc] : 0.0, ci64 = F64 ? scs64[tc] : 0.0;
     unsigned long long tested = 0;
-    auto flush = [&](int cnt) {
+    // `cached`: the density pass's pair mask of this chunk (the replay), else the mask is computed here
+    auto flush = [&](int cnt, bool have_mask, uint64_t cached) {
         if (lane < cnt) {
             uint32_t j = cand[lane];
             const float4 p = spos[j];
@@ -327,11 +332,11 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             note_owner(own, j, grp.x, grp.y, lane);
         }
         __syncthreads();
-        // 1) The radius test against every candidate, one bit per pair (plain: the exact test of
-        //    sph_step.wgsl:262-272; mixed: an f32 pre-filter with a 1e-4 margin, a superset of the exact f64
-        //    test below). 2) The pair terms over this lane's hits only, in candidate order.
-        uint64_t rest = (F64 ? pair_mask(cP, cnt, me, qi, false, 1.0001f) : pair_mask(cP, cnt, me, qi, true)) &
-                        ~self_bit(own, cand, cnt, lane, t);
+        // 1) The radius test against every candidate, one bit per pair: r < (h_i + h_j) / 2 in f32
+        //    (sph_step.wgsl:262-272), the density's own test, so a replayed chunk takes the density's mask.
+        //    2) The pair terms over this lane's hits only, in candidate order; coincident pairs
+        //    (r^2 <= 1e-18, sph_step.wgsl:268) are skipped there.
+        uint64_t rest = have_mask ? cached : pair_mask(cP, cnt, me, qi) & ~self_bit(own, cand, cnt, lane, t);
         while (__ballot(rest != 0ull)) {
             if (!rest) continue;
             const int jj = __builtin_ctzll(rest);
@@ -343,7 +348,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                        dz = (double)me.z - (double)pj.z;
                 double r2 = dx * dx + dy * dy + dz * dz;
                 double hij = (double)qi + (double)pj.w;
-                if (r2 < hij * hij && r2 > 1.0e-18) {
+                if (r2 > 1.0e-18) {
                     float4 vj = cV[jj];
                     double inv_r = nr_rsq(r2);
                     double inv_h = nr_rcp(hij);
@@ -375,6 +380,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             } else {
                 float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;  // dpos = pos_i - pos_j
                 float r2 = dx * dx + dy * dy + dz * dz;
+                if (!(r2 > 1.0e-18f)) continue;
                 float hij = qi + pj.w;
                 float4 vj = cV[jj];
                 float r = hw_sqrt(r2);
@@ -405,14 +411,15 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     const uint32_t head = chead ? chead[gi] : kChunkOverflow;
     if (head == kChunkOverflow) {
         neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
-                       counters ? counters + (size_t)blockIdx.x * kCounterSlots + 11 : nullptr, flush,
-                       sinst ? iroot[sinst[grp.x]] : 0u);
+                       counters ? counters + (size_t)blockIdx.x * kCounterSlots + 11 : nullptr,
+                       [&](int cnt) { flush(cnt, false, 0ull); }, sinst ? iroot[sinst[grp.x]] : 0u);
     } else {
         for (uint32_t id = head; id != kNoChunk;) {
             const uint32_t cnt = ccnt[id], nx = cnext[id];
             if ((uint32_t)lane < cnt) cand[lane] = cpool[(size_t)id * kWave + lane];
+            const uint64_t m = cmask[(size_t)id * kWave + lane];
             __syncthreads();
-            flush((int)cnt);
+            flush((int)cnt, true, m);
             __syncthreads();
             id = nx;
         }
@@ -732,7 +739,9 @@ __global__ __launch_bounds__(256) void k_diag_pe(uint32_t n, const float4* posh,
 
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-static CandCache cand_cache(const Dev& d) { return CandCache{d.chead, d.cnext, d.ccnt, d.cpool, d.cblocks}; }
+static CandCache cand_cache(const Dev& d) {
+    return CandCache{d.chead, d.cnext, d.ccnt, d.cpool, d.cmask, d.cblocks};
+}
 
 static Boundary boundary(const Dev& d, const StepParams& p) {
     return Boundary{p.bulk_cr, p.bulk_vm, d.ext, p.n_ext, p.soft2};
@@ -762,13 +771,14 @@ void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
                            d.n, d.K, d.L, d.nS, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
                            d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
                            (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext,
-                           d.ccnt, d.cpool);
+                           d.ccnt, d.cpool, d.cmask);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
                            d.scs, d.n, d.K, d.L, d.nS, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
                            p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand,
-                           (double*)nullptr, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext, d.ccnt, d.cpool);
+                           (double*)nullptr, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext, d.ccnt, d.cpool,
+                           d.cmask);
 }
 void launch_kick_drift(const Dev& d, const StepParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,

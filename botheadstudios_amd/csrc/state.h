@@ -85,6 +85,7 @@ struct Dev {
     uint32_t* cnext = nullptr;
     uint32_t* ccnt = nullptr;
     uint32_t* cpool = nullptr;
+    uint64_t* cmask = nullptr;  // per chunk and group lane: the density's pair mask over the chunk's slots
     uint32_t cblocks = 0;
     uint32_t* gflag = nullptr;  // group marks at their first particle (all zero between builds)
     uint32_t* gscan = nullptr;  // per-64-particle mark counts / offsets

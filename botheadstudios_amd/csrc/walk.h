@@ -112,28 +112,23 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
 }
 
 // The pair test of one lane against the candidates in LDS (x, y, z, h/2 per entry): bit jj is set when
-// r < (h_i + h_j) / 2 (sph_step.wgsl:201,210 / :262-272); `not_coincident` adds the force loop's r > 1e-9.
-// `margin` widens the radius test (the mixed form's f32 pre-filter keeps a superset of its exact f64 test).
-// The lane's own particle is removed afterwards with `self_slot`. Halving is exact, so h_i/2 + h_j/2 is
-// bit for bit the reference's (h_i + h_j) * 0.5. Each half of the mask is built from its highest slot down,
-// one shift-and-insert per candidate.
-__device__ __forceinline__ bool pair_hit(float4 pj, float4 me, float qi, bool not_coincident, float margin) {
+// r < (h_i + h_j) / 2 (sph_step.wgsl:201,210 / :262-272), one test for the density and the force passes of
+// both precision forms (the force skips coincident pairs, r <= 1e-9, inside its pair loop). The lane's own
+// particle is removed afterwards with `self_bit`. Halving is exact, so h_i/2 + h_j/2 is bit for bit the
+// reference's (h_i + h_j) * 0.5. Each half of the mask is built from its highest slot down, one
+// shift-and-insert per candidate.
+__device__ __forceinline__ bool pair_hit(float4 pj, float4 me, float qi) {
     const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float hij = qi + pj.w;
-    bool ok = r2 < margin * (hij * hij);
-    if (not_coincident) ok = ok && r2 > 1.0e-18f;
-    return ok;
+    return r2 < hij * hij;
 }
-__device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 me, float qi, bool not_coincident,
-                                              float margin = 1.0f) {
+__device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 me, float qi) {
     uint32_t hi = 0u, lo = 0u;
 #pragma unroll 4
-    for (int jj = min(cnt, kWave) - 1; jj >= 32; --jj)
-        hi = (hi << 1) | (pair_hit(cP[jj], me, qi, not_coincident, margin) ? 1u : 0u);
+    for (int jj = min(cnt, kWave) - 1; jj >= 32; --jj) hi = (hi << 1) | (pair_hit(cP[jj], me, qi) ? 1u : 0u);
 #pragma unroll 4
-    for (int jj = min(cnt, 32) - 1; jj >= 0; --jj)
-        lo = (lo << 1) | (pair_hit(cP[jj], me, qi, not_coincident, margin) ? 1u : 0u);
+    for (int jj = min(cnt, 32) - 1; jj >= 0; --jj) lo = (lo << 1) | (pair_hit(cP[jj], me, qi) ? 1u : 0u);
     return ((uint64_t)hi << 32) | lo;
 }
 
