@@ -115,20 +115,29 @@ __device__ __forceinline__ void neighbour_walk(uint32_t* stk, uint32_t* cand, ui
 // r < (h_i + h_j) / 2 (sph_step.wgsl:201,210 / :262-272), one test for the density and the force passes of
 // both precision forms (the force skips coincident pairs, r <= 1e-9, inside its pair loop). The lane's own
 // particle is removed afterwards with `self_bit`. Halving is exact, so h_i/2 + h_j/2 is bit for bit the
-// reference's (h_i + h_j) * 0.5. Each half of the mask is built from its highest slot down, one
-// shift-and-insert per candidate.
-__device__ __forceinline__ bool pair_hit(float4 pj, float4 me, float qi) {
+// reference's (h_i + h_j) * 0.5. Each half of the mask is built from its highest slot down: the bit is the
+// sign of r^2 - h_ij^2 (negative, or -0 after a flush to zero, exactly when r^2 < h_ij^2), shifted in with
+// one funnel shift per candidate.
+__device__ __forceinline__ uint32_t pair_miss_sign(float4 pj, float4 me, float qi) {
     const float dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float hij = qi + pj.w;
-    return r2 < hij * hij;
+    float d;
+    {
+#pragma clang fp contract(off)
+        const float hh = hij * hij;  // h_ij^2 rounded on its own, never fused into r^2 - h_ij^2
+        d = r2 - hh;
+    }
+    return __float_as_uint(d);
 }
 __device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 me, float qi) {
     uint32_t hi = 0u, lo = 0u;
 #pragma unroll 4
-    for (int jj = min(cnt, kWave) - 1; jj >= 32; --jj) hi = (hi << 1) | (pair_hit(cP[jj], me, qi) ? 1u : 0u);
+    for (int jj = min(cnt, kWave) - 1; jj >= 32; --jj)
+        hi = __builtin_amdgcn_alignbit(hi, pair_miss_sign(cP[jj], me, qi), 31u);
 #pragma unroll 4
-    for (int jj = min(cnt, 32) - 1; jj >= 0; --jj) lo = (lo << 1) | (pair_hit(cP[jj], me, qi) ? 1u : 0u);
+    for (int jj = min(cnt, 32) - 1; jj >= 0; --jj)
+        lo = __builtin_amdgcn_alignbit(lo, pair_miss_sign(cP[jj], me, qi), 31u);
     return ((uint64_t)hi << 32) | lo;
 }
 
