@@ -190,24 +190,19 @@ __device__ __forceinline__ void till_p_c2(const Till& t, double rho_in, double u
 // library forms expand into refinement sequences several times longer.
 __device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-// f64 reciprocal and reciprocal square root: the hardware estimate refined by two Newton-Raphson steps
-// (each roughly doubles the correct bits of the ~2^-22 estimate), for the mixed form's pair terms. For
-// finite positive x the result is within a few ulp of the correctly rounded value; a compiled f64
-// division or sqrt costs about twice as many instructions.
+// f64 reciprocal and reciprocal square root: the hardware estimate refined by one Newton-Raphson step,
+// which squares its relative error. Measured on gfx950 over 4 Mi inputs in [2^-100, 2^100]: estimates
+// within 4.6e-8 (rcp) and 5.2e-8 (rsq), refined within 2.2e-15 and 4.3e-15 of the correctly rounded
+// results. The mixed form's pair terms need f64 range and cancellation-free sums, not the last ulp: the
+// f32-stored coordinates limit its per-particle error near 1e-7. A compiled f64 division or sqrt costs
+// about three times as many instructions.
 __device__ __forceinline__ double nr_rcp(double x) {
-    double y = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-x, y, 1.0);
-    return fma(y, e, y);
+    const double y = __builtin_amdgcn_rcp(x);
+    return fma(y, fma(-x, y, 1.0), y);
 }
 __device__ __forceinline__ double nr_rsq(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    double t = fma(-h * y, y, 0.5);
-    y = fma(y, t, y);
-    t = fma(-h * y, y, 0.5);
-    return fma(y, t, y);
+    const double y = __builtin_amdgcn_rsq(x);
+    return fma(y, fma(-0.5 * x * y, y, 0.5), y);
 }
 
 __device__ __forceinline__ float spline_w(float q, float inv_h) {
