@@ -413,15 +413,31 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
         neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                        counters ? counters + (size_t)blockIdx.x * kCounterSlots + 11 : nullptr,
                        [&](int cnt) { flush(cnt, false, 0ull); }, sinst ? iroot[sinst[grp.x]] : 0u);
-    } else {
-        for (uint32_t id = head; id != kNoChunk;) {
-            const uint32_t cnt = ccnt[id], nx = cnext[id];
-            if ((uint32_t)lane < cnt) cand[lane] = cpool[(size_t)id * kWave + lane];
-            const uint64_t m = cmask[(size_t)id * kWave + lane];
+    } else if (head != kNoChunk) {  // (a group without candidates has no chunk)
+        // One chunk ahead: the next chunk's count, link, candidate ids and masks load while this one is
+        // processed, so a chunk waits for one round trip (its particles' data) instead of three. Every
+        // chunk owns 64 slots, so the slot reads past its count stay inside the pool (and go unused).
+        uint32_t cnt = ccnt[head], nx = cnext[head];
+        uint32_t cj = cpool[(size_t)head * kWave + lane];
+        uint64_t m = cmask[(size_t)head * kWave + lane];
+        while (true) {
+            uint32_t cnt2 = 0, nx2 = kNoChunk, cj2 = 0;
+            uint64_t m2 = 0ull;
+            if (nx != kNoChunk) {
+                cnt2 = ccnt[nx];
+                nx2 = cnext[nx];
+                cj2 = cpool[(size_t)nx * kWave + lane];
+                m2 = cmask[(size_t)nx * kWave + lane];
+            }
+            if ((uint32_t)lane < cnt) cand[lane] = cj;
             __syncthreads();
             flush((int)cnt, true, m);
             __syncthreads();
-            id = nx;
+            if (nx == kNoChunk) break;
+            cnt = cnt2;
+            nx = nx2;
+            cj = cj2;
+            m = m2;
         }
     }
     if (F64) {
