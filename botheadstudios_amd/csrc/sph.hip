@@ -315,28 +315,36 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     double ax64 = 0.0, ay64 = 0.0, az64 = 0.0, de64 = 0.0;
     const double rhoi64 = F64 ? srho64[tc] : 0.0, pri64 = F64 ? spr64[tc] : 0.0, ci64 = F64 ? scs64[tc] : 0.0;
     unsigned long long tested = 0;
-    // `cached`: the density pass's pair mask of this chunk (the replay), else the mask is computed here
-    auto flush = [&](int cnt, bool have_mask, uint64_t cached) {
-        if (lane < cnt) {
-            uint32_t j = cand[lane];
-            const float4 p = spos[j];
-            cP[lane] = make_float4(p.x, p.y, p.z, 0.5f * sh[j]);
-            float4 v = svel[j];
-            cV[lane] = make_float4(v.x, v.y, v.z, srho[j]);
-            cS[lane] = make_float4(p.w, spr[j], scs[j], 0.f);
-            if (F64) {
-                cR64[lane] = srho64[j];
-                cPr64[lane] = spr64[j];
-                cC64[lane] = scs64[j];
-            }
-            note_owner(own, j, grp.x, grp.y, lane);
+    // One candidate's data, loaded into registers (`fetch`) and staged in LDS (`put`).
+    struct Cand {
+        float4 p, v;
+        float h, pr, cs;
+    };
+    auto fetch = [&](uint32_t j) {
+        Cand c;
+        c.p = spos[j];
+        c.h = sh[j];
+        c.v = svel[j];
+        c.v.w = srho[j];
+        c.pr = spr[j];
+        c.cs = scs[j];
+        return c;
+    };
+    auto put = [&](const Cand& c, uint32_t j) {
+        cP[lane] = make_float4(c.p.x, c.p.y, c.p.z, 0.5f * c.h);
+        cV[lane] = c.v;
+        cS[lane] = make_float4(c.p.w, c.pr, c.cs, 0.f);
+        if (F64) {
+            cR64[lane] = srho64[j];
+            cPr64[lane] = spr64[j];
+            cC64[lane] = scs64[j];
         }
-        __syncthreads();
-        // 1) The radius test against every candidate, one bit per pair: r < (h_i + h_j) / 2 in f32
-        //    (sph_step.wgsl:262-272), the density's own test, so a replayed chunk takes the density's mask.
-        //    2) The pair terms over this lane's hits only, in candidate order; coincident pairs
-        //    (r^2 <= 1e-18, sph_step.wgsl:268) are skipped there.
-        uint64_t rest = have_mask ? cached : pair_mask(cP, cnt, me, qi) & ~self_bit(own, cand, cnt, lane, t);
+    };
+    // The pair terms of one staged batch over `rest`, this lane's hits. 1) The radius test is r < (h_i +
+    // h_j) / 2 in f32 (sph_step.wgsl:262-272), the density's own test, so a replayed chunk takes the
+    // density's mask. 2) Each lane walks its own hits in candidate order; coincident pairs (r^2 <= 1e-18,
+    // sph_step.wgsl:268) are skipped there.
+    auto pairs = [&](int cnt, uint64_t rest) {
         while (__ballot(rest != 0ull)) {
             if (!rest) continue;
             const int jj = __builtin_ctzll(rest);
@@ -406,38 +414,60 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
         }
         tested += cnt;
     };
+    auto flush = [&](int cnt) {  // a batch of the walk: load, stage, test, pair terms
+        if (lane < cnt) {
+            const uint32_t j = cand[lane];
+            put(fetch(j), j);
+            note_owner(own, j, grp.x, grp.y, lane);
+        }
+        __syncthreads();
+        pairs(cnt, pair_mask(cP, cnt, me, qi) & ~self_bit(own, cand, cnt, lane, t));
+    };
     // The density walk of this evaluation stored the group's candidates batch by batch: replay them in the
     // same order (so the sums are the walk's, bit for bit), or walk when the list did not fit.
     const uint32_t head = chead ? chead[gi] : kChunkOverflow;
     if (head == kChunkOverflow) {
         neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                        counters ? counters + (size_t)blockIdx.x * kCounterSlots + 11 : nullptr,
-                       [&](int cnt) { flush(cnt, false, 0ull); }, sinst ? iroot[sinst[grp.x]] : 0u);
+                       flush, sinst ? iroot[sinst[grp.x]] : 0u);
     } else if (head != kNoChunk) {  // (a group without candidates has no chunk)
-        // One chunk ahead: the next chunk's count, link, candidate ids and masks load while this one is
-        // processed, so a chunk waits for one round trip (its particles' data) instead of three. Every
-        // chunk owns 64 slots, so the slot reads past its count stay inside the pool (and go unused).
+        // The replay runs ahead of the pair terms: while chunk k is processed, the count, link, candidate
+        // ids and masks of chunk k+2 are in flight and those of chunk k+1 have arrived, so a chunk waits for
+        // one round trip (its particles' data). Loading that data a chunk ahead too measured no faster.
+        // Every chunk owns 64 slots, so the slot reads past its count stay inside the pool (and go unused).
         uint32_t cnt = ccnt[head], nx = cnext[head];
         uint32_t cj = cpool[(size_t)head * kWave + lane];
         uint64_t m = cmask[(size_t)head * kWave + lane];
+        uint32_t cnt1 = 0, nx1 = kNoChunk, cj1 = 0;
+        uint64_t m1 = 0ull;
+        if (nx != kNoChunk) {
+            cnt1 = ccnt[nx];
+            nx1 = cnext[nx];
+            cj1 = cpool[(size_t)nx * kWave + lane];
+            m1 = cmask[(size_t)nx * kWave + lane];
+        }
         while (true) {
+            if ((uint32_t)lane < cnt) put(fetch(cj), cj);
             uint32_t cnt2 = 0, nx2 = kNoChunk, cj2 = 0;
             uint64_t m2 = 0ull;
-            if (nx != kNoChunk) {
-                cnt2 = ccnt[nx];
-                nx2 = cnext[nx];
-                cj2 = cpool[(size_t)nx * kWave + lane];
-                m2 = cmask[(size_t)nx * kWave + lane];
+            if (nx1 != kNoChunk) {
+                cnt2 = ccnt[nx1];
+                nx2 = cnext[nx1];
+                cj2 = cpool[(size_t)nx1 * kWave + lane];
+                m2 = cmask[(size_t)nx1 * kWave + lane];
             }
-            if ((uint32_t)lane < cnt) cand[lane] = cj;
             __syncthreads();
-            flush((int)cnt, true, m);
+            pairs((int)cnt, m);
             __syncthreads();
             if (nx == kNoChunk) break;
-            cnt = cnt2;
-            nx = nx2;
-            cj = cj2;
-            m = m2;
+            cnt = cnt1;
+            nx = nx1;
+            cj = cj1;
+            m = m1;
+            cnt1 = cnt2;
+            nx1 = nx2;
+            cj1 = cj2;
+            m1 = m2;
         }
     }
     if (F64) {
