@@ -262,11 +262,12 @@ __global__ __launch_bounds__(kDirTile) void k_vjp_gravity_direct(uint32_t n, con
 }
 
 // Gravity, tree: replays the forward walk of each group (same groups, same MAC, same node records) and
-// differentiates its interactions. Within a flush of up to 64 sources, lane l owns source l and lane p
-// owns group particle p; at rotation k lane l evaluates the pair (source l, particle (l + k) mod 64), keeps
-// the source-side gradient in its registers and passes the particle-side gradient to lane (l + k) mod 64
-// with a cross-lane permute. Source sums leave through f64 atomics: node COM / mass adjoints (pushed down
-// to particles afterwards) and the partner particles' position / mass adjoints.
+// differentiates its interactions. Within a flush of up to 64 sources, lane l owns source l; at rotation k
+// lane l evaluates the pair (source l, particle (l + k) mod 64) and keeps the source-side gradient in its
+// registers. The particle-side sums travel: lane l holds particle (l + k) mod 64's partial sum at rotation
+// k, and a wave rotate by one lane (DPP wave_rol, dst[l] = src[l + 1]) hands it on, so after 64 rotations
+// every sum is home. Source sums leave through f64 atomics: node COM / mass adjoints (pushed down to
+// particles afterwards) and the partner particles' position / mass adjoints.
 constexpr int kGStack = 512;
 constexpr int kGStackMargin = 128;
 constexpr int kGBuf = 128;
@@ -303,17 +304,24 @@ __global__ __launch_bounds__(64) void k_vjp_gravity_tree(const float4* __restric
         gI[lane] = valid ? t : 0xffffffffu;
         double xbx = 0, xby = 0, xbz = 0;
 
-        // Rotation over the 64 group particles for sources held by lanes < cnt.
-        auto rotate = [&](int cnt, const float4 src, const uint32_t src_idx, float& sx, float& sy, float& sz,
-                          float& sm) {
+        // Rotation over the 64 group particles for sources held by lanes < cnt. `self_check`: the sources
+        // are particles, which skip their own pair (cells never coincide with a group particle's index).
+        auto rol1 = [](float v) { return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false)); };
+        auto rotate = [&](int cnt, const float4 src, const uint32_t src_idx, bool self_check, float& sx, float& sy,
+                          float& sz, float& sm) {
             float px = 0.f, py = 0.f, pz = 0.f;
             sx = sy = sz = sm = 0.f;
             for (int k = 0; k < kWave; ++k) {
+                if (k > 0) {
+                    px = rol1(px);
+                    py = rol1(py);
+                    pz = rol1(pz);
+                }
                 const int pidx = (lane + k) & (kWave - 1);
                 const float4 xp = gP[pidx];
                 const float4 bp = gB[pidx];
                 float gx = 0.f, gy = 0.f, gz = 0.f;
-                if (lane < cnt && xp.w > 0.f && gI[pidx] != src_idx) {
+                if (lane < cnt && xp.w > 0.f && (!self_check || gI[pidx] != src_idx)) {
                     const float dx = src.x - xp.x, dy = src.y - xp.y, dz = src.z - xp.z;
                     const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
                     const float inv = __builtin_amdgcn_rsqf(r2);
@@ -329,22 +337,21 @@ __global__ __launch_bounds__(64) void k_vjp_gravity_tree(const float4* __restric
                     sz += gz;
                     sm += s * da;
                 }
-                // the particle-side gradient of pair (lane, pidx) goes to lane pidx
-                const int from = (lane - k) & (kWave - 1);
-                px -= __shfl(gx, from);
-                py -= __shfl(gy, from);
-                pz -= __shfl(gz, from);
+                // the particle-side gradient of pair (lane, pidx) joins pidx's travelling sum
+                px -= gx;
+                py -= gy;
+                pz -= gz;
             }
-            xbx += px;
-            xby += py;
-            xbz += pz;
+            xbx += rol1(px);
+            xby += rol1(py);
+            xbz += rol1(pz);
         };
         auto flush_cells = [&](int cnt) {
             __syncthreads();
             const float4 src = lane < cnt ? cA[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
             const uint32_t node = lane < cnt ? cN[lane] : 0u;
             float sx, sy, sz, sm;
-            rotate(cnt, src, 0xfffffffeu, sx, sy, sz, sm);
+            rotate(cnt, src, 0xfffffffeu, false, sx, sy, sz, sm);
             if (lane < cnt) {
                 double* nb = reinterpret_cast<double*>(
                     &nbar[(size_t)(blockIdx.x % kNbarSlots) * nbar_stride + node]);
@@ -359,7 +366,7 @@ __global__ __launch_bounds__(64) void k_vjp_gravity_tree(const float4* __restric
             const float4 src = lane < cnt ? pA[lane] : make_float4(0.f, 0.f, 0.f, 0.f);
             const uint32_t idx = lane < cnt ? pI[lane] : 0xfffffffeu;
             float sx, sy, sz, sm;
-            rotate(cnt, src, idx, sx, sy, sz, sm);
+            rotate(cnt, src, idx, true, sx, sy, sz, sm);
             if (lane < cnt) {
                 double* pb = reinterpret_cast<double*>(&sxb[idx]);
                 atomic_add_f64(pb + 0, kG64 * sx);
