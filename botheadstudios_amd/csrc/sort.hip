@@ -114,36 +114,78 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __r
     hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (256 * nb entries) by one block of 1024 threads, through LDS
-// tiles of 16384 entries: coalesced loads, a contiguous 16-entry run per thread, a block scan of the runs.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, uint32_t total) {
-    constexpr uint32_t kTile = 16384, kPer = kTile / 1024;
-    __shared__ uint32_t tile[kTile];
-    __shared__ uint32_t part[1024];
+// Exclusive scan of `total` counts in place by one block of 1024 threads, 16384 entries per tile: each
+// thread takes 16 consecutive entries (four 16-byte loads when the run is whole), the run sums are scanned
+// within each wave by shuffles and across the 16 waves through LDS, then each thread writes its run.
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* data, uint32_t total) {
+    constexpr uint32_t kPer = 16, kTile = 1024 * kPer, kWaves = 1024 / kWave;
+    __shared__ uint32_t wsum[kWaves + 1];
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < total; base += kTile) {
-        uint32_t cnt = min(kTile, total - base);
-        for (uint32_t k = threadIdx.x; k < kTile; k += 1024) tile[k] = k < cnt ? hist[base + k] : 0u;
-        __syncthreads();
+        const uint32_t i0 = base + threadIdx.x * kPer;
+        uint32_t v[kPer];
+        if (i0 + kPer <= total) {
+            const uint4* q = reinterpret_cast<const uint4*>(data + i0);
+#pragma unroll
+            for (uint32_t k = 0; k < kPer / 4; ++k) {
+                const uint4 x = q[k];
+                v[4 * k] = x.x;
+                v[4 * k + 1] = x.y;
+                v[4 * k + 2] = x.z;
+                v[4 * k + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) v[k] = i0 + k < total ? data[i0 + k] : 0u;
+        }
         uint32_t s = 0;
-        for (uint32_t k = 0; k < kPer; ++k) s += tile[threadIdx.x * kPer + k];
-        part[threadIdx.x] = s;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            uint32_t v = threadIdx.x >= (uint32_t)o ? part[threadIdx.x - o] : 0u;
-            __syncthreads();
-            part[threadIdx.x] += v;
-            __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) s += v[k];
+        uint32_t incl = s;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
         }
-        uint32_t run = carry + part[threadIdx.x] - s;
-        for (uint32_t k = 0; k < kPer; ++k) {
-            uint32_t v = tile[threadIdx.x * kPer + k];
-            tile[threadIdx.x * kPer + k] = run;
-            run += v;
-        }
-        carry += part[1023];
+        if (lane == kWave - 1) wsum[w] = incl;
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < cnt; k += 1024) hist[base + k] = tile[k];
+        if (w == 0) {
+            const uint32_t x = lane < (int)kWaves ? wsum[lane] : 0u;
+            uint32_t xi = x;
+#pragma unroll
+            for (int o = 1; o < (int)kWaves; o <<= 1) {
+                const uint32_t t = __shfl_up(xi, o);
+                if (lane >= o) xi += t;
+            }
+            if (lane < (int)kWaves) wsum[lane] = xi - x;
+            if (lane == (int)kWaves - 1) wsum[kWaves] = xi;
+        }
+        __syncthreads();
+        uint32_t run = carry + wsum[w] + incl - s;
+        if (i0 + kPer <= total) {
+            uint4* q = reinterpret_cast<uint4*>(data + i0);
+#pragma unroll
+            for (uint32_t k = 0; k < kPer / 4; ++k) {
+                uint4 x;
+                x.x = run;
+                run += v[4 * k];
+                x.y = run;
+                run += v[4 * k + 1];
+                x.z = run;
+                run += v[4 * k + 2];
+                x.w = run;
+                run += v[4 * k + 3];
+                q[k] = x;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) {
+                if (i0 + k < total) data[i0 + k] = run;
+                run += v[k];
+            }
+        }
+        carry += wsum[kWaves];
         __syncthreads();
     }
 }
