@@ -53,13 +53,17 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
     uint32_t gi;
     while (queue.next(gi)) {
         const uint2 grp = groups[gi];
+        // A group of at most 32 particles takes both halves of the wave: lane l and lane l + 32 hold the same
+        // particle and split each batch of sources in two (two LDS broadcast addresses per step), so the
+        // batch costs half the steps; the halves' sums are added at the end.
+        const bool split = P == 1 && UNI && grp.y <= (uint32_t)(kWave / 2);
         uint32_t t[P];
         bool valid[P];
         float4 me[P];
         float lx0 = INFINITY, ly0 = INFINITY, lz0 = INFINITY, lx1 = -INFINITY, ly1 = -INFINITY, lz1 = -INFINITY;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            const uint32_t slot = (uint32_t)lane + (uint32_t)k * kWave;
+            const uint32_t slot = split ? ((uint32_t)lane & (kWave / 2 - 1)) : (uint32_t)lane + (uint32_t)k * kWave;
             t[k] = grp.x + slot;
             valid[k] = slot < grp.y;
             me[k] = spos[valid[k] ? t[k] : grp.x];
@@ -95,7 +99,8 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
             float bx[P], by[P], bz[P];
 #pragma unroll
             for (int k = 0; k < P; ++k) bx[k] = by[k] = bz[k] = 0.f;
-            for (int j = 0; j < cnt; ++j) {
+            // one source for this lane's particle(s); `in` is false only past the end of an odd split batch
+            auto source = [&](int j, bool in) {
                 const float4 c = cA[j];
                 const uint32_t idx = (UNI && SELF) ? cI[j] : 0u;
                 float4 q0;
@@ -124,11 +129,22 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
                         bz[k] += w * (dz * s - qz * inv2);
                     } else {
                         if (UNI && SELF) w = (idx == t[k]) ? 0.f : w;  // s != t (bh_gravity.wgsl:362)
+                        if (!in) w = 0.f;
                         bx[k] += dx * w;
                         by[k] += dy * w;
                         bz[k] += dz * w;
                     }
                 }
+            };
+            if (split) {
+                const int h = (cnt + 1) >> 1;
+                const int off = lane >= kWave / 2 ? h : 0;
+                for (int j = 0; j < h; ++j) {
+                    const bool in = j + off < cnt;
+                    source(in ? j + off : j, in);
+                }
+            } else {
+                for (int j = 0; j < cnt; ++j) source(j, true);
             }
 #pragma unroll
             for (int k = 0; k < P; ++k) add(k, bx[k], by[k], bz[k]);
@@ -302,6 +318,23 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
         }
         if (nc > 0) flush_cells(nc);
         if (np > 0) flush_parts(np);
+        if (split) {  // lane l adds the upper half's sums (lane l + 32); a compensated sum takes s - c
+            if (COMP) {
+                const float sx = __shfl_down(kx[0].s, kWave / 2), cx = __shfl_down(kx[0].c, kWave / 2);
+                const float sy = __shfl_down(ky[0].s, kWave / 2), cy = __shfl_down(ky[0].c, kWave / 2);
+                const float sz = __shfl_down(kz[0].s, kWave / 2), cz = __shfl_down(kz[0].c, kWave / 2);
+                kx[0].add(sx);
+                kx[0].add(-cx);
+                ky[0].add(sy);
+                ky[0].add(-cy);
+                kz[0].add(sz);
+                kz[0].add(-cz);
+            } else {
+                ax[0] += __shfl_down(ax[0], kWave / 2);
+                ay[0] += __shfl_down(ay[0], kWave / 2);
+                az[0] += __shfl_down(az[0], kWave / 2);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             if (COMP) {
@@ -309,7 +342,7 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
                 ay[k] = ky[k].s;
                 az[k] = kz[k].s;
             }
-            if (valid[k]) sgrav[t[k]] = make_float4(kG * ax[k], kG * ay[k], kG * az[k], 0.f);
+            if (valid[k] && (!split || lane < kWave / 2)) sgrav[t[k]] = make_float4(kG * ax[k], kG * ay[k], kG * az[k], 0.f);
         }
         tot_cell += ncell * grp.y;
         tot_part += npart * grp.y;
