@@ -122,8 +122,11 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     uint32_t gi;
     while (queue.next(gi)) {
     const uint2 grp = groups[gi];
-    const uint32_t t = grp.x + lane;
-    const bool valid = (uint32_t)lane < grp.y;
+    // a group of at most 32 particles runs on both wave halves (pair_mask_half), member = lane mod 32
+    const bool split = grp.y <= (uint32_t)(kWave / 2);
+    const int member = split ? (lane & (kWave / 2 - 1)) : lane;
+    const uint32_t t = grp.x + (uint32_t)member;
+    const bool valid = (uint32_t)member < grp.y;
     const uint32_t tc = valid ? t : grp.x;
     __syncthreads();
     const float4 me = spos[tc];
@@ -153,7 +156,8 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
         __syncthreads();
         // 1) The radius test against every candidate (LDS broadcast reads): one bit per true pair, the
         //    lane's own particle excluded (sph_step.wgsl:210).
-        const uint64_t hit = pair_mask(cP, cnt, me, qi) & ~self_bit(own, cand, cnt, lane, t);
+        const uint64_t hit = (split ? pair_mask_half(cP, cnt, me, qi, lane >= kWave / 2) : pair_mask(cP, cnt, me, qi)) &
+                             ~self_bit(own, cand, cnt, member, t);
         // the force pass of this evaluation replays the chunk with this mask (its pair set is the same)
         if (cw.ok) cc.mask[(size_t)cw.last * kWave + lane] = hit;
         // 2) The kernel sum over this lane's hits only, in candidate order. Lanes walk their own hit lists
@@ -191,7 +195,12 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
         tested += cnt;
     }, sinst ? iroot[sinst[grp.x]] : 0u);
     cw.end(cc, gi, lane);
-    const bool live = valid && me.w > 0.f;  // retired (mass 0) particles keep their density (:185-189)
+    if (split) {  // the upper half's sum joins its member's lane
+        rho += __shfl_down(rho, kWave / 2);
+        if (F64) rho64 += __shfl_down(rho64, kWave / 2);
+    }
+    const bool owner = valid && (!split || lane < kWave / 2);
+    const bool live = owner && me.w > 0.f;  // retired (mass 0) particles keep their density (:185-189)
     double rho_i;
     if (live) {
         if (F64) {
@@ -202,11 +211,11 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
         }
         rho_orig[order[t]] = (float)rho_i;
     } else {
-        rho_i = valid ? (double)rho_orig[order[t]] : 1.0;
+        rho_i = owner ? (double)rho_orig[order[t]] : 1.0;
     }
     // The per-particle EOS runs in its own pass (k_eos): its f64 Tillotson evaluations would otherwise set
     // this walk's register budget and halve the resident waves.
-    if (valid) {
+    if (owner) {
         srho[t] = (float)rho_i;
         if (F64) srho64[t] = rho_i;
     }
@@ -295,8 +304,11 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     uint32_t gi;
     while (queue.next(gi)) {
     const uint2 grp = groups[gi];
-    const uint32_t t = grp.x + lane;
-    const bool valid = (uint32_t)lane < grp.y;
+    // as in the density pass: a group of at most 32 particles runs on both wave halves, member = lane mod 32
+    const bool split = grp.y <= (uint32_t)(kWave / 2);
+    const int member = split ? (lane & (kWave / 2 - 1)) : lane;
+    const uint32_t t = grp.x + (uint32_t)member;
+    const bool valid = (uint32_t)member < grp.y;
     const uint32_t tc = valid ? t : grp.x;
     __syncthreads();
     const float4 me = spos[tc];
@@ -421,7 +433,8 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             note_owner(own, j, grp.x, grp.y, lane);
         }
         __syncthreads();
-        pairs(cnt, pair_mask(cP, cnt, me, qi) & ~self_bit(own, cand, cnt, lane, t));
+        pairs(cnt, (split ? pair_mask_half(cP, cnt, me, qi, lane >= kWave / 2) : pair_mask(cP, cnt, me, qi)) &
+                       ~self_bit(own, cand, cnt, member, t));
     };
     // The density walk of this evaluation stored the group's candidates batch by batch: replay them in the
     // same order (so the sums are the walk's, bit for bit), or walk when the list did not fit.
@@ -470,13 +483,26 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             m1 = m2;
         }
     }
+    if (split) {  // the upper half's sums join their member's lane
+        if (F64) {
+            ax64 += __shfl_down(ax64, kWave / 2);
+            ay64 += __shfl_down(ay64, kWave / 2);
+            az64 += __shfl_down(az64, kWave / 2);
+            de64 += __shfl_down(de64, kWave / 2);
+        } else {
+            ax += __shfl_down(ax, kWave / 2);
+            ay += __shfl_down(ay, kWave / 2);
+            az += __shfl_down(az, kWave / 2);
+            de += __shfl_down(de, kWave / 2);
+        }
+    }
     if (F64) {
         ax = (float)ax64;
         ay = (float)ay64;
         az = (float)az64;
         de = (float)de64;
     }
-    if (valid) {
+    if (valid && (!split || lane < kWave / 2)) {
         float4 gr = sgrav[t];
         // the un-resolved bulk and the external point masses (sph_step.wgsl:242-254), f32 as the shader
         float3 gb = bulk_gravity(make_float3(me.x, me.y, me.z), bnd.bulk_cr, bnd.bulk_vm);

@@ -148,10 +148,25 @@ __device__ __forceinline__ void note_owner(uint32_t* own, uint32_t j, uint32_t g
     const uint32_t d = j - g0;
     if (d < gn) own[d] = (uint32_t)lane;
 }
-__device__ __forceinline__ uint64_t self_bit(const uint32_t* own, const uint32_t* cand, int cnt, int lane,
+__device__ __forceinline__ uint64_t self_bit(const uint32_t* own, const uint32_t* cand, int cnt, int member,
                                              uint32_t t) {
-    const uint32_t ms = own[lane];
+    const uint32_t ms = own[member];
     return (ms < (uint32_t)cnt && cand[ms] == t) ? (1ull << ms) : 0ull;
+}
+// The split form for a group of at most 32 particles held by both wave halves (group member = lane mod
+// 32): the lower half tests slots 0..31 and the upper half slots 32..63 of the batch, each bit landing in
+// its own half of the mask, so a batch costs half the steps; the two lanes of a member later add their
+// sums. The test and its bits are pair_mask's.
+__device__ __forceinline__ uint64_t pair_mask_half(const float4* cP, int cnt, float4 me, float qi, bool upper) {
+    const int base = upper ? 32 : 0, h = min(cnt, 32);
+    uint32_t bits = 0u;
+#pragma unroll 4
+    for (int jj = h - 1; jj >= 0; --jj) {
+        const bool in = jj + base < cnt;
+        const uint32_t sgn = pair_miss_sign(cP[in ? jj + base : jj], me, qi);
+        bits = __builtin_amdgcn_alignbit(bits, in ? sgn : 0u, 31u);
+    }
+    return upper ? ((uint64_t)bits << 32) : (uint64_t)bits;
 }
 
 }  // namespace isph
