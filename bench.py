@@ -24,8 +24,10 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak (packed FP32: two lanes' worth per VALU op)
+FP32_NONPACKED_TFLOPS = 78.6  # the same chip without packed FP32, which this build's HIP code does not emit
 HBM_PEAK_GBS = 8000.0
+DE_BOUND = 1e-4  # the energy bound the headline metric names
 
 
 def parse():
@@ -85,9 +87,41 @@ def measured_traffic():
         return None
 
 
+def host_cores():
+    """Logical CPUs this process may run on (the box's share), and the machine's total."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return avail, os.cpu_count()
+
+
+def cpu_c1_run(steps=100, cfl=0.3):
+    """The CPU path's own workload: the oracle's restatement of HydroBody::step (hydrostatic.rs:338-480,
+    octree gravity theta 0.5, f64) on the 8192-particle two-body scene for `steps` KDK steps at its Courant
+    dt, 1 thread, with the time split into density, gravity and SPH force + du/dt."""
+    from botheadstudios_amd import scene
+    from oracle import oracle
+    oracle.set_threads(1)
+    sc = scene.two_body_8k()
+    pos, vel, mass, u, h, mat = scene.f64_state(sc.particles)
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, sc.eos64, sc.softening)
+    t0 = time.perf_counter()
+    hb.compute_density()
+    for _ in range(steps):
+        hb.step(hb.courant_dt(cfl))
+    wall = time.perf_counter() - t0
+    ph = hb.phase_ms()
+    n = len(mass)
+    return {"workload": f"two-body impact N={n}, {steps} KDK steps at CFL {cfl}, f64, octree theta 0.5",
+            "wall_s": wall, "value": n * steps / wall, "unit": "particle-steps/s", "cores": 1,
+            "ms_per_step": {k: v / steps for k, v in ph.items()}, "ms_per_step_total": wall * 1e3 / steps}
+
+
 def cpu_baseline(sc):
     """One force evaluation (density + forces_and_dudt = half a KDK step) of the oracle's restatement of
-    HydroBody (hydrostatic.rs:338-446, octree gravity theta 0.5) on the same scene, 1 thread."""
+    HydroBody (hydrostatic.rs:338-446, octree gravity theta 0.5) on the same scene, 1 thread; plus the CPU
+    path's own 100-step run on the 8192-particle scene."""
     from botheadstudios_amd import scene
     from oracle import oracle
     oracle.set_threads(1)
@@ -97,10 +131,15 @@ def cpu_baseline(sc):
     hb.compute_density()
     hb.forces()
     dt = time.perf_counter() - t0
+    ph = hb.phase_ms()
+    hb.close()
     n = len(mass)
+    avail, total = host_cores()
     return {"value": n / (2.0 * dt), "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": f"one force evaluation (density + forces_and_dudt, half a KDK step) of the N={n} scene, "
-                      f"{dt:.1f} s on 1 thread of {lscpu_model()}"}
+                      f"{dt:.1f} s on 1 thread of {lscpu_model()}",
+            "host": {"cpu_model": lscpu_model(), "logical_cpus_available": avail, "logical_cpus_machine": total},
+            "eval_ms": ph, "c1_run": cpu_c1_run()}
 
 
 def time_disk_view(s, reps=5):
@@ -256,30 +295,13 @@ def run_4m(args, steps=10):
             "unit": "particle-steps/s"}
 
 
-def batch_scenes(count, seed=0xC4):
-    """The batching workload: `count` impacts, N log-uniform in [16384, 262144] (7:1 target:impactor),
-    angle U(30, 60) deg, speed U(1.0, 1.4) v_esc, impactor / target mass ratio U(0.05, 0.15), core mass
-    fraction U(0.25, 0.35); seeded."""
-    from botheadstudios_amd import scene
-    rng = np.random.default_rng(seed)
-    sizes = np.exp(rng.uniform(np.log(16384), np.log(262144), count)).astype(np.int64)
-    out = []
-    for k in range(count):
-        n = int(sizes[k])
-        ni = n // 8
-        out.append(scene.impact_scene(n - ni, ni, seed + 1 + k, m_i=float(rng.uniform(0.05, 0.15)),
-                                      core_t=float(rng.uniform(0.25, 0.35)), core_i=0.3,
-                                      angle_deg=float(rng.uniform(30.0, 60.0)),
-                                      v_over_vesc=float(rng.uniform(1.0, 1.4))))
-    return out
-
-
 def run_batch(args):
     """All instances as one ensemble (mixed form, captured step graph, autotuned walk shape) against the same
     instances run one after another in the plain form, same number of steps each."""
     import tempfile
     from botheadstudios_amd import isph
-    scs = batch_scenes(args.batch_count)
+    from botheadstudios_amd import scene
+    scs = scene.impact_ensemble(args.batch_count)
     total = sum(len(sc.particles) for sc in scs)
     cache = os.path.join(tempfile.gettempdir(), "isph_autotune.txt")
     with isph.GpuSph(total) as s:
@@ -358,9 +380,10 @@ def main():
                                   if args.gravity == "tree" else "direct-sum gravity, ")
                                + f"SPH 48 neighbours, Tillotson granite/iron, KDK at CFL {args.cfl}",
                    "n": n, "precision": args.precision, "reuse": bool(args.reuse)},
-        "roofline": {"bound": "mfma", "engine": "valu-fp32", "kernel": "k_gravity_tree",
+        "roofline": {"bound": "valu", "engine": "valu-fp32", "kernel": "k_gravity_tree",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
+                     "peak_nonpacked": FP32_NONPACKED_TFLOPS, "frac_nonpacked": achieved / FP32_NONPACKED_TFLOPS,
                      "traffic": measured_traffic() if (n == 1048576 and args.precision == "fp32") else None,
                      "flops_per_launch": flops / launches,
                      "avg_launch_ms": st["gravity_ms"] / launches},
@@ -373,6 +396,7 @@ def main():
     if "dE_rel" in r:
         line["dE_rel"] = r["dE_rel"]
         line["dL_rel"] = r["dL_rel"]
+        line["dE_ok"] = bool(r["dE_rel"] < DE_BOUND)
     line["measure"] = {"impact_state": r["measure"]}
     debris, debris_rec = run_debris(n)
     line["measure"]["debris_field"] = debris
@@ -384,7 +408,8 @@ def main():
         line[v] = {"value": n * args.steps / rv["elapsed"], "ms_per_step": rv["elapsed"] * 1e3 / args.steps,
                    "ratio_to_value": r["elapsed"] / rv["elapsed"],
                    "phases_ms_per_step": {k: x / args.steps for k, x in rv["phases"].items()},
-                   "dE_rel": rv.get("dE_rel"), "dL_rel": rv.get("dL_rel")}
+                   "dE_rel": rv.get("dE_rel"), "dL_rel": rv.get("dL_rel"),
+                   "dE_ok": None if rv.get("dE_rel") is None else bool(rv["dE_rel"] < DE_BOUND)}
     if not args.no_adjoint:
         line["adjoint"] = run_adjoint(args)
     if not args.no_batch:

@@ -622,6 +622,7 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
     return guarded(h, [&]() -> int {
         drop_graph(h);
         if (n > 0 && !particles) return fail(h, ISPH_EINVAL, "particles is NULL");
+        if (n_eos > 0 && !eos) return fail(h, ISPH_EINVAL, "eos is NULL");
         size_t ne = std::min<size_t>(n_eos, kMaxMaterials);  // gpu_sph.rs:1337
         size_t take = std::min<size_t>(n, h->d.cap);          // gpu_store.rs:33
         for (size_t i = 0; i < take; ++i)
@@ -780,6 +781,14 @@ int isph_set_walk(isph_t* h, float group_extent, int grav_particles_per_lane, in
         h->p.grav_ppl = grav_particles_per_lane;
         h->p.sph_expand = sph_expand;
         h->reuse_valid = false;
+        return ISPH_OK;
+    });
+}
+
+int isph_set_candidate_cache(isph_t* h, uint32_t max_blocks) {
+    return guarded(h, [&]() -> int {
+        drop_graph(h);
+        h->p.cache_blocks = max_blocks;
         return ISPH_OK;
     });
 }

@@ -2,7 +2,8 @@
 //
 // pick:   every live particle chooses the neighbour that will absorb it: same material, strictly "greater"
 //         (heavier, or equal mass and lower upload index), closer than half the pair's mean smoothing length
-//         and moving subsonically relative to it (|v_i - v_j| < c_i); the heaviest such neighbour wins.
+//         and moving subsonically relative to it (|v_i - v_j| < c_i); the heaviest such neighbour wins, the
+//         lowest upload index among equally heavy ones (so the choice does not depend on the walk order).
 // apply:  every root (a particle that picked nobody) absorbs the neighbours that picked it: mass, momentum
 //         and centre of mass add up, the destroyed relative kinetic energy becomes internal energy, and h
 //         grows as (m_new / m)^(1/3).
@@ -59,7 +60,9 @@ __global__ __launch_bounds__(64) void k_merge_pick(const float4* __restrict__ sp
                 const float4 vj = svel[j];
                 const float ux = vi.x - vj.x, uy = vi.y - vj.y, uz = vi.z - vj.z;
                 if (!(sqrtf(ux * ux + uy * uy + uz * uz) < ci)) continue;
-                if (pj.w > best_mass || best == kNoMerge) {
+                // the heaviest wins; among equally heavy absorbers the lowest upload index (the reference
+                // keeps the first in its hash-bucket order, which is not deterministic)
+                if (best == kNoMerge || pj.w > best_mass || (pj.w == best_mass && oj < best)) {
                     best = oj;
                     best_mass = pj.w;
                 }

@@ -68,6 +68,10 @@ struct CacheWriter {
     __device__ __forceinline__ void put(const CandCache& cc, uint32_t* scal, const uint32_t* cand, int cnt,
                                         int lane) {
         if (!ok) return;
+        if (blk >= cc.blocks) {  // only below kWalkWaves blocks: a capped cache (isph_set_candidate_cache)
+            ok = false;
+            return;
+        }
         if (used == kChunksPerBlock) {
             uint32_t b = 0;
             if (lane == 0) b = kWalkWaves + atomicAdd(&scal[kPoolHead], 1u);
@@ -811,8 +815,8 @@ __global__ __launch_bounds__(256) void k_diag_pe(uint32_t n, const float4* posh,
 
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-static CandCache cand_cache(const Dev& d) {
-    return CandCache{d.chead, d.cnext, d.ccnt, d.cpool, d.cmask, d.cblocks};
+static CandCache cand_cache(const Dev& d, const StepParams& p) {
+    return CandCache{d.chead, d.cnext, d.ccnt, d.cpool, d.cmask, std::min(d.cblocks, p.cache_blocks)};
 }
 
 static Boundary boundary(const Dev& d, const StepParams& p) {
@@ -824,11 +828,11 @@ void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     if (p.mixed)
         hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nS,
                            d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
-                           (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d));
+                           (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d, p));
     else
         hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nS,
                            d.vals, d.srho, d.rho, (double*)nullptr, d.scal, d.counters, d.groups,
-                           d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d));
+                           d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d, p));
     launch_eos(d, s);
 }
 void launch_eos(const Dev& d, hipStream_t s) {

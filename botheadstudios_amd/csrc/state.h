@@ -145,6 +145,9 @@ struct StepParams {
     uint32_t n_ext = 0;          // external point masses (sph_step.wgsl:249-254)
     float omega = 0.0f;          // relaxation frame spin about +z (sph_step.wgsl:353)
     uint32_t merge_budget = 0;   // pairwise merging while n > budget (sph_step.wgsl:415-541); 0 = never
+    // candidate-cache blocks the density walk may use (capped at what is allocated); a group whose list
+    // does not fit is walked again by the force pass, so a low cap exercises that path
+    uint32_t cache_blocks = 0xffffffffu;
 };
 
 constexpr uint32_t kMaxExtBodies = 32;  // gpu_sph.rs:155

@@ -18,7 +18,9 @@ LIB_PATH = os.environ.get("ISPH_LIBRARY", os.path.join(_HERE, "libisph.so"))
 
 OK = 0
 EAGAIN = -11
+EINVAL = -22
 GRAVITY_TREE, GRAVITY_DIRECT, GRAVITY_NONE = 0, 1, 2
+CACHE_ALL = 0xFFFFFFFF
 PRECISION_FP32, PRECISION_MIXED = 0, 1
 PHASES = ("sort", "tree", "upsweep", "gravity", "density", "force", "integrate")
 
@@ -36,6 +38,7 @@ _SIGS = {
     "isph_set_gravity": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, ctypes.c_int]),
     "isph_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_walk": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int, ctypes.c_int]),
+    "isph_set_candidate_cache": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "isph_set_reuse": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_bulk": (ctypes.c_int, [_P] + [ctypes.c_double] * 8),
@@ -211,6 +214,10 @@ class GpuSph:
     def set_walk(self, group_extent=4.0, grav_particles_per_lane=1, sph_expand=4):
         self._check(self._lib.isph_set_walk(self._h, float(group_extent), int(grav_particles_per_lane),
                                             int(sph_expand)))
+
+    def set_candidate_cache(self, max_blocks=CACHE_ALL):
+        """Cap the SPH candidate-cache blocks in use (0: the force pass walks every group again)."""
+        self._check(self._lib.isph_set_candidate_cache(self._h, int(max_blocks)))
 
     def set_precision(self, mode):
         self._check(self._lib.isph_set_precision(self._h, int(mode)))

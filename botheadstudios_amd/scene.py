@@ -181,6 +181,23 @@ def two_body(n, seed=0xC5):
     return impact_scene(n - n_i, n_i, seed)
 
 
+def impact_ensemble(count=64, seed=0xC4):
+    """The batching workload: `count` independent impacts with N log-uniform in [16384, 262144] (7:1
+    target:impactor), impact angle U(30, 60) deg, speed U(1.0, 1.4) v_esc, impactor / target mass ratio
+    U(0.05, 0.15) and target core mass fraction U(0.25, 0.35); seeded (numpy PCG64 for the draws, each
+    instance's lattice from its own splitmix64 seed)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(16384), np.log(262144), count)).astype(np.int64)
+    out = []
+    for k in range(count):
+        n = int(sizes[k])
+        ni = n // 8
+        out.append(impact_scene(n - ni, ni, seed + 1 + k, m_i=float(rng.uniform(0.05, 0.15)),
+                                core_t=float(rng.uniform(0.25, 0.35)), core_i=0.3,
+                                angle_deg=float(rng.uniform(30.0, 60.0)), v_over_vesc=float(rng.uniform(1.0, 1.4))))
+    return out
+
+
 def f64_state(rec):
     """Widen `SphParticle` records to the f64 arrays the CPU path takes (bit-identical start state)."""
     return (rec["pos"].astype(np.float64), rec["vel"].astype(np.float64), rec["mass"].astype(np.float64),

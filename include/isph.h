@@ -143,6 +143,14 @@ int isph_set_gravity(isph_t* h, int mode, float theta, uint32_t leaf_size, int q
  * Defaults: 4.0, 1, 4. Returns ISPH_EINVAL for group_extent <= 0, grav_particles_per_lane not 1 or 2, or
  * sph_expand outside [1, 64]. */
 int isph_set_walk(isph_t* h, float group_extent, int grav_particles_per_lane, int sph_expand);
+/* Candidate cache of the SPH walks (no reference counterpart). The density walk stores each walk group's
+ * candidate batches and pair masks, in blocks of 32 chunks of 64 candidates (one block per walk wave to
+ * start with, more from a pool), and the force pass of the same evaluation replays them; a group whose
+ * list does not fit is walked again by the force pass with the same result bit for bit. This caps the
+ * blocks in use: ISPH_CACHE_ALL (the default) = every allocated block, 0 = none (every group is walked
+ * twice). It exists to exercise and time the walk fallback. */
+#define ISPH_CACHE_ALL 0xffffffffu
+int isph_set_candidate_cache(isph_t* h, uint32_t max_blocks);
 int isph_set_precision(isph_t* h, int mode);
 /* Reuse the tree, gravity and density of the previous evaluation when positions have not moved since
  * (eval 2 of step n and eval 1 of step n+1 see identical positions). Results are bit-identical. */

@@ -23,6 +23,7 @@
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off -fno-fast-math), output oracle/liboracle.so.
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -369,6 +370,11 @@ struct Hydro {
     // 0 = octree (the reference CPU path), 1 = f64 direct sum, 2 = none (SPH part only), 3 = the reference
     // GPU tree path (f32 LBVH walk, gpu_gravity.rs accelerations_tree)
     int gravity_mode = 0;
+    // accumulated wall milliseconds of the CPU step's phases: density, gravity, SPH force + du/dt
+    mutable double phase_ms[3] = {0.0, 0.0, 0.0};
+    static double now_ms() {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    }
 
     size_t n() const { return pos.size(); }
     double max_h() const {
@@ -378,6 +384,7 @@ struct Hydro {
     }
 
     void compute_density() {  // hydrostatic.rs:338-355
+        const double t0 = now_ms();
         size_t N = n();
         Grid grid;
         grid.build(pos.data(), N, max_h());
@@ -393,6 +400,7 @@ struct Hydro {
             }
         });
         rho.swap(r);
+        phase_ms[0] += now_ms() - t0;
     }
 
     void gravity(std::vector<V3>& acc) const {
@@ -425,7 +433,10 @@ struct Hydro {
 
     void forces_and_dudt(std::vector<V3>& acc, std::vector<double>& dudt) const {  // hydrostatic.rs:401-446
         size_t N = n();
+        const double t0 = now_ms();
         gravity(acc);
+        const double t1 = now_ms();
+        phase_ms[1] += t1 - t0;
         dudt.assign(N, 0.0);
         std::vector<double> p(N), c(N);
         for (size_t i = 0; i < N; ++i) p[i] = till_pressure(eos[mat[i]], rho[i], u[i]);
@@ -456,6 +467,7 @@ struct Hydro {
             dudt[i] += 0.5 * mass[j] * coeff * vdotgrad;
             dudt[j] += 0.5 * mass[i] * coeff * vdotgrad;
         });
+        phase_ms[2] += now_ms() - t1;
     }
 
     void step(double dt) {  // hydrostatic.rs:451-465
@@ -474,6 +486,37 @@ struct Hydro {
         for (size_t i = 0; i < N; ++i) {
             vel[i] = vel[i] + a[i] * (0.5 * dt);
             u[i] = std::max(u[i] + du[i] * (0.5 * dt), 0.0);
+        }
+    }
+
+    // hydrostatic.rs:359-380: self-gravity + the symmetric pressure force only (no viscosity, no du/dt).
+    void accelerations(std::vector<V3>& acc) const {
+        size_t N = n();
+        gravity(acc);
+        std::vector<double> p(N);
+        for (size_t i = 0; i < N; ++i) p[i] = till_pressure(eos[mat[i]], rho[i], u[i]);
+        Grid grid;
+        grid.build(pos.data(), N, max_h());
+        grid.for_each_pair(pos.data(), [&](size_t i, size_t j) {
+            V3 dv = pos[i] - pos[j];
+            double r = length(dv);
+            double hij = 0.5 * (h[i] + h[j]);
+            if (r >= hij || r < 1.0e-9) return;
+            double term = p[i] / (rho[i] * rho[i]) + p[j] / (rho[j] * rho[j]);
+            double dw = sph_dw(r, hij);
+            V3 grad{dv.x / r * dw, dv.y / r * dw, dv.z / r * dw};
+            acc[i] = acc[i] + grad * (-term * mass[j]);
+            acc[j] = acc[j] + grad * (term * mass[i]);
+        });
+    }
+
+    void relax_step(double dt, double damp) {  // hydrostatic.rs:386-393
+        compute_density();
+        std::vector<V3> a;
+        accelerations(a);
+        for (size_t i = 0; i < n(); ++i) {
+            vel[i] = (vel[i] + a[i] * dt) * damp;
+            pos[i] = pos[i] + vel[i] * dt;
         }
     }
 
@@ -1152,6 +1195,22 @@ void orc_hydro_forces(void* hb, double* acc3, double* dudt) {
     }
 }
 void orc_hydro_step(void* hb, double dt) { static_cast<Hydro*>(hb)->step(dt); }
+// Accumulated phase milliseconds of this body's CPU step: [density, gravity, SPH force + du/dt].
+void orc_hydro_phase_ms(void* hb, double* out3) {
+    const Hydro* b = static_cast<Hydro*>(hb);
+    for (int k = 0; k < 3; ++k) out3[k] = b->phase_ms[k];
+}
+void orc_hydro_relax(void* hb, double dt, double damp, uint32_t steps) {
+    for (uint32_t s = 0; s < steps; ++s) static_cast<Hydro*>(hb)->relax_step(dt, damp);
+}
+// Overwrite positions and velocities (the reference's tests edit body.pos / body.vel in place).
+void orc_hydro_set_state(void* hb, const double* pos3, const double* vel3) {
+    Hydro* b = static_cast<Hydro*>(hb);
+    for (size_t i = 0; i < b->n(); ++i) {
+        b->pos[i] = {pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2]};
+        b->vel[i] = {vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2]};
+    }
+}
 double orc_hydro_courant_dt(void* hb, double cfl) { return static_cast<Hydro*>(hb)->courant_dt(cfl); }
 void orc_hydro_get(void* hb, double* pos3, double* vel3, double* u, double* rho) {
     Hydro* b = static_cast<Hydro*>(hb);
@@ -1249,11 +1308,18 @@ void orc_total_energy(uint64_t n, const double* pos3, const double* vel3, const 
         ie += mass[i] * u[i];
     }
     double s2 = softening * softening;
-    for (uint64_t i = 0; i < n; ++i)
+    // one partial sum per row i (over j > i), rows added in index order: the same result at any thread count
+    std::vector<double> row(n, 0.0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (uint64_t i = 0; i < n; ++i) {
+        double r = 0.0;
         for (uint64_t j = i + 1; j < n; ++j) {
             V3 d = pos[i] - pos[j];
-            pe -= G64 * mass[i] * mass[j] / std::sqrt(dot(d, d) + s2);
+            r -= G64 * mass[i] * mass[j] / std::sqrt(dot(d, d) + s2);
         }
+        row[i] = r;
+    }
+    for (uint64_t i = 0; i < n; ++i) pe += row[i];
     out3[0] = ke;
     out3[1] = ie;
     out3[2] = pe;

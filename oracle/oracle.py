@@ -47,6 +47,9 @@ _SIGS = {
     "orc_hydro_compute_density": (None, [_P]),
     "orc_hydro_forces": (None, [_P, _P, _P]),
     "orc_hydro_step": (None, [_P, _D]),
+    "orc_hydro_relax": (None, [_P, _D, _D, ctypes.c_uint32]),
+    "orc_hydro_phase_ms": (None, [_P, _P]),
+    "orc_hydro_set_state": (None, [_P, _P, _P]),
     "orc_hydro_courant_dt": (_D, [_P, _D]),
     "orc_hydro_get": (None, [_P, _P, _P, _P, _P]),
     "orc_hydro_eos": (None, [_P, _P, _P]),
@@ -296,6 +299,22 @@ class Hydro:
 
     def courant_dt(self, cfl):
         return lib().orc_hydro_courant_dt(self.h, float(cfl))
+
+    def phase_ms(self):
+        """Accumulated wall ms of the CPU step's phases: density, gravity, SPH force + du/dt."""
+        out = np.zeros(3)
+        lib().orc_hydro_phase_ms(self.h, _p(out))
+        return {"density": out[0], "gravity": out[1], "sph_force": out[2]}
+
+    def relax(self, dt, damp, steps=1):
+        """`steps` damped relaxation steps (HydroBody::relax_step, hydrostatic.rs:386-393)."""
+        lib().orc_hydro_relax(self.h, float(dt), float(damp), int(steps))
+
+    def set_state(self, pos, vel):
+        p = c64(pos).reshape(-1, 3)
+        v = c64(vel).reshape(-1, 3)
+        assert p.shape[0] == self.n and v.shape[0] == self.n
+        lib().orc_hydro_set_state(self.h, _p(p), _p(v))
 
     def state(self):
         pos = np.zeros((self.n, 3))

@@ -100,3 +100,48 @@ def hydro_from(rec, eos64, softening, gravity="none"):
     hb = oracle.Hydro(pos, vel, mass, u, h, mat, eos64, softening)
     hb.configure(theta=0.5, av_alpha=1.0, av_beta=2.0, gravity=gravity)
     return hb
+
+
+def new_sphere(eos10, total_mass, temp_k, specific_heat, n):
+    """HydroBody::new_sphere (hydrostatic.rs:78-101): n equal-mass particles on a Fibonacci-sphere spiral at
+    uniform number density, u = c T, h = smoothing_for(m, rho0), softening half a mean spacing, rho = rho0.
+    Returns (pos, vel, mass, u, h, softening) in f64."""
+    from botheadstudios_amd import bodies
+    rho0 = eos10[0]
+    r0 = np.cbrt(total_mass / (4.0 / 3.0 * np.pi * rho0))
+    m_i = total_mass / n
+    i = np.arange(n, dtype=np.float64)
+    pos = bodies.fib_dir(i, n, 0.0) * (r0 * np.cbrt((i + 0.5) / n))[:, None]
+    h = np.full(n, bodies.smoothing_for(m_i, rho0))
+    return (pos, np.zeros((n, 3)), np.full(n, m_i), np.full(n, specific_heat * temp_k), h,
+            0.5 * np.cbrt(m_i / rho0))
+
+
+def head_on_scene():
+    """The initial state of `a_head_on_collision_conserves_energy_and_shock_heats` (hydrostatic.rs:1354-1396):
+    two identical 600-particle basalt spheres of radius 400 km at 300 K, each relaxed for 1500 damped steps
+    (dt = relax_dt(0.2), damping 0.94) on the f64 CPU step, then placed at x = -/+2.2 r0 with their x
+    velocities set to +/-1.5 km/s (their relaxed y, z velocities kept). Both bodies are built and relaxed
+    identically, so one relaxation serves both. Returns (pos, vel, mass, u, h, mat, eos64, softening)."""
+    from botheadstudios_amd import materials
+    from oracle import oracle
+    eos = np.array([materials.BASALT], np.float64)
+    r0 = 4.0e5
+    m_body = 4.0 / 3.0 * np.pi * r0 ** 3 * eos[0][0]
+    pos, vel, mass, u, h, soft = new_sphere(eos[0], m_body, 300.0, materials.SPECIFIC_HEAT["basalt"], 600)
+    mat = np.zeros(600, np.uint32)
+    hb = oracle.Hydro(pos, vel, mass, u, h, mat, eos, soft)
+    # relax_dt (hydrostatic.rs:483-492): cfl * min h / max(c(rho0, u0), 1)
+    dt_relax = 0.2 * h.min() / max(np.sqrt(oracle.till_c2(eos[0], eos[0][0], u[0])), 1.0)
+    hb.relax(dt_relax, 0.94, 1500)
+    p, v, _, _ = hb.state()
+    hb.close()
+    sep, v_app = 2.2 * r0, 1500.0
+    pa, va = p.copy(), v.copy()
+    pa[:, 0] -= sep
+    va[:, 0] = v_app
+    pb, vb = p.copy(), v.copy()
+    pb[:, 0] += sep
+    vb[:, 0] = -v_app
+    cat = np.concatenate
+    return (cat([pa, pb]), cat([va, vb]), cat([mass, mass]), cat([u, u]), cat([h, h]), cat([mat, mat]), eos, soft)

@@ -110,3 +110,39 @@ def test_autotune_picks_and_caches_a_walk_shape(tmp_path):
     assert from_cache
     assert (t2.leaf_size, t2.group_extent, t2.grav_particles_per_lane, t2.sph_expand) == \
         (t.leaf_size, t.group_extent, t.grav_particles_per_lane, t.sph_expand)
+
+
+@pytest.fixture(scope="module")
+def batch_workload():
+    return scene.impact_ensemble(64)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", [isph.PRECISION_FP32, isph.PRECISION_MIXED])
+def test_the_64_impact_workload_reproduces_every_standalone_run(batch_workload, precision):
+    """The batching workload at its real sizes: 64 impacts, N log-uniform in [16384, 262144] (5.1 M
+    particles in all), angle, speed, mass ratio and core fraction drawn per instance. Two adaptive steps of
+    the ensemble equal, for every instance bit for bit, two steps of that instance alone (positions,
+    velocities, u, density, simulated time and dt)."""
+    scs = batch_workload
+    total = sum(len(sc.particles) for sc in scs)
+    print(f"ensemble of {len(scs)} instances, {total} particles", flush=True)
+    with isph.GpuSph(total) as s:
+        off = s.upload_ensemble([sc.particles for sc in scs], EOS, scs[0].softening)
+        s.set_eos_f64(EOS64)
+        s.set_precision(precision)
+        s.step_cfl(0.3, 2)
+        out = s.readback_sync()
+        t, dt = s.ensemble_times()
+    for k, sc in enumerate(scs):
+        with isph.GpuSph(len(sc.particles)) as s:
+            s.upload(sc.particles, EOS, sc.softening)
+            s.set_eos_f64(EOS64)
+            s.set_precision(precision)
+            s.step_cfl(0.3, 2)
+            rec = s.readback_sync()
+            t_alone, dt_alone = s.sim_time()
+        part = out[off[k]:off[k + 1]]
+        for f in ("pos", "vel", "u", "rho"):
+            assert np.array_equal(part[f], rec[f]), (k, len(rec), f)
+        assert t[k] == t_alone and dt[k] == dt_alone, k

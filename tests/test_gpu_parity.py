@@ -6,6 +6,8 @@ size). Tolerance: gravity against the f64 direct sum within the reference's own 
 against the f64 CPU path (hydrostatic.rs:338-446) within 1e-5 relative per particle (denominator floored
 at 1e-3 of the RMS magnitude), KDK trajectories and the device Courant dt against the CPU step.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -121,6 +123,44 @@ def test_karras_topology_bit_exact(name, K):
     com = (p * m[:, None]).sum(0) / m.sum()
     assert np.linalg.norm(root[:3] - com) / max(np.linalg.norm(com), 1.0) < 1e-4
     assert np.array_equal(t["bmin"][0, :3], olo) and np.array_equal(t["bmax"][0, :3], ohi)
+
+
+SCENES = {"c1_8192": scene.two_body_8k, "two_body_1m": scene.two_body_1m}
+
+
+@pytest.mark.parametrize("which", list(SCENES))
+def test_sort_tree_and_moments_on_the_impact_scenes(which):
+    """The check the plain GPU workload states (bit-exact sort and tree against the CPU leg,
+    gpu_gravity.rs:351-388 + bh_gravity.wgsl:122-241) on the 8192-particle scene and on the 1,048,576-
+    particle scene itself, where the radix sort runs 256 tiles per pass: codes, the sorted (code, index)
+    permutation and left/right/parent equal the oracle's bit for bit. Every node's moments against the
+    oracle's upsweep: AABB bit for bit, mass and COM to a few f32 ulps (the device climbs level by level
+    with contracted multiply-adds, the oracle node by node without)."""
+    sc = SCENES[which]()
+    rec = sc.particles
+    print(f"{which}: N={len(rec)}", flush=True)
+    with _sim(rec, sc.softening) as s:
+        s.set_gravity(isph.GRAVITY_TREE, 0.5, 1)
+        codes, lo, hi = s.probe_morton()
+        scodes, order = s.probe_sort()
+        t = s.probe_tree(1)
+    olo, ohi = oracle.bbox_f32(rec["pos"])
+    ocodes = oracle.morton_f32(rec["pos"], olo, ohi)
+    assert np.array_equal(codes, ocodes)
+    osc, oorder = oracle.sort_pairs(ocodes)
+    assert np.array_equal(scodes, osc)
+    assert np.array_equal(order, oorder)
+    ol, orr, op = oracle.karras(oracle.cluster_codes(osc, 1))
+    assert np.array_equal(t["left"], ol) and np.array_equal(t["right"], orr) and np.array_equal(t["parent"], op)
+    sb = np.concatenate([rec["pos"][oorder], rec["mass"][oorder, None]], axis=1).astype(np.float32)
+    com, bmin, bmax = oracle.upsweep_f32(sb, 1, ol, orr)
+    assert np.array_equal(t["bmin"][:, :3], bmin) and np.array_equal(t["bmax"][:, :3], bmax)
+    m_err = np.abs(t["com"][:, 3].astype(np.float64) / com[:, 3] - 1.0).max()
+    # COM error in units of the node's largest coordinate magnitude (the f32 resolution of its particles)
+    scale = np.maximum(np.abs(bmin), np.abs(bmax)).max(axis=1, keepdims=True).astype(np.float64)
+    c_err = (np.abs(t["com"][:, :3].astype(np.float64) - com[:, :3]) / np.maximum(scale, 1e-30)).max()
+    print(f"{which}: node mass max rel {m_err:.2e}, COM max rel to the node's coordinates {c_err:.2e}")
+    assert m_err < 2e-6 and c_err < 2e-6
 
 
 def test_gravity_direct_matches_cpu_sum():
@@ -354,6 +394,11 @@ def test_readback_semantics_and_clamps(c1):
         bad["mat"][3] = 7
         with pytest.raises(isph.IsphError):
             s.upload(bad, EOS, c1.softening)
+        # a NULL EOS table with a non-zero length is an argument error, not a dereference
+        lib = isph.load_library()
+        good = np.ascontiguousarray(rec[:10])
+        assert lib.isph_upload(s._h, good.ctypes.data, 10, None, 2, ctypes.c_float(c1.softening)) == isph.EINVAL
+        assert "eos" in lib.isph_last_error(s._h).decode()
 
 
 def test_retired_particles_are_inert(c1):
@@ -384,6 +429,34 @@ def test_reuse_is_bit_identical(c1, precision):
         assert np.array_equal(a[f], b[f]), f
     assert ta == tb
     assert da["ke"] == db["ke"] and da["ie"] == db["ie"]
+
+
+@pytest.mark.parametrize("extent", [4.0, 1.0])
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_candidate_cache_overflow_walks_to_the_same_bits(c1, precision, extent):
+    """When a walk group's candidate list does not fit the cache the force pass walks the tree again
+    instead of replaying the list. With the cache capped at one block per walk wave (groups that outgrow
+    their wave's block overflow) and at none (every group is walked twice), density, acceleration and
+    du/dt equal the fully cached evaluation bit for bit, and the force walk did run. Walk-group extents 4
+    and 1 (times the largest h) give groups of up to 64 particles and groups of at most 32 (which split
+    each candidate batch over both wave halves)."""
+    rec = with_random_velocities(c1)
+    outs = {}
+    for cap in (isph.CACHE_ALL, 8192, 0):
+        with _sim(rec, c1.softening) as s:
+            s.set_precision(isph.PRECISION_MIXED if precision == "mixed" else isph.PRECISION_FP32)
+            s.set_walk(extent, 1, 4)
+            s.set_candidate_cache(cap)
+            s.reset_stats()
+            outs[cap] = s.force_eval() + (s.walk_stats()["force_rounds"],)
+    full = outs[isph.CACHE_ALL]
+    assert full[3] == 0.0
+    print(f"{precision} extent {extent}: force walk rounds {outs[8192][3]:.0f} (one block per wave), "
+          f"{outs[0][3]:.0f} (none)")
+    assert outs[0][3] > 0.0
+    for cap in (8192, 0):
+        for k in range(3):
+            assert np.array_equal(outs[cap][k], full[k]), (cap, k)
 
 
 _C1_STEPS = 100

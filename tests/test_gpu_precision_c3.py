@@ -1,0 +1,174 @@
+"""The precision workload against the f64 oracle: N = 65,536 over 500 steps, and the full size N = 4,194,304.
+
+65,536 particles (the oracle's size for this workload; the CPU run is the committed fixture written by
+tests/golden/make_c3_fixture.py, since 500 f64 CPU steps take the better part of an hour):
+  * run-level drift: the device takes the CPU run's own dt sequence for 500 KDK steps in each precision
+    form; the energy drift (KE + IE + exact softened PE) and the angular-momentum change are compared
+    with the CPU run's every 25 steps. With exact gravity on both sides the difference is precision
+    alone: |dE_gpu - dE_cpu| and |dL_gpu - dL_cpu| <= 1e-4 (relative to |E0|, |L0|) at every checkpoint.
+    With the tree on both sides (LBVH group walk against the CPU path's octree, theta 0.5) the two
+    approximations differ and the bound is 1e-3.
+  * per-quantity error at the state after the 500 steps (where pressure waves make du/dt non-zero): the
+    device evaluated on the read-back state against the oracle evaluated on the same bits: density, SPH
+    acceleration and du/dt per particle (floored relative error) and gravity against the f64 direct sum.
+4,194,304 particles (no f64 reference at this size): two adaptive steps per form stay finite, the SPH part
+conserves momentum and the pair work balances du/dt, and the two forms agree on density.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from botheadstudios_amd import isph, materials, scene
+from oracle import oracle
+
+from helpers import hydro_from, rel_err_floor
+
+pytestmark = pytest.mark.gpu
+
+EOS = materials.eos_table()
+EOS64 = materials.eos_f64()
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_cpu_run_65k.json")
+FORMS = {"fp32": isph.PRECISION_FP32, "mixed": isph.PRECISION_MIXED}
+# per-quantity bounds at the state after 500 steps (max over particles of the floored relative error):
+# the same statement as on the 8192-particle scene
+BOUNDS = {"fp32": {"rho": 1e-5, "acc": 1e-4, "dudt": 1e-3}, "mixed": {"rho": 1e-5, "acc": 1e-5, "dudt": 1e-5}}
+
+
+@pytest.fixture(scope="module")
+def c3():
+    if not os.path.exists(FIXTURE):
+        pytest.skip("the CPU-run fixture is written by tests/golden/make_c3_fixture.py")
+    with open(FIXTURE) as f:
+        fix = json.load(f)
+    sc = scene.two_body(fix["scene"]["n"], seed=fix["scene"]["seed"])
+    assert len(sc.particles) == fix["scene"]["n"] and sc.softening == fix["scene"]["softening"]
+    return fix, sc
+
+
+def _device_run(sc, dts, every, form, gravity):
+    """The device over the given dt sequence; (dE, dL) every `every` steps, plus the final records."""
+    series = []
+    with isph.GpuSph(len(sc.particles)) as s:
+        s.upload(sc.particles, EOS, sc.softening)
+        s.set_eos_f64(EOS64)
+        s.set_precision(FORMS[form])
+        s.set_gravity(isph.GRAVITY_DIRECT if gravity == "direct" else isph.GRAVITY_TREE, 0.5, 1)
+        d0 = s.diagnostics(with_pe=True)
+        e0 = d0["ke"] + d0["ie"] + d0["pe"]
+        for k, dt in enumerate(dts, start=1):
+            s.set_dt(dt, 1.0)
+            s.encode_kdk(1)
+            if k % every == 0:
+                d = s.diagnostics(with_pe=True)
+                series.append(((d["ke"] + d["ie"] + d["pe"] - e0) / abs(e0),
+                               (d["L"] - d0["L"]) / np.linalg.norm(d0["L"])))
+        return series, s.readback_sync()
+
+
+@pytest.fixture(scope="module")
+def runs(c3):
+    fix, sc = c3
+    out = {}
+    for gravity in ("direct", "octree"):
+        for form in FORMS:
+            print(f"device run: {gravity} gravity, {form}", flush=True)
+            out[(gravity, form)] = _device_run(sc, fix["runs"][gravity]["dts"], fix["every"], form, gravity)
+    return out
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("gravity,bound", [("direct", 1e-4), ("octree", 1e-3)])
+@pytest.mark.parametrize("form", list(FORMS))
+def test_drift_tracks_the_f64_cpu_run(c3, runs, gravity, bound, form):
+    fix, _ = c3
+    cpu = fix["runs"][gravity]["series"][1:]
+    dev, _ = runs[(gravity, form)]
+    assert len(dev) == len(cpu)
+    de = max(abs(g[0] - c["dE_rel"]) for g, c in zip(dev, cpu))
+    dl = max(float(np.linalg.norm(g[1] - np.array(c["dL_rel"]))) for g, c in zip(dev, cpu))
+    print(f"{gravity} {form}: dE after 500 steps device {dev[-1][0]:.6e} cpu {cpu[-1]['dE_rel']:.6e}; "
+          f"max |dE diff| {de:.2e}, max |dL diff| {dl:.2e}")
+    assert de <= bound
+    if gravity == "direct":
+        assert dl <= bound
+
+
+def _maxrms(x, ref):
+    e = rel_err_floor(x, ref)
+    return float(e.max()), float(np.sqrt(np.mean(e ** 2)))
+
+
+@pytest.mark.timeout(600)
+def test_per_quantity_error_after_500_steps(c3, runs):
+    """Device against oracle on the state the mixed form reached after 500 steps with exact gravity."""
+    _, sc = c3
+    rec = runs[("direct", "mixed")][1]
+    hb = hydro_from(rec, EOS64, sc.softening, gravity="none")
+    hb.compute_density()
+    acc_ref, dudt_ref = hb.forces()
+    rho_ref = hb.state()[3]
+    pos, vel, mass, u, h, mat = scene.f64_state(rec)
+    g_ref = oracle.direct_gravity(pos, mass, sc.softening)
+    assert np.sqrt(np.mean(dudt_ref ** 2)) > 0.0
+    report = {}
+    for form, prec in FORMS.items():
+        with isph.GpuSph(len(rec)) as s:
+            s.upload(rec, EOS, sc.softening)
+            s.set_eos_f64(EOS64)
+            s.set_precision(prec)
+            s.set_gravity(isph.GRAVITY_NONE, 0.5, 1)
+            acc, dudt, rho = s.force_eval()
+            g = s.gravity_accelerations(pos, mass, sc.softening, isph.GRAVITY_DIRECT)
+            s.set_gravity(isph.GRAVITY_TREE, 0.5, 1)
+            gt = s.gravity_accelerations(pos, mass, sc.softening, isph.GRAVITY_TREE)
+        r = {"rho": _maxrms(rho, rho_ref), "acc": _maxrms(acc, acc_ref), "dudt": _maxrms(dudt, dudt_ref),
+             "gravity_direct": _maxrms(g, g_ref), "gravity_tree": _maxrms(gt, g_ref)}
+        report[form] = r
+        print(form, {k: f"{v[0]:.2e}/{v[1]:.2e}" for k, v in r.items()})
+        for q, b in BOUNDS[form].items():
+            assert r[q][0] <= b, (form, q, r[q])
+        assert r["gravity_direct"][0] < 1e-4
+    assert report["mixed"]["rho"][0] <= report["fp32"]["rho"][0]
+
+
+@pytest.mark.timeout(600)
+def test_full_size_properties():
+    """N = 4,194,304: both forms take two adaptive steps; on the state reached, the SPH accelerations sum
+    to zero momentum and the pair work balances du/dt (sum m v.a_SPH + sum m du/dt = 0), to the form's
+    rounding; plain and mixed density agree to 1e-5."""
+    sc = scene.two_body(4194304, seed=0xC3)
+    n = len(sc.particles)
+    with isph.GpuSph(n) as s:
+        s.upload(sc.particles, EOS, sc.softening)
+        s.set_eos_f64(EOS64)
+        s.step_cfl(0.3, 2)
+        rec = s.readback_sync()
+    for f in ("pos", "vel", "u", "rho"):
+        assert np.isfinite(rec[f]).all(), f
+    m = rec["mass"].astype(np.float64)
+    v = rec["vel"].astype(np.float64)
+    res = {}
+    for form, prec in FORMS.items():
+        with isph.GpuSph(n) as s:
+            s.upload(rec, EOS, sc.softening)
+            s.set_eos_f64(EOS64)
+            s.set_precision(prec)
+            s.step_cfl(0.3, 2)
+            out = s.readback_sync()
+            assert np.isfinite(out["pos"]).all() and np.isfinite(out["u"]).all()
+            s.upload(rec, EOS, sc.softening)
+            s.set_gravity(isph.GRAVITY_NONE, 0.5, 1)
+            acc, dudt, rho = s.force_eval()
+        acc = acc.astype(np.float64)
+        mom = np.abs((m[:, None] * acc).sum(0)).max() / (m[:, None] * np.abs(acc)).sum()
+        work = abs((m[:, None] * v * acc).sum() + (m * dudt).sum()) / (
+            (m[:, None] * np.abs(v * acc)).sum() + (m * np.abs(dudt)).sum())
+        res[form] = rho
+        print(f"4M {form}: momentum {mom:.2e}, energy balance {work:.2e}")
+        assert mom < (1e-5 if form == "fp32" else 1e-9)
+        assert work < (1e-4 if form == "fp32" else 1e-9)
+    d = np.abs(res["fp32"].astype(np.float64) / res["mixed"] - 1.0).max()
+    print(f"4M density plain vs mixed: max rel {d:.2e}")
+    assert d < 1e-5

@@ -184,3 +184,61 @@ def test_particle_buffer_holds_the_packed_records():
         rb = s.readback_sync()
     assert host.tobytes() == rb.tobytes()
     assert not np.array_equal(rb["pos"], sc.particles["pos"])  # the steps moved the state
+
+
+def _equal_mass_cloud():
+    """12^3 equal-mass basalt particles at rest on a 1 m lattice jittered by +/-0.4 m (seeded), h = 2 m, hot
+    (u = 4e6 J/kg, below E_iv) and at about rho0 inside, so that many pairs are closer than half their mean
+    h and most particles have a sound speed (the thinned edges are in tension)."""
+    rng = scene.SplitMix64(0xE0A1)
+    g = np.stack(np.meshgrid(*[np.arange(12.0)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = (g + (rng.uniform(g.size).reshape(-1, 3) * 0.8 - 0.4)).astype(np.float32)
+    rec = np.zeros(len(pos), dtype=scene.PARTICLE_DTYPE)
+    rec["pos"] = pos
+    rec["h"] = 2.0
+    rec["u"] = 4.0e6
+    rec["mass"] = 2700.0
+    rec["rho"] = 2700.0
+    return rec
+
+
+def test_merge_pick_among_equal_masses_is_the_lowest_index():
+    """cs_merge_pick (sph_step.wgsl:421-470) on equal masses: the absorber must be "greater" (here: a lower
+    index) and closer than half the pair's mean h; of several, this build takes the lowest index (the
+    reference takes the first in its hash-bucket order). At rest the subsonic gate |v_i - v_j| < c_i passes
+    wherever c_i > 0 (c from the oracle's f64 density and Tillotson sound speed). Brute force over all pairs
+    gives every particle's target, hence the retired set (a particle whose target picked nobody); one merge
+    pass on the device must retire exactly that set under two different walk shapes, with the same merged
+    masses."""
+    from oracle import oracle
+    rec = _equal_mass_cloud()
+    p = rec["pos"].astype(np.float32)
+    d = p[:, None, :] - p[None, :, :]
+    r = np.sqrt((d * d).sum(-1, dtype=np.float32))
+    close = r < np.float32(0.5) * np.float32(2.0)
+    n = len(rec)
+    f64 = lambda k: rec[k].astype(np.float64)
+    rho = oracle.brute_sph(f64("pos"), f64("vel"), f64("mass"), f64("u"), f64("h"), rec["mat"],
+                           materials.eos_f64(("basalt",)))[0]
+    c = np.sqrt([oracle.till_c2(materials.BASALT, rr, 4.0e6) for rr in rho])
+    assert (c > 0).sum() > n // 2
+    idx = np.arange(n)
+    elig = close & (idx[None, :] < idx[:, None]) & (c[:, None] > 0)
+    target = np.where(elig.any(1), np.argmax(elig, axis=1), -1)
+    retired = (target >= 0) & (target[np.maximum(target, 0)] < 0)
+    assert 50 < retired.sum() < n // 2
+    outs = []
+    for extent, expand in ((4.0, 4), (1.5, 1)):
+        with isph.GpuSph(n) as s:
+            s.upload(rec, materials.eos_table(("basalt",)), SOFT)
+            s.set_eos_f64(materials.eos_f64(("basalt",)))
+            s.set_walk(extent, 1, expand)
+            s.set_merge_budget(1)
+            s.encode_merge()
+            outs.append(s.readback_sync())
+    for out in outs:
+        assert np.array_equal(out["mass"] == 0.0, retired)
+    assert np.array_equal(outs[0]["mass"], outs[1]["mass"])
+    # each root holds its own mass plus that of the particles it absorbed
+    absorbed = np.bincount(target[retired], minlength=n)
+    assert np.array_equal(outs[0]["mass"][~retired], (2700.0 * (1 + absorbed[~retired])).astype(np.float32))
