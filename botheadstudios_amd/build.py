@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libisph.so")
-SOURCES = ["sort.hip", "tree.hip", "gravity.hip", "sph.hip", "adjoint.hip", "ensemble.hip", "merge.hip", "measure.hip", "isph.cpp"]
+SOURCES = ["sort.hip", "tree.hip", "gravity.hip", "sph.hip", "adjoint.hip", "ensemble.hip", "merge.hip", "measure.hip", "noalloc.hip", "isph.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Werror=return-type", "-Werror=uninitialized",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

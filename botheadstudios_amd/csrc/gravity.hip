@@ -447,6 +447,12 @@ void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s) {
         hipLaunchKernelGGL(k_gravity_direct<false>, dim3((d.n + 255) / 256), dim3(256), 0, s, d.spos, d.n, p.soft2,
                            d.sgrav);
 }
+void launch_gravity_direct_raw(const float4* bodies, uint32_t n, float soft2, float4* out, bool comp, hipStream_t s) {
+    if (comp)
+        hipLaunchKernelGGL(k_gravity_direct<true>, dim3((n + 255) / 256), dim3(256), 0, s, bodies, n, soft2, out);
+    else
+        hipLaunchKernelGGL(k_gravity_direct<false>, dim3((n + 255) / 256), dim3(256), 0, s, bodies, n, soft2, out);
+}
 void launch_gravity_zero(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_fill4, dim3((d.n + 255) / 256), dim3(256), 0, s, d.sgrav, d.n, make_float4(0.f, 0.f, 0.f, 0.f));
 }

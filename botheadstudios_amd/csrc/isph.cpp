@@ -36,6 +36,14 @@ struct HipError : std::runtime_error {
             throw HipError(std::string(#call) + ": " + hipGetErrorString(e_) + " (" + std::to_string(e_) + ")"); \
     } while (0)
 
+}  // namespace
+
+namespace isph {
+void set_thread_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace isph
+
+namespace {
+
 struct EvPair {
     int phase;
     hipEvent_t a, b;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <algorithm>
+#include <string>
 
 #include "common.h"
 
@@ -273,6 +274,11 @@ void launch_radix_pass_n(uint32_t n, uint32_t* hist, const uint32_t* kin, const 
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s);
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s);
+// The exact sum on explicit arrays: bodies (x, y, z, m) in, kG-scaled accelerations out, compensated sums
+// when `comp`; no handle, no allocation.
+void launch_gravity_direct_raw(const float4* bodies, uint32_t n, float soft2, float4* out, bool comp, hipStream_t s);
+// The calling thread's error message for entry points that take no handle (read by isph_last_error(NULL)).
+void set_thread_error(const std::string& msg);
 void launch_gravity_zero(const Dev& d, hipStream_t s);
 void launch_scatter_gravity(const Dev& d, hipStream_t s);  // sgrav (sorted) -> acc (orig), for the gravity API
 

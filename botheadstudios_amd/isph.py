@@ -39,6 +39,7 @@ _SIGS = {
     "isph_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_walk": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_int, ctypes.c_int]),
     "isph_set_candidate_cache": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "isph_gravity_direct_noalloc": (ctypes.c_int, [_P, _P, ctypes.c_size_t, ctypes.c_double, _P, _P]),
     "isph_set_reuse": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_graph": (ctypes.c_int, [_P, ctypes.c_int]),
     "isph_set_bulk": (ctypes.c_int, [_P] + [ctypes.c_double] * 8),

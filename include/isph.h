@@ -83,7 +83,8 @@ typedef struct isph_ctx isph_t;
  * gpu_host.rs:31). Capacity is clamped to >= 1 like the reference (:1117). NULL on failure. */
 isph_t* isph_create(uint32_t capacity, int device);
 void isph_destroy(isph_t* h);
-/* Last error message of `h` (or of the calling thread's last isph_create when h is NULL). */
+/* Last error message of `h` (or, when h is NULL, of the calling thread's last isph_create or handle-less
+ * call). */
 const char* isph_last_error(const isph_t* h);
 
 /* GpuSph::upload(queue, particles, eos, softening) gpu_sph.rs:1323-1340: particles beyond capacity are
@@ -213,6 +214,17 @@ int isph_force_eval(isph_t* h, float* acc4, float* rho);
  * out. n must not exceed the handle's capacity. Does not touch the stepper's particle state. */
 int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass, size_t n, double softening,
                                double* acc3, int mode);
+
+/* No counterpart in the reference: the direct mode of isph_gravity_accelerations for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Takes no handle; pos3[3n], mass[n] and acc3[3n] are DEVICE arrays of the current device,
+ * f64 in and out, f32 compensated sums on the device (bodies (x, y, z, m) rounded to f32 as the allocating
+ * call does). Enqueued on `stream` (a hipStream_t, NULL = the null stream); returns without waiting and
+ * calls no allocator, memset or synchronisation, so it can be captured. Its staging lives in a fixed
+ * per-device scratch array of 32 MiB held until the process ends: n <= 1,048,576 (ISPH_ERANGE above);
+ * calls that overlap on one GPU share that array and race. Errors: isph_last_error(NULL). */
+int isph_gravity_direct_noalloc(const double* pos3, const double* mass, size_t n, double softening, double* acc3,
+                                void* stream);
 
 /* ---- diagnostics ---------------------------------------------------------------------------------- */
 /* Totals in f64 over the live particles: out[0]=KE, [1]=IE, [2]=softened PE (exact O(N^2) device sum
