@@ -57,6 +57,11 @@ def parse():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint/forward cost measurement and print its object (for profiling it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-precision-study", action="store_true",
+                    help="skip the precision-attributable energy error of the plain and mixed forms")
+    ap.add_argument("--study-steps", type=int, default=200, help="KDK steps of the precision study")
+    ap.add_argument("--drift-study", action="store_true",
+                    help="run only the energy-drift decomposition (time step, gravity, clamp) and print it")
     ap.add_argument("--no-energy", action="store_true", help="skip the O(N^2) energy diagnostics")
     ap.add_argument("--variants", default="mixed,reuse",
                     help="comma list of extra timed runs of the same workload: mixed (precision form), reuse "
@@ -210,6 +215,7 @@ def run_gpu(sc, args, precision, steps, warmup, reuse, measure=False):
     phases = s.phase_times()
     stats = s.kernel_stats()
     stats.update(s.walk_stats())
+    clamp = s.clamp_energy()
     s.set_timing(False)
     t_sim1, dt_last = s.sim_time()
     d1 = None if args.no_energy else s.diagnostics(with_pe=True)
@@ -228,6 +234,7 @@ def run_gpu(sc, args, precision, steps, warmup, reuse, measure=False):
         l0 = np.linalg.norm(d0["L"])
         out["dE_rel"] = abs(e1 - e0) / abs(e0)
         out["dL_rel"] = float(np.linalg.norm(d1["L"] - d0["L"]) / l0)
+        out["clamp_energy_rel"] = clamp / abs(e0)
     return out
 
 
@@ -271,6 +278,98 @@ def run_adjoint(args):
             "adjoint_over_forward": rev / fwd, "plain_fp32_forward_ms": plain_ms,
             "adjoint_over_plain_forward": rev / plain_ms,
             "granite_dloss_d_a_b_A_B": [float(x) for x in g]}
+
+
+def _energy(d):
+    return d["ke"] + d["ie"] + d["pe"]
+
+
+def run_precision_study(args, sc):
+    """Precision-attributable energy error: the plain and mixed forms run the same discrete step as the f64
+    reference form (LBVH theta, leaf 1, monopole, one particle per lane, the reference run's own Courant dt
+    sequence) from the same state. A form's energy minus the reference's after the run is what that form's
+    rounding costs, relative to |E0|; the same for the angular momentum about the COM."""
+    from botheadstudios_amd import isph
+    n = len(sc.particles)
+    steps = args.study_steps
+
+    def handle(form):
+        s = isph.GpuSph(n)
+        s.upload(sc.particles, sc.eos, sc.softening)
+        s.set_eos_f64(sc.eos64)
+        s.set_gravity(isph.GRAVITY_TREE, args.theta, 1)
+        s.set_walk(args.group_extent, 1, args.expand)
+        s.set_precision(form)
+        return s
+
+    res = {}
+    with handle(isph.PRECISION_F64) as s:
+        d0 = s.diagnostics(with_pe=True)
+        s.reset_stats()
+        dts = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step_cfl(args.cfl, 1)
+            dts.append(s.sim_time()[1])
+        ref_ms = (time.perf_counter() - t0) * 1e3 / steps
+        d1 = s.diagnostics(with_pe=True)
+        res["f64"] = (d0, d1, s.clamp_energy())
+    for name, form in (("fp32", isph.PRECISION_FP32), ("mixed", isph.PRECISION_MIXED)):
+        with handle(form) as s:
+            d0 = s.diagnostics(with_pe=True)
+            s.reset_stats()
+            for dt in dts:
+                s.set_dt(dt, 1.0)
+                s.encode_kdk(1)
+            d1 = s.diagnostics(with_pe=True)
+            res[name] = (d0, d1, s.clamp_energy())
+    e0 = _energy(res["f64"][0])
+    l0 = np.linalg.norm(res["f64"][0]["L"])
+    e_ref, l_ref = _energy(res["f64"][1]), res["f64"][1]["L"]
+    out = {"workload": f"two-body impact N={n}, LBVH theta={args.theta} monopole, {steps} KDK steps at the f64 "
+                       f"reference run's Courant dt (CFL {args.cfl}), all forms from the same state",
+           "steps": steps, "reference_ms_per_step": ref_ms,
+           "dE_rel": {k: (_energy(v[1]) - _energy(v[0])) / abs(e0) for k, v in res.items()},
+           "clamp_energy_rel": {k: v[2] / abs(e0) for k, v in res.items()}}
+    for name in ("fp32", "mixed"):
+        d1 = res[name][1]
+        out[f"{name}_precision_dE"] = abs(_energy(d1) - e_ref) / abs(e0)
+        out[f"{name}_precision_dL"] = float(np.linalg.norm(d1["L"] - l_ref) / l0)
+    out["precision_dE_ratio_fp32_over_mixed"] = out["fp32_precision_dE"] / max(out["mixed_precision_dE"], 1e-300)
+    out["precision_dL_ratio_fp32_over_mixed"] = out["fp32_precision_dL"] / max(out["mixed_precision_dL"], 1e-300)
+    return out
+
+
+def run_drift_study(args, sc):
+    """Where the energy drift of the bench workload comes from: the plain form at CFL 0.3, 0.1 and 0.03 over
+    about the same simulated time (steps scaled by 0.3 / CFL), with the clamp's added energy and the KE / IE /
+    PE split, on the monopole tree, on the quadrupole tree and (up to 262,144 particles) on the exact sum."""
+    from botheadstudios_amd import isph
+    n = len(sc.particles)
+    rows = []
+    cases = [(0.3, "tree"), (0.1, "tree"), (0.03, "tree"), (0.3, "quadrupole"), (0.1, "quadrupole"),
+             (0.03, "quadrupole")]
+    if n <= 262144:  # the exact sum at this size takes seconds per step at most
+        cases += [(0.3, "direct"), (0.1, "direct")]
+    for cfl, grav in cases:
+        with isph.GpuSph(n) as s:
+            s.upload(sc.particles, sc.eos, sc.softening)
+            s.set_eos_f64(sc.eos64)
+            s.set_gravity(isph.GRAVITY_DIRECT if grav == "direct" else isph.GRAVITY_TREE, args.theta, 1,
+                          grav == "quadrupole")
+            d0 = s.diagnostics(with_pe=True)
+            s.reset_stats()
+            steps = int(round(args.steps * 0.3 / cfl))
+            s.step_cfl(cfl, steps)
+            d1 = s.diagnostics(with_pe=True)
+            t, _ = s.sim_time()
+            e0 = _energy(d0)
+            rows.append({"cfl": cfl, "gravity": grav, "steps": steps, "sim_time_s": t,
+                         "dE_rel": (_energy(d1) - e0) / abs(e0), "clamp_energy_rel": s.clamp_energy() / abs(e0),
+                         "dKE_rel": (d1["ke"] - d0["ke"]) / abs(e0), "dIE_rel": (d1["ie"] - d0["ie"]) / abs(e0),
+                         "dPE_rel": (d1["pe"] - d0["pe"]) / abs(e0)})
+            print(json.dumps(rows[-1]), flush=True)
+    return {"workload": f"two-body impact N={n}, plain form, theta={args.theta}", "runs": rows}
 
 
 def run_4m(args, steps=10):
@@ -354,6 +453,9 @@ def main():
 
     sc = scene.two_body_1m() if args.n == 1048576 else scene.two_body(args.n)
     n = len(sc.particles)
+    if args.drift_study:
+        print(json.dumps({"metric": "energy drift decomposition", "drift": run_drift_study(args, sc)}), flush=True)
+        return
     r = run_gpu(sc, args, args.precision, args.steps, args.warmup, args.reuse, measure=True)
     st = r["stats"]
     launches = max(st["gravity_launches"], 1.0)
@@ -397,6 +499,7 @@ def main():
         line["dE_rel"] = r["dE_rel"]
         line["dL_rel"] = r["dL_rel"]
         line["dE_ok"] = bool(r["dE_rel"] < DE_BOUND)
+        line["clamp_energy_rel"] = r["clamp_energy_rel"]
     line["measure"] = {"impact_state": r["measure"]}
     debris, debris_rec = run_debris(n)
     line["measure"]["debris_field"] = debris
@@ -416,6 +519,8 @@ def main():
         line["batch"] = run_batch(args)
     if not args.no_4m:
         line["precision_4m"] = run_4m(args)
+    if not args.no_precision_study:
+        line["precision_study"] = run_precision_study(args, sc)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc)
         line["measure"]["impact_state"]["cpu_oracle_ms"] = measure_cpu(r["measure_rec"])

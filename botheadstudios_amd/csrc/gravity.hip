@@ -401,6 +401,211 @@ __global__ __launch_bounds__(256) void k_gravity_direct(const float4* __restrict
     if (i < n) sgrav[i] = make_float4(kG * ax, kG * ay, kG * az, 0.f);
 }
 
+// ---- the f64 reference form ---------------------------------------------------------------------------
+// The reference form runs the discrete step of the mixed form (same tree, same walk decisions, same pair
+// sets, same dt) with f64 coordinates in every pair term and f64 gravity, so that the difference between a
+// form's run and the reference run is that form's rounding alone. Coordinates are the f32 copies the walks
+// read plus f32 residuals of the f64 state (x64 = x32 + lo, |lo| <= half an f32 ulp, so x32 + lo carries
+// the f64 value to ~2^-48 of |x|).
+
+__global__ __launch_bounds__(256) void k_gather_lo(uint32_t n, const uint32_t* __restrict__ order,
+                                                   const double* __restrict__ pos64, const double* __restrict__ vel64,
+                                                   const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                                   float4* __restrict__ slo, float4* __restrict__ svlo) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t o = order[t];
+    const float4 p = spos[t], v = svel[t];
+    slo[t] = make_float4((float)(pos64[3 * o] - (double)p.x), (float)(pos64[3 * o + 1] - (double)p.y),
+                         (float)(pos64[3 * o + 2] - (double)p.z), 0.f);
+    svlo[t] = make_float4((float)(vel64[3 * o] - (double)v.x), (float)(vel64[3 * o + 1] - (double)v.y),
+                          (float)(vel64[3 * o + 2] - (double)v.z), 0.f);
+}
+
+// Depth of every internal node (parent hops to the root). A Karras child's common (code, index) prefix is
+// longer than its parent's, and the key has 30 + 32 bits, so depths stay below kMaxTreeDepth.
+constexpr uint32_t kMaxTreeDepth = 64;
+__global__ __launch_bounds__(256) void k_node_depth(uint32_t L, const uint32_t* __restrict__ parent,
+                                                    uint8_t* __restrict__ ndepth) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 >= L) return;
+    uint32_t d = 0, p = i;
+    while (d < kMaxTreeDepth) {
+        const uint32_t q = parent[p];
+        if (q == kNoParent) break;
+        p = q;
+        ++d;
+    }
+    ndepth[i] = (uint8_t)d;
+}
+
+__device__ __forceinline__ double4 body64(const float4* __restrict__ spos, const float4* __restrict__ slo, uint32_t s) {
+    const float4 p = spos[s], lo = slo[s];
+    return make_double4((double)p.x + (double)lo.x, (double)p.y + (double)lo.y, (double)p.z + (double)lo.z,
+                        (double)p.w);
+}
+
+// f64 COM and mass of the internal nodes at one depth, from their children (leaves: the particles, K = 1).
+// Launched from the deepest level up, so a node's children are complete when it is combined.
+__global__ __launch_bounds__(256) void k_com64_level(uint32_t L, uint32_t level, const uint8_t* __restrict__ ndepth,
+                                                     const uint32_t* __restrict__ left,
+                                                     const uint32_t* __restrict__ right,
+                                                     const float4* __restrict__ spos, const float4* __restrict__ slo,
+                                                     double4* __restrict__ nA64) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 >= L || ndepth[i] != level) return;
+    const uint32_t leaf0 = L - 1;
+    const uint32_t lc = left[i], rc = right[i];
+    const double4 a = lc >= leaf0 ? body64(spos, slo, lc - leaf0) : nA64[lc];
+    const double4 b = rc >= leaf0 ? body64(spos, slo, rc - leaf0) : nA64[rc];
+    const double m = a.w + b.w;
+    double4 c;
+    if (m > 0.0)
+        c = make_double4((a.w * a.x + b.w * b.x) / m, (a.w * a.y + b.w * b.y) / m, (a.w * a.z + b.w * b.z) / m, m);
+    else
+        c = make_double4(0.5 * (a.x + b.x), 0.5 * (a.y + b.y), 0.5 * (a.z + b.z), 0.0);
+    nA64[i] = c;
+}
+
+// The group walk of k_gravity_tree (one particle per lane, monopole, leaf size 1) with the same acceptance
+// test on the same f32 node records, so the same nodes and leaf particles are taken; each accepted node
+// contributes its f64 COM and mass and each leaf particle its f64 position, summed in f64 with IEEE sqrt
+// and division. With softening a particle's own term has d = 0 and adds exactly zero; without, r^2 = 0
+// terms are skipped.
+__global__ __launch_bounds__(64) void k_gravity_tree_ref(const float4* __restrict__ spos,
+                                                         const float4* __restrict__ slo, uint32_t n, uint32_t L,
+                                                         const uint4* __restrict__ nG,
+                                                         const double4* __restrict__ nA64, double soft2,
+                                                         double4* __restrict__ sgrav64, float4* __restrict__ sgrav,
+                                                         const uint2* __restrict__ groups, uint32_t* scal,
+                                                         uint32_t count_slot) {
+    __shared__ uint32_t stk[kStackCap];
+    __shared__ double4 cA[kBuf];
+    const int lane = threadIdx.x;
+    const uint32_t leaf0 = L - 1;
+    WalkQueue queue(scal + kQueueGrav, scal[count_slot], 1);
+    uint32_t gi;
+    while (queue.next(gi)) {
+        const uint2 grp = groups[gi];
+        const uint32_t t = grp.x + (uint32_t)lane;
+        const bool valid = (uint32_t)lane < grp.y;
+        const uint32_t tc = valid ? t : grp.x;
+        const float4 me = spos[tc];
+        const double4 me64 = body64(spos, slo, tc);
+        const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
+        const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        auto flush = [&](int cnt) {
+            for (int j = 0; j < cnt; ++j) {
+                const double4 c = cA[j];
+                const double dx = c.x - me64.x, dy = c.y - me64.y, dz = c.z - me64.z;
+                const double r2 = dx * dx + dy * dy + dz * dz + soft2;
+                const double w = r2 > 0.0 ? c.w / (r2 * sqrt(r2)) : 0.0;
+                ax += dx * w;
+                ay += dy * w;
+                az += dz * w;
+            }
+        };
+        auto shift = [&](int cnt) {
+            __syncthreads();
+            if (lane + kWave < cnt) cA[lane] = cA[lane + kWave];
+            __syncthreads();
+        };
+        int sp = 1, nc = 0;
+        __syncthreads();
+        if (lane == 0) stk[0] = 0u;
+        __syncthreads();
+        while (sp > 0) {
+            int k = min(kWave, sp);
+            int room = kStackCap - kStackMargin - sp;
+            if (room < k) k = max(1, room);
+            sp -= k;
+            const bool has = lane < k;
+            const uint32_t node = has ? stk[sp + lane] : 0u;
+            __syncthreads();
+            const bool is_leaf = has && node >= leaf0;
+            bool accept = false, open = false;
+            uint32_t lc = 0, rc = 0;
+            if (has && !is_leaf) {  // the acceptance test of k_gravity_tree, on the same f32 records
+                const uint4 r0 = nG[2 * node];
+                const uint4 r1 = nG[2 * node + 1];
+                const float Ax = __uint_as_float(r0.x), Ay = __uint_as_float(r0.y), Az = __uint_as_float(r0.z);
+                const float rcrit = __uint_as_float(r1.x);
+                lc = r1.y;
+                rc = r1.z;
+                float dx = fmaxf(fmaxf(gx0 - Ax, Ax - gx1), 0.f);
+                float dy = fmaxf(fmaxf(gy0 - Ay, Ay - gy1), 0.f);
+                float dz = fmaxf(fmaxf(gz0 - Az, Az - gz1), 0.f);
+                float d2 = dx * dx + dy * dy + dz * dz;
+                accept = d2 > 0.f && d2 >= rcrit * rcrit;
+                open = !accept;
+            }
+            const uint64_t mo = __ballot(open);
+            if (open) {
+                const int pos = sp + 2 * prefix_count(mo);
+                stk[pos] = lc;
+                stk[pos + 1] = rc;
+            }
+            sp += 2 * __popcll(mo);
+            const uint64_t ma = __ballot(accept);
+            if (accept) cA[nc + prefix_count(ma)] = nA64[node];
+            nc += __popcll(ma);
+            const uint64_t ml = __ballot(is_leaf);
+            if (ml && nc >= kWave) {
+                __syncthreads();
+                flush(kWave);
+                shift(nc);
+                nc -= kWave;
+            }
+            if (ml) {
+                if (is_leaf) cA[nc + prefix_count(ml)] = body64(spos, slo, node - leaf0);
+                nc += __popcll(ml);
+            }
+            __syncthreads();
+            if (nc >= kWave) {
+                flush(kWave);
+                shift(nc);
+                nc -= kWave;
+            }
+        }
+        if (nc > 0) flush(nc);
+        if (valid) {
+            sgrav64[t] = make_double4(kG64 * ax, kG64 * ay, kG64 * az, 0.0);
+            sgrav[t] = make_float4((float)(kG64 * ax), (float)(kG64 * ay), (float)(kG64 * az), 0.f);
+        }
+    }
+}
+
+// The exact softened sum in f64 (IEEE sqrt and division), tiled through LDS.
+__global__ __launch_bounds__(256) void k_gravity_direct_ref(const float4* __restrict__ spos,
+                                                            const float4* __restrict__ slo, uint32_t n, double soft2,
+                                                            double4* __restrict__ sgrav64, float4* __restrict__ sgrav) {
+    __shared__ double4 tile[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double4 me = body64(spos, slo, i < n ? i : n - 1);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (uint32_t base = 0; base < n; base += 256) {
+        const uint32_t j = base + threadIdx.x;
+        tile[threadIdx.x] = j < n ? body64(spos, slo, j) : make_double4(0.0, 0.0, 0.0, 0.0);
+        __syncthreads();
+        const uint32_t cnt = min(256u, n - base);
+        for (uint32_t jj = 0; jj < cnt; ++jj) {
+            const double4 c = tile[jj];
+            const double dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+            const double r2 = dx * dx + dy * dy + dz * dz + soft2;
+            const double w = (base + jj == i || !(r2 > 0.0)) ? 0.0 : c.w / (r2 * sqrt(r2));
+            ax += dx * w;
+            ay += dy * w;
+            az += dz * w;
+        }
+        __syncthreads();
+    }
+    if (i < n) {
+        sgrav64[i] = make_double4(kG64 * ax, kG64 * ay, kG64 * az, 0.0);
+        sgrav[i] = make_float4((float)(kG64 * ax), (float)(kG64 * ay), (float)(kG64 * az), 0.f);
+    }
+}
+
 __global__ void k_fill4(float4* p, uint32_t n, float4 v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -452,6 +657,35 @@ void launch_gravity_direct_raw(const float4* bodies, uint32_t n, float soft2, fl
         hipLaunchKernelGGL(k_gravity_direct<true>, dim3((n + 255) / 256), dim3(256), 0, s, bodies, n, soft2, out);
     else
         hipLaunchKernelGGL(k_gravity_direct<false>, dim3((n + 255) / 256), dim3(256), 0, s, bodies, n, soft2, out);
+}
+void launch_gather_lo(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_lo, dim3((d.n + 255) / 256), dim3(256), 0, s, d.n, d.vals, d.pos64, d.vel64, d.spos,
+                       d.svel, d.slo, d.svlo);
+}
+void launch_com64(const Dev& d, hipStream_t s) {
+    if (d.L < 2) return;
+    const uint32_t blocks = (d.L - 1 + 255) / 256;
+    hipLaunchKernelGGL(k_node_depth, dim3(blocks), dim3(256), 0, s, d.L, d.parent, d.ndepth);
+    for (uint32_t level = kMaxTreeDepth; level-- > 0;)
+        hipLaunchKernelGGL(k_com64_level, dim3(blocks), dim3(256), 0, s, d.L, level, d.ndepth, d.left, d.right,
+                           d.spos, d.slo, d.nA64);
+}
+void launch_gravity_tree_ref(const Dev& d, const StepParams& p, hipStream_t s) {
+    const uint32_t waves = std::min<uint32_t>(kWalkWaves, d.n);
+    hipLaunchKernelGGL(k_gravity_tree_ref, dim3(waves), dim3(kWave), 0, s, d.spos, d.slo, d.n, d.L, d.nG, d.nA64,
+                       p.soft2_64, d.sgrav64, d.sgrav, d.groups, d.scal, (uint32_t)kGroupsSph);
+}
+void launch_gravity_direct_ref(const Dev& d, const StepParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_gravity_direct_ref, dim3((d.n + 255) / 256), dim3(256), 0, s, d.spos, d.slo, d.n, p.soft2_64,
+                       d.sgrav64, d.sgrav);
+}
+__global__ void k_fill_d4(double4* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+void launch_gravity_zero_ref(const Dev& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill4, dim3((d.n + 255) / 256), dim3(256), 0, s, d.sgrav, d.n, make_float4(0.f, 0.f, 0.f, 0.f));
+    hipLaunchKernelGGL(k_fill_d4, dim3((d.n + 255) / 256), dim3(256), 0, s, d.sgrav64, d.n);
 }
 void launch_gravity_zero(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_fill4, dim3((d.n + 255) / 256), dim3(256), 0, s, d.sgrav, d.n, make_float4(0.f, 0.f, 0.f, 0.f));

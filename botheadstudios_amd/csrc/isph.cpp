@@ -108,6 +108,9 @@ struct isph_ctx {
 
 namespace {
 
+// one slot per 256-particle block of the kick kernels (their clamped-energy sums)
+size_t inject_slots(const Dev& d) { return ((size_t)d.cap + 255) / 256; }
+
 template <class T>
 void dalloc(T** p, size_t count) {
     HIP_OK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
@@ -119,10 +122,11 @@ void free_all(isph_ctx* c) {
                     d.keys_alt, d.vals_alt, d.hist, d.spos, d.svel, d.sh, d.smat, d.srho, d.spr, d.scs, d.sgrav,
                     d.lcode, d.left, d.right, d.parent, d.cslot, d.nrange, d.nlevel, d.carry, d.ncarry, d.groups, d.ggroups, d.chead,
                     d.cnext, d.ccnt, d.cpool, d.cmask, d.gflag, d.gscan, d.srho64, d.spr64, d.scs64, d.acc64, d.nG, d.nS, d.nA, d.nQ0, d.nQ1, d.scal, d.dt,
-                    d.time, d.counters, d.red, d.eos, d.aos, c->gpos, c->gmass};
+                    d.time, d.counters, d.red, d.inject, d.eos, d.aos, c->gpos, c->gmass};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
-    void* eptrs[] = {d.inst, d.mcode, d.lkey, d.ioff, d.ibox, d.iroot, d.isig, d.idt, d.itime, d.ext, d.mtarget};
+    void* eptrs[] = {d.inst, d.mcode, d.lkey, d.ioff, d.ibox, d.iroot, d.isig, d.idt, d.itime, d.ext, d.mtarget,
+                     d.slo, d.svlo, d.ndepth, d.nA64, d.sgrav64};
     for (void* q : eptrs)
         if (q) (void)hipFree(q);
     const Meas& m = c->meas;
@@ -205,6 +209,8 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.time, 1);
     dalloc(&d.counters, (size_t)kCounterSlots * kWalkWaves);
     dalloc(&d.red, 1024 * 16);
+    dalloc(&d.inject, inject_slots(d));
+    HIP_OK(hipMemset(d.inject, 0, inject_slots(d) * sizeof(double)));
     dalloc(&d.eos, kMaxMaterials);
     dalloc(&d.ext, kMaxExtBodies);
     dalloc(&d.mtarget, cap);
@@ -226,6 +232,17 @@ void ensure64(isph_ctx* c) {
         dalloc(&c->d.scs64, c->d.cap);
         dalloc(&c->d.acc64, 3 * (size_t)c->d.cap);
         HIP_OK(hipMemset(c->d.acc64, 0, 3 * (size_t)c->d.cap * sizeof(double)));
+    }
+}
+
+void ensure_ref(isph_ctx* c) {
+    Dev& d = c->d;
+    if (!d.slo) {
+        dalloc(&d.slo, d.cap);
+        dalloc(&d.svlo, d.cap);
+        dalloc(&d.ndepth, d.cap);
+        dalloc(&d.nA64, d.cap);
+        dalloc(&d.sgrav64, d.cap);
     }
 }
 
@@ -318,9 +335,16 @@ void build_tree(isph_ctx* c, const Dev& d) {
         launch_groups(d, c->p, 2 * kGroupMax, d.ggroups, d.scal + kGroupsGrav, c->stream);
 }
 
-void run_gravity(isph_ctx* c, const Dev& d, int mode) {
+void run_gravity(isph_ctx* c, const Dev& d, int mode, bool ref = false) {
     Phase ph(c, ISPH_PHASE_GRAVITY);
-    if (mode == ISPH_GRAVITY_TREE) {
+    if (ref && mode == ISPH_GRAVITY_TREE) {
+        launch_com64(d, c->stream);
+        launch_gravity_tree_ref(d, c->p, c->stream);
+    } else if (ref && mode == ISPH_GRAVITY_DIRECT) {
+        launch_gravity_direct_ref(d, c->p, c->stream);
+    } else if (ref) {
+        launch_gravity_zero_ref(d, c->stream);
+    } else if (mode == ISPH_GRAVITY_TREE) {
         launch_gravity_tree(d, c->p, c->stream);
         c->grav_launches += 1;
     } else if (mode == ISPH_GRAVITY_DIRECT) {
@@ -334,10 +358,15 @@ void run_gravity(isph_ctx* c, const Dev& d, int mode) {
 // One force evaluation (gpu_sph.rs:1468-1474): sort + tree + gravity + density/EOS + force.
 void eval(isph_ctx* c, bool rebuild) {
     const Dev& d = c->d;
+    const bool ref = c->p.ref64 != 0;
+    if (ref && (d.K != 1 || c->p.quad || c->p.grav_ppl != 1 || d.ninst > 1 || !d.pos64))
+        throw std::runtime_error("the f64 reference form takes one system, leaf size 1, monopole gravity and "
+                                 "one particle per lane");
     if (rebuild) {
         build_sorted(c, d, d.posh, d.mass, true);
+        if (ref) launch_gather_lo(d, c->stream);
         build_tree(c, d);
-        run_gravity(c, d, c->gravity_mode);
+        run_gravity(c, d, c->gravity_mode, ref);
         Phase ph(c, ISPH_PHASE_DENSITY);
         launch_density(d, c->p, c->stream);
     } else {
@@ -345,6 +374,7 @@ void eval(isph_ctx* c, bool rebuild) {
         launch_reset_scalars(d, c->stream, 1);
         if (d.ninst > 1) launch_reset_seg(d, c->stream, 1);
         launch_gather_vel(d, c->stream);
+        if (ref) launch_gather_lo(d, c->stream);
         launch_eos(d, c->stream);
     }
     Phase ph(c, ISPH_PHASE_FORCE);
@@ -649,7 +679,8 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
         h->meas.valid = false;
         h->softening = softening;
         h->p.soft2 = softening * softening;
-        if (h->precision == ISPH_PRECISION_MIXED && take > 0) {
+        h->p.soft2_64 = (double)softening * (double)softening;
+        if (h->precision != ISPH_PRECISION_FP32 && take > 0) {
             ensure64(h);
             launch_sync64_from32(h->d, h->stream);
         }
@@ -804,15 +835,19 @@ int isph_set_candidate_cache(isph_t* h, uint32_t max_blocks) {
 int isph_set_precision(isph_t* h, int mode) {
     return guarded(h, [&]() -> int {
         drop_graph(h);
-        if (mode != ISPH_PRECISION_FP32 && mode != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "bad precision");
+        if (mode != ISPH_PRECISION_FP32 && mode != ISPH_PRECISION_MIXED && mode != ISPH_PRECISION_F64)
+            return fail(h, ISPH_EINVAL, "bad precision");
         HIP_OK(hipSetDevice(h->device));
-        if (mode == ISPH_PRECISION_MIXED) {
+        if (mode != ISPH_PRECISION_FP32) {
             ensure64(h);
-            if (h->d.n > 0) launch_sync64_from32(h->d, h->stream);
+            if (mode == ISPH_PRECISION_F64) ensure_ref(h);
+            // entering an f64 form from the plain one takes the f32 state; between f64 forms the state stays
+            if (h->d.n > 0 && h->precision == ISPH_PRECISION_FP32) launch_sync64_from32(h->d, h->stream);
             h->p.mixed = 1;
         } else {
             h->p.mixed = 0;
         }
+        h->p.ref64 = mode == ISPH_PRECISION_F64 ? 1 : 0;
         h->precision = mode;
         // The integrators take the f64 arrays only in mixed mode.
         h->reuse_valid = false;
@@ -920,8 +955,10 @@ int isph_encode_merge(isph_t* h) {
             build_sorted(h, h->d, h->d.posh, h->d.mass, true);
             build_tree(h, h->d);
             Phase ph(h, ISPH_PHASE_DENSITY);
-            launch_density(h->d, h->p, h->stream);
-            launch_merge(h->d, h->p, h->stream);
+            StepParams pm = h->p;
+            pm.ref64 = 0;  // the merge neighbours and density need no f64 coordinates
+            launch_density(h->d, pm, h->stream);
+            launch_merge(h->d, pm, h->stream);
         } catch (...) {
             h->d = saved;
             throw;
@@ -954,7 +991,7 @@ int isph_set_reuse(isph_t* h, int enable) {
 
 static Dev integ_view(isph_ctx* c) {
     Dev d = c->d;
-    if (c->precision != ISPH_PRECISION_MIXED) {
+    if (c->precision == ISPH_PRECISION_FP32) {
         d.pos64 = nullptr;
         d.vel64 = nullptr;
         d.srho64 = nullptr;
@@ -1557,6 +1594,20 @@ int isph_walk_stats(isph_t* h, double* out9) {
     });
 }
 
+int isph_clamp_energy(isph_t* h, double* joules) {
+    return guarded(h, [&]() -> int {
+        if (!joules) return fail(h, ISPH_EINVAL, "NULL output");
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        std::vector<double> slots(inject_slots(h->d));
+        HIP_OK(hipMemcpy(slots.data(), h->d.inject, slots.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double sum = 0.0;
+        for (double x : slots) sum += x;
+        *joules = sum;
+        return ISPH_OK;
+    });
+}
+
 int isph_reset_stats(isph_t* h) {
     return guarded(h, [&]() -> int {
         HIP_OK(hipSetDevice(h->device));
@@ -1565,6 +1616,7 @@ int isph_reset_stats(isph_t* h) {
         h->grav_launches = 0;
         HIP_OK(hipMemsetAsync(h->d.counters, 0, (size_t)kCounterSlots * kWalkWaves * sizeof(unsigned long long),
                               h->stream));
+        HIP_OK(hipMemsetAsync(h->d.inject, 0, inject_slots(h->d) * sizeof(double), h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
         return ISPH_OK;
     });
@@ -1637,7 +1689,7 @@ static LossParams to_loss(const isph_loss* l) {
 
 int isph_set_state_f64(isph_t* h, const double* pos3, const double* vel3) {
     return guarded(h, [&]() -> int {
-        if (h->precision != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "the f64 state needs the mixed form");
+        if (h->precision == ISPH_PRECISION_FP32) return fail(h, ISPH_EINVAL, "the f64 state needs an f64 form");
         if (h->d.n == 0) return ISPH_OK;
         if (!pos3 || !vel3) return fail(h, ISPH_EINVAL, "NULL array");
         HIP_OK(hipSetDevice(h->device));
@@ -1664,10 +1716,23 @@ int isph_set_state_f64(isph_t* h, const double* pos3, const double* vel3) {
     });
 }
 
+int isph_state_f64(isph_t* h, double* pos3, double* vel3) {
+    return guarded(h, [&]() -> int {
+        if (h->precision == ISPH_PRECISION_FP32) return fail(h, ISPH_EINVAL, "the f64 state needs an f64 form");
+        if (h->d.n == 0) return ISPH_OK;
+        HIP_OK(hipSetDevice(h->device));
+        HIP_OK(hipStreamSynchronize(h->stream));
+        const size_t n = h->d.n;
+        if (pos3) HIP_OK(hipMemcpy(pos3, h->d.pos64, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        if (vel3) HIP_OK(hipMemcpy(vel3, h->d.vel64, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+        return ISPH_OK;
+    });
+}
+
 int isph_loss_value(isph_t* h, const isph_loss* loss, double* value) {
     return guarded(h, [&]() -> int {
         if (!loss || !value) return fail(h, ISPH_EINVAL, "NULL argument");
-        if (h->precision != ISPH_PRECISION_MIXED) return fail(h, ISPH_EINVAL, "the loss needs the mixed form");
+        if (h->precision == ISPH_PRECISION_FP32) return fail(h, ISPH_EINVAL, "the loss needs an f64 form");
         if (h->d.n == 0) {
             *value = 0.0;
             return ISPH_OK;
@@ -1831,7 +1896,7 @@ struct Snapshot {
         dalloc(&velu, n);
         HIP_OK(hipMemcpyAsync(posh, d.posh, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
         HIP_OK(hipMemcpyAsync(velu, d.velu, n * sizeof(float4), hipMemcpyDeviceToDevice, c->stream));
-        if (d.pos64 && c->precision == ISPH_PRECISION_MIXED) {
+        if (d.pos64 && c->precision != ISPH_PRECISION_FP32) {
             dalloc(&pos64, 3 * n);
             dalloc(&vel64, 3 * n);
             HIP_OK(hipMemcpyAsync(pos64, d.pos64, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));

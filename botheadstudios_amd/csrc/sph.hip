@@ -103,7 +103,9 @@ struct CacheWriter {
     }
 };
 
-template <bool F64>
+// REF (the f64 reference form, F64 too): the pair terms take f64 coordinates, each particle's f32 copy plus
+// its f32 residual against the f64 state (x64 = x32 + lo), so differences are the f64 state's to ~1e-16.
+template <bool F64, bool REF = false>
 __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __restrict__ spos,
                                                              const float* __restrict__ sh, uint32_t n, uint32_t K, uint32_t L, const SphNode* __restrict__ nS,
                                                 const uint32_t* __restrict__ order,
@@ -112,11 +114,13 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
                                                 unsigned long long* counters, const uint2* __restrict__ groups,
                                                 const uint2* __restrict__ nrange, uint32_t expand,
                                                 const uint32_t* __restrict__ sinst,
-                                                const uint32_t* __restrict__ iroot, CandCache cc) {
+                                                const uint32_t* __restrict__ iroot, CandCache cc,
+                                                const float4* __restrict__ slo) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, h
     __shared__ float cM[kWave];   // mass
+    __shared__ float4 cL[REF ? kWave : 1];  // position residuals (reference form)
     __shared__ uint32_t own[kWave];
     const int lane = threadIdx.x;
     own[lane] = kWave;
@@ -134,6 +138,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     const uint32_t tc = valid ? t : grp.x;
     __syncthreads();
     const float4 me = spos[tc];
+    const float4 melo = REF ? slo[tc] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float hi = sh[tc];
     const float qi = 0.5f * hi;  // half smoothing length, as the candidates carry it
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
@@ -155,6 +160,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
             const float4 p = spos[j];
             cP[lane] = make_float4(p.x, p.y, p.z, 0.5f * sh[j]);
             cM[lane] = p.w;
+            if (REF) cL[lane] = slo[j];
             note_owner(own, j, grp.x, grp.y, lane);
         }
         __syncthreads();
@@ -178,6 +184,12 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
                 if (F64) {
                     double ddx = (double)me.x - (double)pj.x, ddy = (double)me.y - (double)pj.y,
                            ddz = (double)me.z - (double)pj.z;
+                    if (REF) {
+                        const float4 lj = cL[jj];
+                        ddx += (double)melo.x - (double)lj.x;
+                        ddy += (double)melo.y - (double)lj.y;
+                        ddz += (double)melo.z - (double)lj.z;
+                    }
                     double inv_h = nr_rcp((double)qi + (double)pj.w);
                     const double rr = ddx * ddx + ddy * ddy + ddz * ddz;
                     double q = (rr > 0.0 ? rr * nr_rsq(rr) : 0.0) * inv_h;
@@ -276,7 +288,7 @@ __global__ __launch_bounds__(kEosThreads) void k_eos(const float* __restrict__ s
 }
 
 // Resident waves per SIMD: five for the plain form, four for the f64 form (up to 128 VGPRs, no spilling).
-template <bool F64>
+template <bool F64, bool REF = false>
 __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restrict__ spos, const float* __restrict__ sh,
                                               const float4* __restrict__ svel, const float* __restrict__ srho,
                                               const float* __restrict__ spr, const float* __restrict__ scs, uint32_t n,
@@ -291,7 +303,8 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                                               const uint32_t* __restrict__ iroot, Boundary bnd,
                                               const uint32_t* __restrict__ chead, const uint32_t* __restrict__ cnext,
                                               const uint32_t* __restrict__ ccnt, const uint32_t* __restrict__ cpool,
-                                              const uint64_t* __restrict__ cmask) {
+                                              const uint64_t* __restrict__ cmask, const float4* __restrict__ slo,
+                                              const float4* __restrict__ svlo, const double4* __restrict__ sgrav64) {
     __shared__ uint32_t stk[kSphStack];
     __shared__ uint32_t cand[kSphBuf];
     __shared__ float4 cP[kWave];  // x, y, z, h
@@ -300,6 +313,8 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     __shared__ double cR64[F64 ? kWave : 1];
     __shared__ double cPr64[F64 ? kWave : 1];
     __shared__ double cC64[F64 ? kWave : 1];
+    __shared__ float4 cL[REF ? kWave : 1];   // position residuals (reference form)
+    __shared__ float4 cVL[REF ? kWave : 1];  // velocity residuals
     __shared__ uint32_t own[kWave];
     const int lane = threadIdx.x;
     own[lane] = kWave;
@@ -319,6 +334,8 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     const float hi = sh[tc];
     const float qi = 0.5f * hi;  // half smoothing length, as the candidates carry it
     const float4 vi = svel[tc];
+    const float4 melo = REF ? slo[tc] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vilo = REF ? svlo[tc] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float rhoi = srho[tc], pri = spr[tc], ci = scs[tc];
     const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
     const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
@@ -355,6 +372,10 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             cPr64[lane] = spr64[j];
             cC64[lane] = scs64[j];
         }
+        if (REF) {
+            cL[lane] = slo[j];
+            cVL[lane] = svlo[j];
+        }
     };
     // The pair terms of one staged batch over `rest`, this lane's hits. 1) The radius test is r < (h_i +
     // h_j) / 2 in f32 (sph_step.wgsl:262-272), the density's own test, so a replayed chunk takes the
@@ -370,6 +391,12 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             if (F64) {
                 double dx = (double)me.x - (double)pj.x, dy = (double)me.y - (double)pj.y,
                        dz = (double)me.z - (double)pj.z;
+                if (REF) {
+                    const float4 lj = cL[jj];
+                    dx += (double)melo.x - (double)lj.x;
+                    dy += (double)melo.y - (double)lj.y;
+                    dz += (double)melo.z - (double)lj.z;
+                }
                 double r2 = dx * dx + dy * dy + dz * dz;
                 double hij = (double)qi + (double)pj.w;
                 if (r2 > 1.0e-18) {
@@ -379,6 +406,12 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
                     double q = r2 * inv_r * inv_h;
                     double dvx = (double)vi.x - (double)vj.x, dvy = (double)vi.y - (double)vj.y,
                            dvz = (double)vi.z - (double)vj.z;
+                    if (REF) {
+                        const float4 wj = cVL[jj];
+                        dvx += (double)vilo.x - (double)wj.x;
+                        dvy += (double)vilo.y - (double)wj.y;
+                        dvz += (double)vilo.z - (double)wj.z;
+                    }
                     double vr = dvx * dx + dvy * dy + dvz * dz;
                     double pij = 0.0;
                     if (vr < 0.0) {
@@ -527,9 +560,19 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
         acc_orig[o] = out;
         if (F64 && acc64) {  // the f64 integrators take the sum unrounded
             const bool live = me.w > 0.f;
-            acc64[3 * o] = live ? (double)gr.x + ax64 : 0.0;
-            acc64[3 * o + 1] = live ? (double)gr.y + ay64 : 0.0;
-            acc64[3 * o + 2] = live ? (double)gr.z + az64 : 0.0;
+            if (REF) {  // the f64 gravity of the reference walk, plus the (f32) bulk and external terms
+                const double4 g64 = sgrav64[t];
+                acc64[3 * o] = live ? g64.x + (double)gb.x + ax64 : 0.0;
+                acc64[3 * o + 1] = live ? g64.y + (double)gb.y + ay64 : 0.0;
+                acc64[3 * o + 2] = live ? g64.z + (double)gb.z + az64 : 0.0;
+            } else {
+                // The f32 gravity carries G rounded to f32, 3.6e-8 too strong: a systematic force scale whose
+                // work over a run shows as energy drift against the f64 energy. The f64 sum undoes it.
+                constexpr double kGfix = kG64 / (double)kG;
+                acc64[3 * o] = live ? (double)gr.x * kGfix + ax64 : 0.0;
+                acc64[3 * o + 1] = live ? (double)gr.y * kGfix + ay64 : 0.0;
+                acc64[3 * o + 2] = live ? (double)gr.z * kGfix + az64 : 0.0;
+            }
         }
     }
     tested_all += tested * grp.y;
@@ -538,19 +581,43 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
 }
 
 // ---- KDK (sph_step.wgsl:304-333; hydrostatic.rs:451-465), retired particles are not integrated ----------
+// The half-kick of u clamps at zero (sph_step.wgsl:310, hydrostatic.rs:457): the energy that adds, m (-u_new)
+// wherever u_new < 0, is counted in f64. Clamps are common in expanding material, so each 256-thread block
+// adds its sum into its own slot (the two kicks of a step run one after the other, so no slot is written
+// concurrently) instead of all blocks meeting on one address; the host sums the slots when asked. Every
+// thread of the block reaches the reduction.
+constexpr int kKickThreads = 256;
+__device__ __forceinline__ float clamp_u(float u_new, float m, double& injected) {
+    if (u_new < 0.0f) injected += -(double)u_new * (double)m;
+    return fmaxf(u_new, 0.0f);
+}
+__device__ __forceinline__ void add_injected(double* inject, double injected) {
+    __shared__ double part[kKickThreads / kWave];
+    if (!inject || !__syncthreads_or(injected != 0.0)) return;
+    const double s = wave_sum_f64(injected);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < kKickThreads / kWave; ++w) b += part[w];
+        inject[blockIdx.x] += b;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, float4* velu,
                                                     const float* __restrict__ mass, const float4* __restrict__ acc,
                                                     const float* __restrict__ dtp, double* pos64, double* vel64,
                                                     const double* __restrict__ acc64,
                                                     const uint32_t* __restrict__ inst, const float* __restrict__ idt,
-                                                    float4 bulk_cr, float4 bulk_vm) {
+                                                    float4 bulk_cr, float4 bulk_vm, double* inject) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !(mass[i] > 0.f)) return;
+    double injected = 0.0;
+    if (i < n && mass[i] > 0.f) {
     const float dt = inst ? idt[inst[i]] : *dtp;
     const float4 a = acc[i];
     float4 v = velu[i];
     float4 p = posh[i];
-    v.w = fmaxf(v.w + a.w * (0.5f * dt), 0.0f);
+    v.w = clamp_u(v.w + a.w * (0.5f * dt), mass[i], injected);
     if (pos64) {
         double hd = 0.5 * (double)dt;
         double vx = vel64[3 * i] + acc64[3 * i] * hd, vy = vel64[3 * i + 1] + acc64[3 * i + 1] * hd,
@@ -581,6 +648,8 @@ __global__ __launch_bounds__(256) void k_kick_drift(uint32_t n, float4* posh, fl
     }
     velu[i] = v;
     posh[i] = p;
+    }
+    add_injected(inject, injected);
 }
 
 // Damped relaxation (sph_step.wgsl:340-359): v = (v + (a + w^2 (x, y, 0)) dt) * damp, x += v dt, floor.
@@ -633,13 +702,15 @@ __global__ __launch_bounds__(256) void k_relax(uint32_t n, float4* posh, float4*
 __global__ __launch_bounds__(256) void k_kick(uint32_t n, float4* velu, const float* __restrict__ mass,
                                               const float4* __restrict__ acc, const float* __restrict__ dtp,
                                               double* vel64, const double* __restrict__ acc64,
-                                              const uint32_t* __restrict__ inst, const float* __restrict__ idt) {
+                                              const uint32_t* __restrict__ inst, const float* __restrict__ idt,
+                                              double* inject) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !(mass[i] > 0.f)) return;
+    double injected = 0.0;
+    if (i < n && mass[i] > 0.f) {
     const float dt = inst ? idt[inst[i]] : *dtp;
     const float4 a = acc[i];
     float4 v = velu[i];
-    v.w = fmaxf(v.w + a.w * (0.5f * dt), 0.0f);
+    v.w = clamp_u(v.w + a.w * (0.5f * dt), mass[i], injected);
     if (vel64) {
         double hd = 0.5 * (double)dt;
         double vx = vel64[3 * i] + acc64[3 * i] * hd, vy = vel64[3 * i + 1] + acc64[3 * i + 1] * hd,
@@ -656,6 +727,8 @@ __global__ __launch_bounds__(256) void k_kick(uint32_t n, float4* velu, const fl
         v.z += a.z * (0.5f * dt);
     }
     velu[i] = v;
+    }
+    add_injected(inject, injected);
 }
 
 // dt = cfl * min_i h_i / max(c_i + |v_i|, 1)  (impact-run main.rs:345-356 done on the device).
@@ -825,14 +898,19 @@ static Boundary boundary(const Dev& d, const StepParams& p) {
 
 void launch_density(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
-    if (p.mixed)
+    if (p.ref64)
+        hipLaunchKernelGGL((k_density<true, true>), dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L,
+                           d.nS, d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
+                           (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d, p), (const float4*)d.slo);
+    else if (p.mixed)
         hipLaunchKernelGGL(k_density<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nS,
                            d.vals, d.srho, d.rho, d.srho64, d.scal, d.counters, d.groups, d.nrange,
-                           (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d, p));
+                           (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d, p), (const float4*)nullptr);
     else
         hipLaunchKernelGGL(k_density<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.n, d.K, d.L, d.nS,
                            d.vals, d.srho, d.rho, (double*)nullptr, d.scal, d.counters, d.groups,
-                           d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d, p));
+                           d.nrange, (uint32_t)p.sph_expand, d.sinst(), d.iroot, cand_cache(d, p),
+                           (const float4*)nullptr);
     launch_eos(d, s);
 }
 void launch_eos(const Dev& d, hipStream_t s) {
@@ -842,23 +920,32 @@ void launch_eos(const Dev& d, hipStream_t s) {
 }
 void launch_force(const Dev& d, const StepParams& p, hipStream_t s) {
     uint32_t groups = std::min<uint32_t>(kWalkWaves, d.n);
-    if (p.mixed)
+    if (p.ref64)
+        hipLaunchKernelGGL((k_force<true, true>), dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho,
+                           d.spr, d.scs, d.n, d.K, d.L, d.nS, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
+                           d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
+                           (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext,
+                           d.ccnt, d.cpool, d.cmask, (const float4*)d.slo, (const float4*)d.svlo,
+                           (const double4*)d.sgrav64);
+    else if (p.mixed)
         hipLaunchKernelGGL(k_force<true>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr, d.scs,
                            d.n, d.K, d.L, d.nS, d.vals, d.sgrav, d.acc, d.srho64, d.spr64,
                            d.scs64, p.av_alpha, p.av_beta, d.counters, d.groups, d.scal, d.nrange,
                            (uint32_t)p.sph_expand, d.acc64, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext,
-                           d.ccnt, d.cpool, d.cmask);
+                           d.ccnt, d.cpool, d.cmask, (const float4*)nullptr, (const float4*)nullptr,
+                           (const double4*)nullptr);
     else
         hipLaunchKernelGGL(k_force<false>, dim3(groups), dim3(kWave), 0, s, d.spos, d.sh, d.svel, d.srho, d.spr,
                            d.scs, d.n, d.K, d.L, d.nS, d.vals, d.sgrav, d.acc,
                            (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, p.av_alpha,
                            p.av_beta, d.counters, d.groups, d.scal, d.nrange, (uint32_t)p.sph_expand,
                            (double*)nullptr, d.sinst(), d.iroot, boundary(d, p), d.chead, d.cnext, d.ccnt, d.cpool,
-                           d.cmask);
+                           d.cmask, (const float4*)nullptr, (const float4*)nullptr, (const double4*)nullptr);
 }
 void launch_kick_drift(const Dev& d, const StepParams& p, hipStream_t s) {
     hipLaunchKernelGGL(k_kick_drift, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,
-                       d.pos64, d.vel64, d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt, p.bulk_cr, p.bulk_vm);
+                       d.pos64, d.vel64, d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt, p.bulk_cr, p.bulk_vm,
+                       d.inject);
 }
 void launch_relax(const Dev& d, const StepParams& p, float damp, hipStream_t s) {
     hipLaunchKernelGGL(k_relax, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.posh, d.velu, d.mass, d.acc, d.dt,
@@ -866,7 +953,7 @@ void launch_relax(const Dev& d, const StepParams& p, float damp, hipStream_t s) 
 }
 void launch_kick(const Dev& d, hipStream_t s) {
     hipLaunchKernelGGL(k_kick, dim3(cdiv(d.n, 256)), dim3(256), 0, s, d.n, d.velu, d.mass, d.acc, d.dt, d.vel64,
-                       d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt);
+                       d.acc64, d.ninst > 1 ? d.inst : nullptr, d.idt, d.inject);
 }
 void launch_set_dt_cfl(const Dev& d, float cfl, hipStream_t s) {
     hipLaunchKernelGGL(k_set_dt_cfl, dim3(1), dim3(1), 0, s, d.scal, cfl, d.dt);

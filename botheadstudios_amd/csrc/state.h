@@ -96,6 +96,14 @@ struct Dev {
     float4* nQ0 = nullptr; // quadrupole xx, yy, zz, xy
     float2* nQ1 = nullptr; // quadrupole xz, yz
 
+    // f64 reference form: residuals of the f64 state against the f32 copies the walks read (x64 = x32 + lo),
+    // sorted; node depths, the f64 COM and mass of every internal node, the f64 gravity of each sorted particle
+    float4* slo = nullptr;      // position residual xyz (w unused)
+    float4* svlo = nullptr;     // velocity residual xyz (w unused)
+    uint8_t* ndepth = nullptr;  // [L-1]
+    double4* nA64 = nullptr;    // [L-1] com.xyz, mass
+    double4* sgrav64 = nullptr; // xyz (w unused)
+
     float4* ext = nullptr;       // [kMaxExtBodies] external point masses (xyz, mass)
     uint32_t* mtarget = nullptr; // merge target of each particle (upload order)
 
@@ -122,6 +130,7 @@ struct Dev {
     double* time = nullptr;    // accumulated simulated time
     unsigned long long* counters = nullptr;  // [0] cells, [1] particles, [2] quad cells, [3] dens pairs, [4] force pairs
     double* red = nullptr;     // reduction partials (diagnostics)
+    double* inject = nullptr;  // energy added by the u >= 0 clamp of the kicks since the last reset (J)
 
     Till* eos = nullptr;       // EOS table [kMaxMaterials]
 
@@ -149,6 +158,8 @@ struct StepParams {
     // candidate-cache blocks the density walk may use (capped at what is allocated); a group whose list
     // does not fit is walked again by the force pass, so a low cap exercises that path
     uint32_t cache_blocks = 0xffffffffu;
+    int ref64 = 0;  // the f64 reference form (mixed as well): f64 coordinates in every pair term, f64 gravity
+    double soft2_64 = 0.0;  // softening^2 in f64 (the reference form's gravity)
 };
 
 constexpr uint32_t kMaxExtBodies = 32;  // gpu_sph.rs:155
@@ -274,6 +285,13 @@ void launch_radix_pass_n(uint32_t n, uint32_t* hist, const uint32_t* kin, const 
 
 void launch_gravity_tree(const Dev& d, const StepParams& p, hipStream_t s);
 void launch_gravity_direct(const Dev& d, const StepParams& p, hipStream_t s);
+// f64 reference form: residuals of the sorted state, node COM and mass in f64 (level by level from the
+// deepest), and the f64 gravity (tree walk with the f32 walk's acceptance decisions, or the exact sum)
+void launch_gather_lo(const Dev& d, hipStream_t s);
+void launch_com64(const Dev& d, hipStream_t s);
+void launch_gravity_tree_ref(const Dev& d, const StepParams& p, hipStream_t s);
+void launch_gravity_direct_ref(const Dev& d, const StepParams& p, hipStream_t s);
+void launch_gravity_zero_ref(const Dev& d, hipStream_t s);
 // The exact sum on explicit arrays: bodies (x, y, z, m) in, kG-scaled accelerations out, compensated sums
 // when `comp`; no handle, no allocation.
 void launch_gravity_direct_raw(const float4* bodies, uint32_t n, float soft2, float4* out, bool comp, hipStream_t s);

@@ -21,7 +21,7 @@ EAGAIN = -11
 EINVAL = -22
 GRAVITY_TREE, GRAVITY_DIRECT, GRAVITY_NONE = 0, 1, 2
 CACHE_ALL = 0xFFFFFFFF
-PRECISION_FP32, PRECISION_MIXED = 0, 1
+PRECISION_FP32, PRECISION_MIXED, PRECISION_F64 = 0, 1, 2
 PHASES = ("sort", "tree", "upsweep", "gravity", "density", "force", "integrate")
 
 # Every symbol include/isph.h declares, with its ctypes signature.
@@ -74,6 +74,8 @@ _SIGS = {
     "isph_ensemble_times": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint32]),
     "isph_loss_value": (ctypes.c_int, [_P, _P, _P]),
     "isph_set_state_f64": (ctypes.c_int, [_P, _P, _P]),
+    "isph_state_f64": (ctypes.c_int, [_P, _P, _P]),
+    "isph_clamp_energy": (ctypes.c_int, [_P, _P]),
     "isph_vjp_run": (ctypes.c_int, [_P, ctypes.c_float, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
                                     _P, _P, _P, _P]),
     "isph_set_measure": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double]),
@@ -348,6 +350,14 @@ class GpuSph:
         v = np.ascontiguousarray(vel, np.float64).reshape(-1, 3)
         self._check(self._lib.isph_set_state_f64(self._h, _ptr(p), _ptr(v)))
 
+    def state_f64(self):
+        """The f64 position and velocity state (either f64 form), upload order."""
+        n = self.count()
+        p = np.zeros((n, 3), np.float64)
+        v = np.zeros((n, 3), np.float64)
+        self._check(self._lib.isph_state_f64(self._h, _ptr(p), _ptr(v)))
+        return p, v
+
     def vjp_run(self, loss, steps, checkpoint_every=20, cfl=0.3, dts=None):
         """Reverse-mode gradient of `loss` after `steps` KDK steps from the current state. Returns a dict:
         loss, dpos (n,3), dvel (n,3), du (n,), dmass (n,), deos (16 materials x (a, b, A, B)), dts, ms
@@ -390,6 +400,12 @@ class GpuSph:
         self._check(self._lib.isph_walk_stats(self._h, _ptr(out)))
         keys = [f"{w}_{k}" for w in ("gravity", "density", "force") for k in ("rounds", "popped", "max_stack")]
         return dict(zip(keys, out.tolist()))
+
+    def clamp_energy(self):
+        """Energy (J) the u >= 0 clamp of the half-kicks added since the last reset_stats."""
+        out = np.zeros(1, np.float64)
+        self._check(self._lib.isph_clamp_energy(self._h, _ptr(out)))
+        return float(out[0])
 
     def reset_stats(self):
         self._check(self._lib.isph_reset_stats(self._h))

@@ -66,6 +66,11 @@ typedef struct isph_ctx isph_t;
 /* Precision modes (isph_set_precision). */
 #define ISPH_PRECISION_FP32 0  /* plain: f32 state, f32 pair sums (the reference's GPU arithmetic) */
 #define ISPH_PRECISION_MIXED 1 /* f64 position/velocity state; f64 SPH pair sums and per-particle rho, P, c; compensated f32 gravity sums */
+/* The f64 reference of the mixed form's discrete step (same tree, walk decisions, pair sets and dt): f64
+ * coordinates and velocities in every pair term, f64 node moments and f64 gravity; u stays f32. It exists to
+ * measure what each form's rounding costs (a run minus the reference run); one system, leaf size 1, monopole
+ * gravity with one particle per lane; no adjoint. */
+#define ISPH_PRECISION_F64 2
 
 /* Phase indices of isph_phase_times. */
 #define ISPH_PHASE_SORT 0      /* bbox + Morton + radix sort + gather */
@@ -243,6 +248,10 @@ int isph_kernel_stats(isph_t* h, double* out8);
  * out[3..5] the same for the density walk; out[6..8] for the force walk. */
 int isph_walk_stats(isph_t* h, double* out9);
 int isph_reset_stats(isph_t* h);
+/* Energy (J, f64) that the u >= 0 clamp of the half-kicks (sph_step.wgsl:310, hydrostatic.rs:457) has added
+ * since the last isph_reset_stats: the sum of m (-u_new) over every clamped update. No reference
+ * counterpart; it separates the clamp from the other sources of energy drift. */
+int isph_clamp_energy(isph_t* h, double* joules);
 
 /* ---- adjoint (no reference counterpart: the differentiable form of the step) ------------------------ */
 /* Smoothed disk mass: loss = sum_i m_i sig(k_r (r_i / radius - r_cut)) sig(k_j (j_i / sqrt(G M r_i) - 1)),
@@ -258,8 +267,10 @@ typedef struct isph_loss {
 
 /* Loss of the current state (mixed form). */
 int isph_loss_value(isph_t* h, const isph_loss* loss, double* value);
-/* Replace the f64 position / velocity state exactly (mixed form; u, h, mass unchanged). */
+/* Replace the f64 position / velocity state exactly (either f64 form; u, h, mass unchanged). */
 int isph_set_state_f64(isph_t* h, const double* pos3, const double* vel3);
+/* Read the f64 position / velocity state (either f64 form), upload order; either output may be NULL. */
+int isph_state_f64(isph_t* h, double* pos3, double* vel3);
 /* Reverse-mode gradient of `loss` after `steps` KDK steps from the current state (mixed form; gravity direct,
  * none, or the monopole tree walk with one particle per lane). Time steps: dts_in[steps] if not NULL, else
  * the device Courant dt at `cfl`; they are held fixed in the derivative. The forward run stores a state
