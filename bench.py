@@ -372,6 +372,59 @@ def run_drift_study(args, sc):
     return {"workload": f"two-body impact N={n}, plain form, theta={args.theta}", "runs": rows}
 
 
+def _impact_gradient(fam, loss, steps, params, checkpoint_every, cfl, theta, dts=None):
+    """Loss after `steps` KDK steps of the two-body scene at `params` = (angle deg, speed / v_esc, mass
+    ratio), mixed form, LBVH theta; and its gradient with respect to the three parameters (the reverse pass's
+    start-state gradient projected on the scene's tangents). Returns (loss, grad[3], dts, ms2)."""
+    from botheadstudios_amd import isph, materials, scene
+    rec, pos, vel = fam.state(*params)
+    n = len(rec)
+    with isph.GpuSph(n) as s:
+        s.upload(rec, materials.eos_table(), fam.softening * scene.L_U)
+        s.set_eos_f64(materials.eos_f64())
+        s.set_precision(isph.PRECISION_MIXED)
+        s.set_state_f64(pos, vel)
+        s.set_gravity(isph.GRAVITY_TREE, theta, 1)
+        out = s.vjp_run(loss, steps, checkpoint_every, cfl=cfl, dts=dts)
+    g = np.array([(out["dpos"] * dp).sum() + (out["dvel"] * dv).sum() + (out["dmass"] * dm).sum()
+                  for dp, dv, dm in fam.tangents(*params)])
+    return out["loss"], g, out["dts"], out["ms"]
+
+
+def run_adjoint_outcome(args, steps=800, fit_steps=400, iterations=3):
+    """The differentiation workload where the impact has an outcome. (1) The reverse pass over `steps` KDK
+    steps from the standard start (the ejecta have left 2.9 target radii by then): loss and cost. (2) A
+    Gauss-Newton fit of (angle, speed, mass ratio) over `fit_steps` steps toward twice the nominal loss,
+    each step scaled by the parameters' nominal sizes: p += s^2 g (L* - L) / sum (s g)^2, with the dt
+    sequence of the first evaluation held fixed."""
+    from botheadstudios_amd import isph, scene
+    n = args.adjoint_n
+    fam = scene.ImpactFamily(n - n // 8, n // 8, 0xC5)
+    loss = isph.Loss.make(scene.L_U, r_cut=2.9, k_r=20.0, k_j=10.0)
+    p0 = np.array([45.0, 1.2, 0.11])
+    l_long, g_long, _, ms = _impact_gradient(fam, loss, steps, p0, args.adjoint_every, args.cfl, args.theta)
+    out = {"workload": f"two-body impact N={n}, mixed form, LBVH theta={args.theta}, {steps} KDK steps at CFL "
+                       f"{args.cfl}, checkpoint every {args.adjoint_every}",
+           "loss_kg": l_long, "loss_over_total_mass": None, "forward_ms": float(ms[0]), "reverse_ms": float(ms[1]),
+           "adjoint_over_forward": float(ms[1] / ms[0]),
+           "grad_angle_speed_ratio": [float(x) for x in g_long]}
+    rec, _, _ = fam.state(*p0)
+    out["loss_over_total_mass"] = l_long / float(rec["mass"].astype(np.float64).sum())
+    scale = np.array([45.0, 1.2, 0.11])
+    p = p0.copy()
+    l, g, dts, _ = _impact_gradient(fam, loss, fit_steps, p, args.adjoint_every, args.cfl, args.theta)
+    target = 2.0 * l
+    hist = [{"params": p.tolist(), "loss_kg": l}]
+    for _ in range(iterations):
+        sg = scale * g
+        p = p + scale * sg * (target - l) / float(sg @ sg)
+        l, g, _, _ = _impact_gradient(fam, loss, fit_steps, p, args.adjoint_every, args.cfl, args.theta, dts)
+        hist.append({"params": p.tolist(), "loss_kg": l})
+    out["fit"] = {"steps": fit_steps, "target_kg": target, "iterations": hist,
+                  "final_over_target": hist[-1]["loss_kg"] / target}
+    return out
+
+
 def run_4m(args, steps=10):
     """Plain vs mixed throughput on the 4,194,304-particle two-body impact (the precision form's size)."""
     from botheadstudios_amd import isph, scene
@@ -515,6 +568,7 @@ def main():
                    "dE_ok": None if rv.get("dE_rel") is None else bool(rv["dE_rel"] < DE_BOUND)}
     if not args.no_adjoint:
         line["adjoint"] = run_adjoint(args)
+        line["adjoint"]["outcome"] = run_adjoint_outcome(args)
     if not args.no_batch:
         line["batch"] = run_batch(args)
     if not args.no_4m:

@@ -240,10 +240,13 @@ __global__ __launch_bounds__(kDirTile) void k_vjp_gravity_direct(uint32_t n, con
             const float dai = dx * ax + dy * ay + dz * az;
             const float daj = dx * aj.x + dy * aj.y + dz * aj.z;
             const float ci = 3.0f * dai * inv2, cj = 3.0f * daj * inv2;
-            // m_i (abar_j - 3 (d.abar_j) d / r2) - m_j (abar_i - 3 (d.abar_i) d / r2)
-            fx += s * (me.w * (aj.x - cj * dx) - pj.w * (ax - ci * dx));
-            fy += s * (me.w * (aj.y - cj * dy) - pj.w * (ay - ci * dy));
-            fz += s * (me.w * (aj.z - cj * dz) - pj.w * (az - ci * dz));
+            // m_i (abar_j - 3 (d.abar_j) d / r2) - m_j (abar_i - 3 (d.abar_i) d / r2), scaled by s first: in SI
+            // units a particle mass (~1e19 kg) times an acceleration adjoint leaves the f32 range over long
+            // horizons, while s m stays small
+            const float smi = s * me.w, smj = s * pj.w;
+            fx += smi * (aj.x - cj * dx) - smj * (ax - ci * dx);
+            fy += smi * (aj.y - cj * dy) - smj * (ay - ci * dy);
+            fz += smi * (aj.z - cj * dz) - smj * (az - ci * dz);
             fm -= s * daj;
         }
         gx += fx;

@@ -215,12 +215,13 @@ class ImpactFamily:
     """
 
     def __init__(self, n_target, n_impactor, seed, r_t=1.0, m_t=1.0, r_i=0.5, m_i=0.11, core=0.3, u0=1e-3,
-                 softening=0.01):
+                 softening=0.01, separation=1.6):
         self.pt, self.mt, self.ht, self.matt, _ = make_body(n_target, r_t, m_t, core, seed ^ 0x7A1)
         self.pi, _, self.hi, self.mati, _ = make_body(n_impactor, r_i, m_i, core, seed ^ 0x1A2)
         self.r_t, self.m_t, self.r_i = r_t, m_t, r_i
         self.n_target, self.n_impactor = n_target, n_impactor
         self.u0, self.softening = u0, softening
+        self.separation = separation  # start distance along x in contact distances (impact-run: 1.6)
 
     def _v(self, speed, ratio):
         return speed * np.sqrt(2.0 * (self.m_t + ratio * self.m_t) / (self.r_t + self.r_i))
@@ -229,7 +230,7 @@ class ImpactFamily:
         """The f64 SI state: pos (n,3), vel (n,3), mass, h, u, mat."""
         contact = self.r_t + self.r_i
         b = contact * np.sin(np.deg2rad(angle_deg))
-        pos = np.concatenate([self.pt, self.pi + np.array([1.6 * contact, b, 0.0])]) * L_U
+        pos = np.concatenate([self.pt, self.pi + np.array([self.separation * contact, b, 0.0])]) * L_U
         vel = np.zeros_like(pos)
         vel[self.n_target:, 0] = -self._v(v_over_vesc, mass_ratio) * V_U
         mass = np.concatenate([self.mt, np.full(self.n_impactor, mass_ratio * self.m_t / self.n_impactor)]) * M_U

