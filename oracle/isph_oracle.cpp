@@ -16,8 +16,10 @@
 // Pinning (tests/test_oracle_*.py): closed-form Morton KATs, brute-force sort and top-down radix-tree
 // build, root-moment closed forms, theta->0 = direct sum, the reference's own seeds and bounds for the
 // octree (bhtree.rs:291-407), the grid pair set (neighbors.rs:106-158), the Tillotson invariants and
-// parameter KATs (eos.rs:233-418), SPH kernel normalisation, grid-vs-brute SPH sums, momentum and energy
-// identities of the symmetric SPH pair loop, KDK time reversibility. Functions with no such check:
+// parameter KATs (eos.rs:233-418), SPH kernel normalisation, grid-vs-brute SPH sums, two-particle SPH sums
+// written out by hand from Monaghan's formulation (density, accelerations, du/dt, with and without
+// viscosity), the engine's own head-on shock test with its bounds (hydrostatic.rs:1354-1452), momentum and
+// energy identities of the symmetric SPH pair loop, KDK time reversibility. Functions with no such check:
 // none at present.
 //
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off -fno-fast-math), output oracle/liboracle.so.
