@@ -140,11 +140,19 @@ def cpu_baseline(sc):
     hb.close()
     n = len(mass)
     avail, total = host_cores()
+    # the reference GPU walk's own work (per-particle MAC, bh_gravity.wgsl:337-394) on a seeded sample
+    oracle.set_threads(avail)
+    sample = np.random.default_rng(0x16).choice(n, 16384, replace=False)
+    counts = oracle.lbvh_walk_counts(sc.particles["pos"], sc.particles["mass"], sc.softening, sample)
+    oracle.set_threads(1)
     return {"value": n / (2.0 * dt), "unit": "particle-steps/s", "cores": 1, "kind": "port",
             "sample": f"one force evaluation (density + forces_and_dudt, half a KDK step) of the N={n} scene, "
                       f"{dt:.1f} s on 1 thread of {lscpu_model()}",
             "host": {"cpu_model": lscpu_model(), "logical_cpus_available": avail, "logical_cpus_machine": total},
-            "eval_ms": ph, "c1_run": cpu_c1_run()}
+            "eval_ms": ph, "c1_run": cpu_c1_run(),
+            "reference_walk": {"sample": "16,384 particles of the start state (seed 0x16)",
+                               "cells_per_particle": float(counts[:, 0].mean()),
+                               "particles_per_particle": float(counts[:, 1].mean())}}
 
 
 def time_disk_view(s, reps=5):
@@ -577,6 +585,16 @@ def main():
         line["precision_study"] = run_precision_study(args, sc)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(sc)
+        # the roofline on the reference walk's interaction count (its per-particle MAC needs fewer
+        # interactions than the group MAC, which accepts a node only for the whole walk group)
+        rw = line["cpu_baseline"]["reference_walk"]
+        ref_inter = rw["cells_per_particle"] + rw["particles_per_particle"]
+        walk_inter = line["interactions_per_particle_eval"]
+        rf = line["roofline"]
+        rf["reference_walk_interactions_per_particle"] = ref_inter
+        rf["walk_interactions_per_particle"] = walk_inter
+        rf["achieved_reference_walk"] = rf["achieved"] * ref_inter / walk_inter
+        rf["frac_reference_walk"] = rf["achieved_reference_walk"] / FP32_PEAK_TFLOPS
         line["measure"]["impact_state"]["cpu_oracle_ms"] = measure_cpu(r["measure_rec"])
         line["measure"]["debris_field"]["cpu_oracle_ms"] = measure_cpu(debris_rec)
     print(json.dumps(line), flush=True)

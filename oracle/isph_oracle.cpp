@@ -1003,56 +1003,83 @@ void orc_upsweep_f32(const float* sbodies, uint64_t n, uint32_t K, uint32_t L, c
     }
 }
 
-// LBVH theta traversal (f32): bh_gravity.wgsl:337-394 (per-particle stack walk, robust MAC, leaf direct).
+// LBVH theta traversal (f32): bh_gravity.wgsl:337-394 (per-particle stack walk, robust MAC, leaf direct), for
+// sorted target t; counts the accepted cells and the particle-particle terms it sums.
+static void lbvh_walk_one(uint64_t t, const float* sbodies, uint64_t n, uint32_t K, uint32_t L, const uint32_t* left,
+                          const uint32_t* right, const float* com4, const float* bmin3, const float* bmax3,
+                          float theta, float soft2, float a[3], uint64_t& ncell, uint64_t& npart) {
+    const uint32_t leaf0 = L - 1;
+    const float* pi = sbodies + 4 * t;
+    a[0] = a[1] = a[2] = 0.0f;
+    uint32_t stack[64];
+    uint32_t sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        uint32_t ni = stack[--sp];
+        if (ni >= leaf0) {
+            uint64_t c = ni - leaf0, start = c * K, end = std::min<uint64_t>(start + K, n);
+            for (uint64_t s = start; s < end; ++s) {
+                if (s == t) continue;
+                const float* bj = sbodies + 4 * s;
+                float d[3] = {bj[0] - pi[0], bj[1] - pi[1], bj[2] - pi[2]};
+                float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + soft2;
+                float g = G32 * bj[3] / (r2 * std::sqrt(r2));
+                for (int k = 0; k < 3; ++k) a[k] += d[k] * g;
+                ++npart;
+            }
+            continue;
+        }
+        const float* c4 = com4 + 4 * ni;
+        float d[3] = {c4[0] - pi[0], c4[1] - pi[1], c4[2] - pi[2]};
+        float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + soft2;
+        float dist = std::sqrt(r2);
+        float e[3], cen[3];
+        for (int k = 0; k < 3; ++k) {
+            e[k] = bmax3[3 * ni + k] - bmin3[3 * ni + k];
+            cen[k] = 0.5f * (bmin3[3 * ni + k] + bmax3[3 * ni + k]) - c4[k];
+        }
+        float size = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        float delta = std::sqrt(cen[0] * cen[0] + cen[1] * cen[1] + cen[2] * cen[2]);
+        if (dist >= size / theta + delta) {
+            float g = G32 * c4[3] / (r2 * dist);
+            for (int k = 0; k < 3; ++k) a[k] += d[k] * g;
+            ++ncell;
+            continue;
+        }
+        if (sp + 2 <= 64) {
+            stack[sp++] = left[ni];
+            stack[sp++] = right[ni];
+        }
+    }
+}
+
 // Output acc3 in ORIGINAL order (scatter via order[], :393).
 void orc_lbvh_gravity_f32(const float* sbodies, uint64_t n, uint32_t K, uint32_t L, const uint32_t* left,
                           const uint32_t* right, const float* com4, const float* bmin3, const float* bmax3,
                           float theta, float soft2, const uint32_t* order, float* acc3) {
-    uint32_t leaf0 = L - 1;
 #pragma omp parallel for schedule(dynamic, 64)
     for (uint64_t t = 0; t < n; ++t) {
-        const float* pi = sbodies + 4 * t;
-        float a[3] = {0, 0, 0};
-        uint32_t stack[64];
-        uint32_t sp = 0;
-        stack[sp++] = 0;
-        while (sp > 0) {
-            uint32_t ni = stack[--sp];
-            if (ni >= leaf0) {
-                uint64_t c = ni - leaf0, start = c * K, end = std::min<uint64_t>(start + K, n);
-                for (uint64_t s = start; s < end; ++s) {
-                    if (s == t) continue;
-                    const float* bj = sbodies + 4 * s;
-                    float d[3] = {bj[0] - pi[0], bj[1] - pi[1], bj[2] - pi[2]};
-                    float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + soft2;
-                    float g = G32 * bj[3] / (r2 * std::sqrt(r2));
-                    for (int k = 0; k < 3; ++k) a[k] += d[k] * g;
-                }
-                continue;
-            }
-            const float* c4 = com4 + 4 * ni;
-            float d[3] = {c4[0] - pi[0], c4[1] - pi[1], c4[2] - pi[2]};
-            float r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + soft2;
-            float dist = std::sqrt(r2);
-            float e[3], cen[3];
-            for (int k = 0; k < 3; ++k) {
-                e[k] = bmax3[3 * ni + k] - bmin3[3 * ni + k];
-                cen[k] = 0.5f * (bmin3[3 * ni + k] + bmax3[3 * ni + k]) - c4[k];
-            }
-            float size = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-            float delta = std::sqrt(cen[0] * cen[0] + cen[1] * cen[1] + cen[2] * cen[2]);
-            if (dist >= size / theta + delta) {
-                float g = G32 * c4[3] / (r2 * dist);
-                for (int k = 0; k < 3; ++k) a[k] += d[k] * g;
-                continue;
-            }
-            if (sp + 2 <= 64) {
-                stack[sp++] = left[ni];
-                stack[sp++] = right[ni];
-            }
-        }
+        float a[3];
+        uint64_t nc = 0, np = 0;
+        lbvh_walk_one(t, sbodies, n, K, L, left, right, com4, bmin3, bmax3, theta, soft2, a, nc, np);
         uint32_t o = order[t];
         for (int k = 0; k < 3; ++k) acc3[3 * o + k] = a[k];
+    }
+}
+
+// The same walk for a subset of sorted targets, returning per target the accepted cells and the
+// particle-particle terms (counts2[2k], counts2[2k + 1]): the reference walk's own interaction count.
+void orc_lbvh_walk_counts(const float* sbodies, uint64_t n, uint32_t K, uint32_t L, const uint32_t* left,
+                          const uint32_t* right, const float* com4, const float* bmin3, const float* bmax3,
+                          float theta, float soft2, const uint32_t* targets, uint64_t n_targets,
+                          uint64_t* counts2) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (uint64_t k = 0; k < n_targets; ++k) {
+        float a[3];
+        uint64_t nc = 0, np = 0;
+        lbvh_walk_one(targets[k], sbodies, n, K, L, left, right, com4, bmin3, bmax3, theta, soft2, a, nc, np);
+        counts2[2 * k] = nc;
+        counts2[2 * k + 1] = np;
     }
 }
 

@@ -28,6 +28,8 @@ _SIGS = {
     "orc_upsweep_f32": (None, [_P, _U64, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P]),
     "orc_lbvh_gravity_f32": (None, [_P, _U64, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
                                     ctypes.c_float, ctypes.c_float, _P, _P]),
+    "orc_lbvh_walk_counts": (None, [_P, _U64, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P, _P,
+                                    ctypes.c_float, ctypes.c_float, _P, _U64, _P]),
     "orc_set_threads": (None, [ctypes.c_int]),
     "orc_get_threads": (ctypes.c_int, []),
     "orc_direct_gravity_f64": (None, [_P, _P, _U64, _D, _P]),
@@ -182,6 +184,29 @@ def lbvh_gravity_f32(pos, mass, soft, theta=0.5, K=1):
                                _p(c64(bmax, np.float32)), float(theta), float(np.float32(soft) * np.float32(soft)),
                                _p(c64(order, np.uint32)), _p(acc))
     return acc
+
+
+def lbvh_walk_counts(pos, mass, soft, targets, theta=0.5):
+    """The reference walk's interactions for the given targets (original indices): (cells, particle terms)
+    per target, from the same LBVH leg as lbvh_gravity_f32 (leaf size 1)."""
+    p = c64(pos, np.float32).reshape(-1, 3)
+    m = c64(mass, np.float32)
+    lo, hi = bbox_f32(p)
+    sc, order = sort_pairs(morton_f32(p, lo, hi))
+    lc = cluster_codes(sc, 1)
+    left, right, _ = karras(lc)
+    sb = np.concatenate([p[order], m[order, None]], axis=1).astype(np.float32)
+    com, bmin, bmax = upsweep_f32(sb, 1, left, right)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order), dtype=order.dtype)
+    tgt = c64(inv[np.asarray(targets)], np.uint32)
+    counts = np.zeros((len(tgt), 2), np.uint64)
+    l = c64(left, np.uint32) if len(left) else np.zeros(1, np.uint32)
+    r = c64(right, np.uint32) if len(right) else np.zeros(1, np.uint32)
+    lib().orc_lbvh_walk_counts(_p(sb), p.shape[0], 1, lc.shape[0], _p(l), _p(r), _p(com), _p(c64(bmin, np.float32)),
+                               _p(c64(bmax, np.float32)), float(theta), float(np.float32(soft) * np.float32(soft)),
+                               _p(tgt), len(tgt), _p(counts))
+    return counts
 
 
 def lbvh_accel(pos, mass, soft, theta=0.5):

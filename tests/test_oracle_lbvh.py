@@ -213,3 +213,20 @@ def test_cluster_codes_and_every_node_moment_by_brute_force():
             assert abs(com[v, 3] - mt) <= 1e-5 * mt
             assert np.allclose(com[v, :3], c, rtol=0, atol=1e-5 * np.abs(q[:, :3]).max())
             assert np.array_equal(bmin[v], sb[a:b + 1, :3].min(0)) and np.array_equal(bmax[v], sb[a:b + 1, :3].max(0))
+
+
+def test_walk_counts_closed_forms():
+    """The reference walk's interaction count: with theta -> 0 every node opens, so each target sums the
+    other N - 1 particles and no cell; the walk that counts is the walk that computes (its forces equal
+    lbvh_gravity_f32's, which the published-accuracy test pins); with a far target and an enormous theta the
+    root is accepted at once (one cell)."""
+    pos, mass = splitmix_cloud(700, 0x5EED)
+    n = len(mass)
+    c = oracle.lbvh_walk_counts(pos, mass, 5e3, np.arange(n), theta=1e-9)
+    assert np.all(c[:, 0] == 0) and np.all(c[:, 1] == n - 1)
+    pos2 = np.vstack([pos, [[1e12, 0.0, 0.0]]])
+    mass2 = np.append(mass, 1.0)
+    c2 = oracle.lbvh_walk_counts(pos2, mass2, 5e3, [n], theta=1e6)
+    assert c2[0, 0] >= 1 and c2[0, 0] + c2[0, 1] <= 64
+    half = oracle.lbvh_walk_counts(pos, mass, 5e3, np.arange(n), theta=0.5)
+    assert np.all(half.sum(1) < n - 1) and np.all(half[:, 0] > 0)
