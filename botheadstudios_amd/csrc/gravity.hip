@@ -23,6 +23,26 @@ constexpr int kStackCap = ISPH_GRAVITY_STACK;
 constexpr int kStackMargin = 128;  // > maximum Karras depth (30 code bits + 32 index tie-break bits)
 constexpr int kBuf = 128;
 
+// Lane slots of a one-particle-per-lane monopole walk. The group's g particles and q strided chunks of each
+// source batch (chunk c = batch entries c, c + q, c + 2q, ...) make q g (particle, chunk) slots: slot s is
+// (particle s % g, chunk s / g), and lane l holds slots l, l + 64, ... (R = ceil(q g / 64) rounds). A
+// batch of 64 sources then costs ceil(64 / q) R steps instead of 64. split_shape picks the q with the
+// fewest steps among R <= rounds (q = 1: one slot per lane, the particle itself).
+__device__ __forceinline__ void split_shape(uint32_t g, int rounds, int& q, int& R) {
+    q = 1;
+    R = 1;
+    int best = kWave;
+    for (int qq = 2; qq <= 16; ++qq) {
+        const int rr = (int)((qq * g + kWave - 1) / kWave);
+        const int cost = (kWave + qq - 1) / qq * rr;
+        if (rr <= rounds && cost < best) {
+            best = cost;
+            q = qq;
+            R = rr;
+        }
+    }
+}
+
 // Monopole walks put leaf particles into the cell list: with softening, a particle's own term has d = 0 and
 // adds exactly zero, so the reference's s != t test (bh_gravity.wgsl:362) is only needed without softening
 // (SELF), where the list also carries each entry's particle index. One list keeps the workgroup's LDS at
@@ -37,6 +57,10 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
                                                      const uint32_t* __restrict__ sinst,
                                                      const uint32_t* __restrict__ iroot) {
     constexpr bool UNI = !QUAD;
+    constexpr bool SPLIT = P == 1 && UNI;          // lane slots (split_shape)
+    constexpr int NR = SPLIT ? ((COMP || SELF) ? 2 : 3) : 1;  // slot rounds held per lane (no spills)
+    constexpr int NA = SPLIT ? NR : P;              // accumulators per lane: one per slot or per particle
+    static_assert(2 * NR * kWave <= kStackCap, "slot partials are reduced through the stack's LDS");
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 cA[kBuf];
     __shared__ float4 cQ0[QUAD ? kBuf : 1];
@@ -53,30 +77,38 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
     uint32_t gi;
     while (queue.next(gi)) {
         const uint2 grp = groups[gi];
-        // A group of at most 32 particles takes both halves of the wave: lane l and lane l + 32 hold the same
-        // particle and split each batch of sources in two (two LDS broadcast addresses per step), so the
-        // batch costs half the steps; the halves' sums are added at the end.
-        const bool split = P == 1 && UNI && grp.y <= (uint32_t)(kWave / 2);
+        // Lane l holds particle l (+ 64 k for P = 2); with SPLIT its slot r takes a particle (read from that
+        // particle's lane by a shuffle) and a batch chunk. Slots past q g are idle and take chunk 0; their
+        // sums are never read.
+        int q = 1, R = 1;
+        if (SPLIT) split_shape(grp.y, NR, q, R);
+        uint32_t slot_pc[NR];  // particle << 8 | chunk
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const uint32_t s = (uint32_t)(r * kWave + lane);
+            const bool used = s < (uint32_t)q * grp.y;
+            slot_pc[r] = SPLIT ? (s % grp.y) << 8 | (used ? s / grp.y : 0u) : (uint32_t)lane << 8;
+        }
         uint32_t t[P];
         bool valid[P];
         float4 me[P];
         float lx0 = INFINITY, ly0 = INFINITY, lz0 = INFINITY, lx1 = -INFINITY, ly1 = -INFINITY, lz1 = -INFINITY;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
-            const uint32_t slot = split ? ((uint32_t)lane & (kWave / 2 - 1)) : (uint32_t)lane + (uint32_t)k * kWave;
+            const uint32_t slot = (uint32_t)lane + (uint32_t)k * kWave;
             t[k] = grp.x + slot;
             valid[k] = slot < grp.y;
             me[k] = spos[valid[k] ? t[k] : grp.x];
             lx0 = fminf(lx0, me[k].x); ly0 = fminf(ly0, me[k].y); lz0 = fminf(lz0, me[k].z);
             lx1 = fmaxf(lx1, me[k].x); ly1 = fmaxf(ly1, me[k].y); lz1 = fmaxf(lz1, me[k].z);
         }
-        const float gx0 = wave_min(lx0), gy0 = wave_min(ly0), gz0 = wave_min(lz0);
-        const float gx1 = wave_max(lx1), gy1 = wave_max(ly1), gz1 = wave_max(lz1);
+        const float gx0 = uniform_f(wave_min(lx0)), gy0 = uniform_f(wave_min(ly0)), gz0 = uniform_f(wave_min(lz0));
+        const float gx1 = uniform_f(wave_max(lx1)), gy1 = uniform_f(wave_max(ly1)), gz1 = uniform_f(wave_max(lz1));
 
-        float ax[P], ay[P], az[P];
-        KahanF kx[P], ky[P], kz[P];
+        float ax[NA], ay[NA], az[NA];
+        KahanF kx[NA], ky[NA], kz[NA];
 #pragma unroll
-        for (int k = 0; k < P; ++k) {
+        for (int k = 0; k < NA; ++k) {
             ax[k] = ay[k] = az[k] = 0.f;
             kx[k] = KahanF{0.f, 0.f};
             ky[k] = KahanF{0.f, 0.f};
@@ -95,12 +127,45 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
                 az[k] += bz;
             }
         };
+        // SPLIT: slot r walks its chunk's entries cr[r], cr[r] + q, ... of the batch; every chunk has cnt / q
+        // whole steps, and the last step exists for the chunks below cnt % q only.
+        auto flush_split = [&](int cnt) {
+            const int full = cnt / q;
+            const bool tail = full * q < cnt;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (r >= R) break;
+                const int pr = (int)(slot_pc[r] >> 8);
+                const float mx = __shfl(me[0].x, pr), my = __shfl(me[0].y, pr), mz = __shfl(me[0].z, pr);
+                const uint32_t tt = grp.x + (uint32_t)pr;
+                float bx = 0.f, by = 0.f, bz = 0.f;
+                auto source = [&](int j, bool in) {
+                    const float4 c = cA[j];
+                    float dx = c.x - mx, dy = c.y - my, dz = c.z - mz;
+                    float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+                    float inv = __builtin_amdgcn_rsqf(r2);
+                    float w = c.w * inv * (inv * inv);
+                    if (SELF) w = (cI[j] == tt) ? 0.f : w;  // s != t (bh_gravity.wgsl:362)
+                    if (!in) w = 0.f;
+                    bx += dx * w;
+                    by += dy * w;
+                    bz += dz * w;
+                };
+                int j = (int)(slot_pc[r] & 0xffu);
+                for (int k = 0; k < full; ++k, j += q) source(j, true);
+                if (tail) source(j < cnt ? j : 0, j < cnt);
+                add(r, bx, by, bz);
+            }
+        };
         auto flush_cells = [&](int cnt) {
+            if (SPLIT) {
+                flush_split(cnt);
+                return;
+            }
             float bx[P], by[P], bz[P];
 #pragma unroll
             for (int k = 0; k < P; ++k) bx[k] = by[k] = bz[k] = 0.f;
-            // one source for this lane's particle(s); `in` is false only past the end of an odd split batch
-            auto source = [&](int j, bool in) {
+            auto source = [&](int j) {
                 const float4 c = cA[j];
                 const uint32_t idx = (UNI && SELF) ? cI[j] : 0u;
                 float4 q0;
@@ -129,23 +194,13 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
                         bz[k] += w * (dz * s - qz * inv2);
                     } else {
                         if (UNI && SELF) w = (idx == t[k]) ? 0.f : w;  // s != t (bh_gravity.wgsl:362)
-                        if (!in) w = 0.f;
                         bx[k] += dx * w;
                         by[k] += dy * w;
                         bz[k] += dz * w;
                     }
                 }
             };
-            if (split) {
-                const int h = (cnt + 1) >> 1;
-                const int off = lane >= kWave / 2 ? h : 0;
-                for (int j = 0; j < h; ++j) {
-                    const bool in = j + off < cnt;
-                    source(in ? j + off : j, in);
-                }
-            } else {
-                for (int j = 0; j < cnt; ++j) source(j, true);
-            }
+            for (int j = 0; j < cnt; ++j) source(j);
 #pragma unroll
             for (int k = 0; k < P; ++k) add(k, bx[k], by[k], bz[k]);
         };
@@ -318,22 +373,35 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
         }
         if (nc > 0) flush_cells(nc);
         if (np > 0) flush_parts(np);
-        if (split) {  // lane l adds the upper half's sums (lane l + 32); a compensated sum takes s - c
-            if (COMP) {
-                const float sx = __shfl_down(kx[0].s, kWave / 2), cx = __shfl_down(kx[0].c, kWave / 2);
-                const float sy = __shfl_down(ky[0].s, kWave / 2), cy = __shfl_down(ky[0].c, kWave / 2);
-                const float sz = __shfl_down(kz[0].s, kWave / 2), cz = __shfl_down(kz[0].c, kWave / 2);
-                kx[0].add(sx);
-                kx[0].add(-cx);
-                ky[0].add(sy);
-                ky[0].add(-cy);
-                kz[0].add(sz);
-                kz[0].add(-cz);
-            } else {
-                ax[0] += __shfl_down(ax[0], kWave / 2);
-                ay[0] += __shfl_down(ay[0], kWave / 2);
-                az[0] += __shfl_down(az[0], kWave / 2);
-            }
+        if (SPLIT && q > 1) {
+            // Particle p's sum is its q slots c g + p, added in chunk order through the stack's LDS (slot s at
+            // red[s], Kahan carries at red[NR 64 + s]); a compensated sum takes s - c.
+            float* red = reinterpret_cast<float*>(stk);
+            const uint32_t g = grp.y;
+            auto reduce = [&](float (&a)[NA], KahanF (&kk)[NA]) {
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    if (r >= R) break;
+                    red[r * kWave + lane] = COMP ? kk[r].s : a[r];
+                    if (COMP) red[NR * kWave + r * kWave + lane] = kk[r].c;
+                }
+                __syncthreads();
+                if ((uint32_t)lane < g) {
+                    for (int c = 1; c < q; ++c) {
+                        const uint32_t s = (uint32_t)c * g + (uint32_t)lane;
+                        if (COMP) {
+                            kk[0].add(red[s]);
+                            kk[0].add(-red[NR * kWave + s]);
+                        } else {
+                            a[0] += red[s];
+                        }
+                    }
+                }
+            };
+            reduce(ax, kx);
+            reduce(ay, ky);
+            reduce(az, kz);
         }
 #pragma unroll
         for (int k = 0; k < P; ++k) {
@@ -342,7 +410,7 @@ __global__ __launch_bounds__(64, (P == 1 && !QUAD) ? 8 : 6) void k_gravity_tree(
                 ay[k] = ky[k].s;
                 az[k] = kz[k].s;
             }
-            if (valid[k] && (!split || lane < kWave / 2)) sgrav[t[k]] = make_float4(kG * ax[k], kG * ay[k], kG * az[k], 0.f);
+            if (valid[k]) sgrav[t[k]] = make_float4(kG * ax[k], kG * ay[k], kG * az[k], 0.f);
         }
         tot_cell += ncell * grp.y;
         tot_part += npart * grp.y;
