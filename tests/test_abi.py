@@ -55,3 +55,23 @@ def test_create_reports_errors_through_last_error():
     # every entry point rejects a NULL handle without touching a device
     assert lib.isph_set_dt(None, ctypes.c_float(1.0), ctypes.c_float(1.0)) == -22
     assert lib.isph_count(None) == 0
+
+
+def test_rust_binding_declares_every_entry_point_with_the_header_arity():
+    """The isph-sys extern block a maintainer would add names each declared function once, with as many
+    parameters as the C prototype."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = text[text.index('extern "C" {'):]
+    block = block[:block.index("\n}\n")]
+    rust = dict(re.findall(r"pub fn (isph_[a-z0-9_]+)\(([^)]*)\)", block.replace("\n", " ")))
+    header = open(HEADER).read().replace("\n", " ")
+    c = {}
+    for name, params in re.findall(r"\b(isph_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", header):
+        c[name] = params
+    assert set(rust) == set(c) == set(declared())
+
+    def arity(params):
+        params = params.strip()
+        return 0 if params in ("", "void") else params.count(",") + 1
+    for name in c:
+        assert arity(rust[name]) == arity(c[name]), name
