@@ -45,7 +45,9 @@ __device__ __forceinline__ float eos_point(uint32_t t, double rho, float h, floa
     return h / fmaxf((float)c + speed, 1.0f);  // sph_step.wgsl:370, hydrostatic.rs:477
 }
 
-// Candidate-list cache: the density walk stores each group's candidate batches, the force pass replays them.
+// Candidate-list cache: the density walk stores, per group, the candidates that pair with some member (in
+// walk order, packed into chunks of 64 slots) and each lane's pair mask over a chunk; the force pass replays
+// them.
 struct CandCache {
     uint32_t* head;
     uint32_t* next;
@@ -103,6 +105,43 @@ struct CacheWriter {
     }
 };
 
+// OR over the 64 lanes (all active), returned wave-uniform: DPP row shifts within each row of 16, then
+// row_bcast:15 and row_bcast:31 carry rows 0 | 1 and 2 | 3 into lane 63.
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+__device__ __forceinline__ uint64_t wave_or_u64(uint64_t v) {
+    return ((uint64_t)wave_or_u32((uint32_t)(v >> 32)) << 32) | wave_or_u32((uint32_t)v);
+}
+// The bits of x selected by m, packed towards bit 0 in order (bit j of x & m lands at the count of m's bits
+// below j): the six-step parallel compress (Warren, Hacker's Delight, 2nd ed., section 7-4). Everything that
+// depends on m alone is wave-uniform when m is.
+__device__ __forceinline__ uint64_t compress_bits(uint64_t x, uint64_t m) {
+    x &= m;
+    uint64_t mk = ~m << 1;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        uint64_t mp = mk ^ (mk << 1);
+        mp ^= mp << 2;
+        mp ^= mp << 4;
+        mp ^= mp << 8;
+        mp ^= mp << 16;
+        mp ^= mp << 32;
+        const uint64_t mv = mp & m;
+        m = (m ^ mv) | (mv >> (1 << i));
+        const uint64_t t = x & mv;
+        x = (x ^ t) | (t >> (1 << i));
+        mk &= ~mp;
+    }
+    return x;
+}
+
 // REF (the f64 reference form, F64 too): the pair terms take f64 coordinates, each particle's f32 copy plus
 // its f32 residual against the f64 state (x64 = x32 + lo), so differences are the f64 state's to ~1e-16.
 template <bool F64, bool REF = false>
@@ -122,6 +161,7 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     __shared__ float cM[kWave];   // mass
     __shared__ float4 cL[REF ? kWave : 1];  // position residuals (reference form)
     __shared__ uint32_t own[kWave];
+    __shared__ uint32_t pend[2 * kWave];  // candidate ids of the chunk being filled, and the overflow
     const int lane = threadIdx.x;
     own[lane] = kWave;
     unsigned long long tested_all = 0;
@@ -141,9 +181,10 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     const float4 melo = REF ? slo[tc] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float hi = sh[tc];
     const float qi = 0.5f * hi;  // half smoothing length, as the candidates carry it
-    const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
-    const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
-    const float hg = wave_max(hi);
+    const float gx0 = uniform_f(wave_min(me.x)), gy0 = uniform_f(wave_min(me.y)), gz0 = uniform_f(wave_min(me.z));
+    const float gx1 = uniform_f(wave_max(me.x)), gy1 = uniform_f(wave_max(me.y)), gz1 = uniform_f(wave_max(me.z));
+    const float hg = uniform_f(wave_max(hi));
+    const float qg = 0.5f * hg;
     // Plain: f32 pair arithmetic and sum (the reference shader's arithmetic). Mixed (F64): the density feeds
     // the Tillotson pressure, whose cold branch P = A (rho/rho0 - 1) + ... cancels near rho0 and amplifies a
     // relative density error by A / P (~1e3), so the true-pair kernel sums run in f64, from f64 differences
@@ -151,30 +192,49 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     float rho = 0.f;
     double rho64 = 0.0;
     unsigned long long tested = 0;
+    // The candidate cache keeps only the candidates that pair with some member of the group, in walk order,
+    // packed into chunks of 64 slots (`pc` slots of the pending chunk are filled); `pm` is this lane's pair
+    // mask over those slots. A lane's hits keep their order, so the force pass's sums are the walk's.
+    uint64_t pm = 0ull;
+    int pc = 0;
     cw.begin();
     neighbour_walk(stk, cand, n, K, L, nS, expand, gx0, gy0, gz0, gx1, gy1, gz1, hg,
                    counters ? counters + (size_t)blockIdx.x * kCounterSlots + 8 : nullptr, [&](int cnt) {
-        cw.put(cc, scal, cand, cnt, lane);
+        // 0) Candidates that pass the prefilter (near_group), packed in walk order (cand[] in place: every
+        //    lane has read its entry before any is overwritten).
+        bool pass = false;
+        uint32_t j = 0;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        float qj = 0.f;
         if (lane < cnt) {
-            uint32_t j = cand[lane];
-            const float4 p = spos[j];
-            cP[lane] = make_float4(p.x, p.y, p.z, 0.5f * sh[j]);
-            cM[lane] = p.w;
-            if (REF) cL[lane] = slo[j];
-            note_owner(own, j, grp.x, grp.y, lane);
+            j = cand[lane];
+            p = spos[j];
+            qj = 0.5f * sh[j];
+            pass = near_group(p, qj, gx0, gy0, gz0, gx1, gy1, gz1, qg);
+        }
+        const uint64_t sv = __ballot(pass);
+        const int nc = __popcll(sv);
+        if (pass) {
+            const int at = prefix_count(sv);
+            cand[at] = j;
+            cP[at] = make_float4(p.x, p.y, p.z, qj);
+            cM[at] = p.w;
+            if (REF) cL[at] = slo[j];
+            note_owner(own, j, grp.x, grp.y, at);
         }
         __syncthreads();
         // 1) The radius test against every candidate (LDS broadcast reads): one bit per true pair, the
         //    lane's own particle excluded (sph_step.wgsl:210).
-        const uint64_t hit = (split ? pair_mask_half(cP, cnt, me, qi, lane >= kWave / 2) : pair_mask(cP, cnt, me, qi)) &
-                             ~self_bit(own, cand, cnt, member, t);
-        // the force pass of this evaluation replays the chunk with this mask (its pair set is the same)
-        if (cw.ok) cc.mask[(size_t)cw.last * kWave + lane] = hit;
-        // 2) The kernel sum over this lane's hits only, in candidate order. Lanes walk their own hit lists
-        //    in step, so the wave runs as many iterations as its busiest lane instead of one per candidate
-        //    with most lanes idle.
+        const uint64_t hit = (split ? pair_mask_half(cP, nc, me, qi, lane >= kWave / 2) : pair_mask(cP, nc, me, qi)) &
+                             ~self_bit(own, cand, nc, member, t);
+        // The kept candidates (a pair of some member) go to the pending chunk in walk order.
+        const uint64_t keep = cw.ok ? wave_or_u64(valid ? hit : 0ull) : 0ull;
+        if (lane < nc && ((keep >> lane) & 1ull)) pend[pc + prefix_count(keep)] = cand[lane];
+        // 2) The kernel sum over this lane's hits only, in candidate order. Lanes walk their own hits in
+        //    step, so the wave runs as many iterations as its busiest lane instead of one per candidate with
+        //    most lanes idle.
         float acc = 0.f;
-        uint64_t rest = hit;
+        uint64_t rest = valid ? hit : 0ull;
         while (__ballot(rest != 0ull)) {
             if (rest) {
                 const int jj = __builtin_ctzll(rest);
@@ -207,9 +267,32 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
                 }
             }
         }
+        // the force pass of this evaluation replays the kept candidates with these masks (its pair set is
+        // the same)
+        if (cw.ok) {
+            const int k = __popcll(keep);
+            const uint64_t cm = compress_bits(valid ? hit : 0ull, keep);  // the hits' slots among the kept
+            pm |= cm << pc;  // pc < 64
+            const uint64_t carry = pc + k > kWave ? cm >> (kWave - pc) : 0ull;  // then pc >= 1
+            pc += k;
+            if (pc >= kWave) {
+                __syncthreads();
+                cw.put(cc, scal, pend, kWave, lane);
+                if (cw.ok) cc.mask[(size_t)cw.last * kWave + lane] = pm;
+                pm = carry;
+                pc -= kWave;
+                __syncthreads();
+                if (lane < pc) pend[lane] = pend[lane + kWave];
+            }
+        }
         rho += acc;
-        tested += cnt;
+        tested += nc;
     }, sinst ? iroot[sinst[grp.x]] : 0u);
+    if (cw.ok && pc > 0) {
+        __syncthreads();
+        cw.put(cc, scal, pend, pc, lane);
+        if (cw.ok) cc.mask[(size_t)cw.last * kWave + lane] = pm;
+    }
     cw.end(cc, gi, lane);
     if (split) {  // the upper half's sum joins its member's lane
         rho += __shfl_down(rho, kWave / 2);
@@ -337,9 +420,10 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
     const float4 melo = REF ? slo[tc] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 vilo = REF ? svlo[tc] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float rhoi = srho[tc], pri = spr[tc], ci = scs[tc];
-    const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
-    const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
-    const float hg = wave_max(hi);
+    const float gx0 = uniform_f(wave_min(me.x)), gy0 = uniform_f(wave_min(me.y)), gz0 = uniform_f(wave_min(me.z));
+    const float gx1 = uniform_f(wave_max(me.x)), gy1 = uniform_f(wave_max(me.y)), gz1 = uniform_f(wave_max(me.z));
+    const float hg = uniform_f(wave_max(hi));
+    const float qg = 0.5f * hg;
     // Plain: f32 pair arithmetic and sums (the reference shader's). Mixed (F64): the true pairs (about 50
     // per particle) are evaluated and summed in f64 from f64 coordinate differences, with the f64
     // density and P / rho^2 of the EOS pass; du/dt cancels across pairs, so f32 pair terms cap its
@@ -363,18 +447,18 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
         c.cs = scs[j];
         return c;
     };
-    auto put = [&](const Cand& c, uint32_t j) {
-        cP[lane] = make_float4(c.p.x, c.p.y, c.p.z, 0.5f * c.h);
-        cV[lane] = c.v;
-        cS[lane] = make_float4(c.p.w, c.pr, c.cs, 0.f);
+    auto put = [&](const Cand& c, uint32_t j, int at) {
+        cP[at] = make_float4(c.p.x, c.p.y, c.p.z, 0.5f * c.h);
+        cV[at] = c.v;
+        cS[at] = make_float4(c.p.w, c.pr, c.cs, 0.f);
         if (F64) {
-            cR64[lane] = srho64[j];
-            cPr64[lane] = spr64[j];
-            cC64[lane] = scs64[j];
+            cR64[at] = srho64[j];
+            cPr64[at] = spr64[j];
+            cC64[at] = scs64[j];
         }
         if (REF) {
-            cL[lane] = slo[j];
-            cVL[lane] = svlo[j];
+            cL[at] = slo[j];
+            cVL[at] = svlo[j];
         }
     };
     // The pair terms of one staged batch over `rest`, this lane's hits. 1) The radius test is r < (h_i +
@@ -463,15 +547,26 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
         }
         tested += cnt;
     };
-    auto flush = [&](int cnt) {  // a batch of the walk: load, stage, test, pair terms
+    auto flush = [&](int cnt) {  // a batch of the walk: load, prefilter and pack as the density walk, test
+        bool pass = false;
+        uint32_t j = 0;
+        Cand c{};
         if (lane < cnt) {
-            const uint32_t j = cand[lane];
-            put(fetch(j), j);
-            note_owner(own, j, grp.x, grp.y, lane);
+            j = cand[lane];
+            c = fetch(j);
+            pass = near_group(c.p, 0.5f * c.h, gx0, gy0, gz0, gx1, gy1, gz1, qg);
+        }
+        const uint64_t sv = __ballot(pass);
+        const int nc = __popcll(sv);
+        if (pass) {
+            const int at = prefix_count(sv);
+            cand[at] = j;
+            put(c, j, at);
+            note_owner(own, j, grp.x, grp.y, at);
         }
         __syncthreads();
-        pairs(cnt, (split ? pair_mask_half(cP, cnt, me, qi, lane >= kWave / 2) : pair_mask(cP, cnt, me, qi)) &
-                       ~self_bit(own, cand, cnt, member, t));
+        pairs(nc, (split ? pair_mask_half(cP, nc, me, qi, lane >= kWave / 2) : pair_mask(cP, nc, me, qi)) &
+                      ~self_bit(own, cand, nc, member, t));
     };
     // The density walk of this evaluation stored the group's candidates batch by batch: replay them in the
     // same order (so the sums are the walk's, bit for bit), or walk when the list did not fit.
@@ -497,7 +592,7 @@ __global__ __launch_bounds__(64, F64 ? 4 : 5) void k_force(const float4* __restr
             m1 = cmask[(size_t)nx * kWave + lane];
         }
         while (true) {
-            if ((uint32_t)lane < cnt) put(fetch(cj), cj);
+            if ((uint32_t)lane < cnt) put(fetch(cj), cj, lane);
             uint32_t cnt2 = 0, nx2 = kNoChunk, cj2 = 0;
             uint64_t m2 = 0ull;
             if (nx1 != kNoChunk) {

@@ -141,6 +141,21 @@ __device__ __forceinline__ uint64_t pair_mask(const float4* cP, int cnt, float4 
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The candidate prefilter of the SPH walks: a candidate (x, y, z, h_j / 2) can pair with a member of the
+// group only if its distance to the group box is below (max h_i + h_j) / 2. The walk keeps whole subtrees of
+// up to `expand` particles, so some candidates fail it. The margin (1e-4 of the squared radius) is far above
+// the few ulps by which the f32 radius test and this bound can differ, so no pair is lost. The density walk
+// and the force pass's own walk drop the same candidates and pack the rest in walk order, so both see the
+// same batches.
+__device__ __forceinline__ bool near_group(float4 p, float qj, float gx0, float gy0, float gz0, float gx1,
+                                           float gy1, float gz1, float qg) {
+    const float dx = fmaxf(fmaxf(gx0 - p.x, p.x - gx1), 0.f);
+    const float dy = fmaxf(fmaxf(gy0 - p.y, p.y - gy1), 0.f);
+    const float dz = fmaxf(fmaxf(gz0 - p.z, p.z - gz1), 0.f);
+    const float rr = qg + qj;
+    return dx * dx + dy * dy + dz * dz <= rr * rr * (1.0f + 1.0e-4f);
+}
+
 // While loading candidate `cand[lane]`, record where a particle of the lane's own group sits in this batch
 // (own[d] = slot for group member d). A lane then finds its own particle with one LDS read and a check,
 // instead of comparing indices at every candidate. Stale entries from earlier batches fail the check.
