@@ -116,6 +116,17 @@ void dalloc(T** p, size_t count) {
     HIP_OK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
 }
 
+// Host-to-device transfers and fills that the handle's kernels read are ordered on the handle's stream. That
+// stream is non-blocking, so it is not ordered after work on the legacy null stream, and a synchronous
+// hipMemcpy from pageable host memory may return once the data is staged, before its DMA has landed: a kernel
+// launched next on the stream could read the old contents of the destination (for an upload, material ids
+// that index the EOS table among them).
+void h2d(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));  // the host buffer may be a temporary
+}
+void fill0(hipStream_t s, void* dst, size_t bytes) { HIP_OK(hipMemsetAsync(dst, 0, bytes, s)); }
+
 void free_all(isph_ctx* c) {
     Dev& d = c->d;
     void* ptrs[] = {d.posh, d.velu, d.mass, d.mat, d.prov, d.rho, d.acc, d.pos64, d.vel64, d.keys, d.vals,
@@ -197,7 +208,7 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     }
     dalloc(&d.gflag, cap);
     dalloc(&d.gscan, cap / kWave + 16);
-    HIP_OK(hipMemset(d.gflag, 0, (size_t)cap * sizeof(uint32_t)));
+    fill0(c->stream, d.gflag, (size_t)cap * sizeof(uint32_t));
     dalloc(&d.nG, 2 * nodes);
 
     dalloc(&d.nA, nodes);
@@ -210,17 +221,18 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     dalloc(&d.counters, (size_t)kCounterSlots * kWalkWaves);
     dalloc(&d.red, 1024 * 16);
     dalloc(&d.inject, inject_slots(d));
-    HIP_OK(hipMemset(d.inject, 0, inject_slots(d) * sizeof(double)));
+    fill0(c->stream, d.inject, inject_slots(d) * sizeof(double));
     dalloc(&d.eos, kMaxMaterials);
     dalloc(&d.ext, kMaxExtBodies);
     dalloc(&d.mtarget, cap);
     HIP_OK(hipMalloc(&d.aos, (size_t)cap * sizeof(isph_particle)));
     HIP_OK(hipHostMalloc((void**)&c->pinned, (size_t)cap * sizeof(isph_particle), hipHostMallocDefault));
     HIP_OK(hipEventCreateWithFlags(&c->rb_event, hipEventDisableTiming));
-    HIP_OK(hipMemset(d.counters, 0, (size_t)kCounterSlots * kWalkWaves * sizeof(unsigned long long)));
-    HIP_OK(hipMemset(d.time, 0, sizeof(double)));
-    HIP_OK(hipMemset(d.dt, 0, sizeof(float)));
-    HIP_OK(hipMemset(d.eos, 0, kMaxMaterials * sizeof(Till)));
+    fill0(c->stream, d.counters, (size_t)kCounterSlots * kWalkWaves * sizeof(unsigned long long));
+    fill0(c->stream, d.time, sizeof(double));
+    fill0(c->stream, d.dt, sizeof(float));
+    fill0(c->stream, d.eos, kMaxMaterials * sizeof(Till));
+    HIP_OK(hipStreamSynchronize(c->stream));
 }
 
 void ensure64(isph_ctx* c) {
@@ -231,7 +243,7 @@ void ensure64(isph_ctx* c) {
         dalloc(&c->d.spr64, c->d.cap);
         dalloc(&c->d.scs64, c->d.cap);
         dalloc(&c->d.acc64, 3 * (size_t)c->d.cap);
-        HIP_OK(hipMemset(c->d.acc64, 0, 3 * (size_t)c->d.cap * sizeof(double)));
+        fill0(c->stream, c->d.acc64, 3 * (size_t)c->d.cap * sizeof(double));
     }
 }
 
@@ -477,7 +489,7 @@ void widen_eos(isph_ctx* c, const isph_eos* eos, size_t n) {
         t[i] = Till{eos[i].rho0, eos[i].a, eos[i].b, eos[i].cap_a, eos[i].cap_b, eos[i].e0, eos[i].e_iv, eos[i].e_cv,
                     eos[i].alpha, eos[i].beta};
     }
-    HIP_OK(hipMemcpy(c->d.eos, t, sizeof(t), hipMemcpyHostToDevice));
+    h2d(c->stream, c->d.eos, t, sizeof(t));
 }
 
 // ---- adjoint driver --------------------------------------------------------------------------------
@@ -668,7 +680,7 @@ int isph_upload(isph_t* h, const isph_particle* particles, size_t n, const isph_
         HIP_OK(hipSetDevice(h->device));
         HIP_OK(hipStreamSynchronize(h->stream));
         if (take > 0) {
-            HIP_OK(hipMemcpy(h->d.aos, particles, take * sizeof(isph_particle), hipMemcpyHostToDevice));
+            h2d(h->stream, h->d.aos, particles, take * sizeof(isph_particle));
             launch_unpack(h->d, (uint32_t)take, h->stream);
         }
         if (ne > 0) widen_eos(h, eos, ne);
@@ -729,10 +741,10 @@ int isph_upload_ensemble(isph_t* h, const isph_particle* particles, const uint32
         std::vector<uint32_t> inst(n);
         for (uint32_t k = 0; k < n_inst; ++k)
             for (uint32_t i = offsets[k]; i < offsets[k + 1]; ++i) inst[i] = k;
-        HIP_OK(hipMemcpy(d.inst, inst.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.ioff, offsets, (n_inst + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
-        HIP_OK(hipMemset(d.idt, 0, n_inst * sizeof(float)));
-        HIP_OK(hipMemset(d.itime, 0, n_inst * sizeof(double)));
+        h2d(h->stream, d.inst, inst.data(), n * sizeof(uint32_t));
+        h2d(h->stream, d.ioff, offsets, (n_inst + 1) * sizeof(uint32_t));
+        fill0(h->stream, d.idt, n_inst * sizeof(float));
+        fill0(h->stream, d.itime, n_inst * sizeof(double));
         d.ninst = n_inst;
         h->hioff.assign(offsets, offsets + n_inst + 1);
         h->reuse_valid = false;
@@ -768,7 +780,7 @@ int isph_set_eos_f64(isph_t* h, const double* params10, size_t n_eos) {
         }
         HIP_OK(hipSetDevice(h->device));
         HIP_OK(hipStreamSynchronize(h->stream));
-        HIP_OK(hipMemcpy(h->d.eos, t, sizeof(t), hipMemcpyHostToDevice));
+        h2d(h->stream, h->d.eos, t, sizeof(t));
         h->n_eos = (uint32_t)n_eos;
         h->reuse_valid = false;
         return ISPH_OK;
@@ -895,7 +907,7 @@ int isph_set_external_masses(isph_t* h, const double* pos_mass4, size_t n) {
                                     (float)pos_mass4[4 * i + 3]);
         HIP_OK(hipSetDevice(h->device));
         HIP_OK(hipStreamSynchronize(h->stream));
-        if (k) HIP_OK(hipMemcpy(h->d.ext, packed.data(), k * sizeof(float4), hipMemcpyHostToDevice));
+        if (k) h2d(h->stream, h->d.ext, packed.data(), k * sizeof(float4));
         drop_graph(h);
         h->p.n_ext = (uint32_t)k;
         h->reuse_valid = false;
@@ -1190,10 +1202,10 @@ void meas_chunks(isph_ctx* h, const std::vector<uint32_t>& ioff) {
     if (ch.size() > m.cap_chunks) throw std::runtime_error("measurement chunk table overflow");
     m.nchunks = (uint32_t)ch.size();
     if (!ch.empty()) {
-        HIP_OK(hipMemcpy(m.chunks, ch.data(), ch.size() * sizeof(uint2), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(m.cinst, ci.data(), ci.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        h2d(h->stream, m.chunks, ch.data(), ch.size() * sizeof(uint2));
+        h2d(h->stream, m.cinst, ci.data(), ci.size() * sizeof(uint32_t));
     }
-    HIP_OK(hipMemcpy(m.ichunk, ic.data(), ic.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    h2d(h->stream, m.ichunk, ic.data(), ic.size() * sizeof(uint32_t));
 }
 
 uint32_t read_word(isph_ctx* h, int k) {
@@ -1359,6 +1371,7 @@ int isph_disk_members(isph_t* h, uint32_t inst, uint32_t* idx, uint32_t* clump_i
         if (rc < 0) return rc;
         if (h->mview[(size_t)inst * kViewFields + kVValid] == 0.0) return 0;
         const uint32_t b = h->mmoff[inst], n = h->mmoff[inst + 1] - b, k = std::min(n, cap);
+        HIP_OK(hipStreamSynchronize(h->stream));
         if (k > 0 && idx)
             HIP_OK(hipMemcpy(idx, h->meas.midx + b, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
         if (k > 0 && clump_id) {
@@ -1436,11 +1449,12 @@ int isph_find_clumps(isph_t* h, const double* pos3, const double* vel3, const do
         for (int a = 0; a < 3; ++a) pr[kPCPos + a] = central_pos3[a];
         pr[kPCMass] = central_mass;
         pr[kPCRad] = central_radius;
-        HIP_OK(hipMemcpy(m.mp, p.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(m.mv, v.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-        HIP_OK(hipMemset(m.minst, 0, n * sizeof(uint32_t)));
-        HIP_OK(hipMemcpy(m.iparam, pr, sizeof(pr), hipMemcpyHostToDevice));
+        h2d(h->stream, m.mp, p.data(), n * sizeof(double4));
+        h2d(h->stream, m.mv, v.data(), n * sizeof(double4));
+        fill0(h->stream, m.minst, n * sizeof(uint32_t));
+        h2d(h->stream, m.iparam, pr, sizeof(pr));
         run_clumps(h, (uint32_t)n, 1u);
+        HIP_OK(hipStreamSynchronize(h->stream));
         if (labels) HIP_OK(hipMemcpy(labels, m.label, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
         return (int)fetch_clumps(h, 0u, std::max(min_members, 1u), 0u, out, cap);
     });
@@ -1466,8 +1480,8 @@ int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass
             m[i] = (float)mass[i];
         }
         HIP_OK(hipStreamSynchronize(h->stream));
-        HIP_OK(hipMemcpy(h->gpos, b.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(h->gmass, m.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        h2d(h->stream, h->gpos, b.data(), n * sizeof(float4));
+        h2d(h->stream, h->gmass, m.data(), n * sizeof(float));
         Dev g = h->d;
         g.n = (uint32_t)n;
         g.L = (g.n + g.K - 1) / g.K;
