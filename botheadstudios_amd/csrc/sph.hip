@@ -295,8 +295,17 @@ __global__ __launch_bounds__(64, F64 ? 6 : 8) void k_density(const float4* __res
     }
     cw.end(cc, gi, lane);
     if (split) {  // the upper half's sum joins its member's lane
-        rho += __shfl_down(rho, kWave / 2);
-        if (F64) rho64 += __shfl_down(rho64, kWave / 2);
+        if (F64) {
+            rho += __shfl_down(rho, kWave / 2);
+            rho64 += __shfl_down(rho64, kWave / 2);
+        } else {
+            // Through LDS (the candidate masses are free here): a cross-lane shuffle's lane address would be
+            // held across the walk and, at the 64-VGPR budget of 8 waves per SIMD, spilled to scratch.
+            __syncthreads();
+            if (lane >= kWave / 2) cM[lane - kWave / 2] = rho;
+            __syncthreads();
+            if (lane < kWave / 2) rho += cM[lane];
+        }
     }
     const bool owner = valid && (!split || lane < kWave / 2);
     const bool live = owner && me.w > 0.f;  // retired (mass 0) particles keep their density (:185-189)
