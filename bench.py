@@ -399,11 +399,69 @@ def _impact_gradient(fam, loss, steps, params, checkpoint_every, cfl, theta, dts
     return out["loss"], g, out["dts"], out["ms"]
 
 
+def fit_along_line(evaluate, p0, scale, target, iterations, first=None):
+    """Fit a scalar loss to `target` over parameters p (`evaluate(p)` -> (loss, gradient)). The first step is
+    Gauss-Newton on the gradient at p0, scaled by the parameters' nominal sizes `scale`: the minimum-norm step
+    that moves the linearised loss by target - L(p0), p = p0 + t d with d = s^2 g / sum (s g)^2 and t = target -
+    L(p0). A gradient that holds the tree walk's decisions fixed misjudges the size of the response over a
+    long horizon (by tens of percent), so the later steps stay on that line and solve L(p0 + t d) = target in
+    t from the measured losses alone: secant steps (at most four times the last step's length) until the
+    target is bracketed, then regula falsi in the bracket with the Illinois halving. A positive loss (a mass)
+    is matched in log L - log target, which makes an exponential response linear in t. Returns the history:
+    params, loss and t of every evaluation."""
+    l0, g0 = first if first is not None else evaluate(p0)
+    p0 = np.asarray(p0, dtype=np.float64)
+    scale = np.asarray(scale, dtype=np.float64)
+    target = float(target)
+    hist = [{"params": [float(x) for x in p0], "loss_kg": float(l0), "t": 0.0}]
+    sg = scale * np.asarray(g0, dtype=np.float64)
+    denom = float(sg @ sg)
+    if not (denom > 0.0) or iterations < 1:
+        return hist
+    d = scale * sg / denom
+    use_log = target > 0.0 and float(l0) > 0.0
+
+    def resid(loss):
+        return float(np.log(max(loss, 1e-12 * target) / target)) if use_log else loss - target
+
+    r = resid(float(l0))
+    lo = (0.0, r) if r < 0.0 else None  # bracket ends (t, residual) below and above the target
+    hi = (0.0, r) if r > 0.0 else None
+    prev, side = (0.0, r), 0
+    t = target - float(l0)
+    for _ in range(iterations):
+        loss = float(evaluate(p0 + t * d)[0])
+        hist.append({"params": [float(x) for x in p0 + t * d], "loss_kg": loss, "t": t})
+        if not np.isfinite(loss):
+            break
+        r = resid(loss)
+        if r == 0.0:
+            break
+        if r < 0.0:
+            if side < 0 and hi is not None:
+                hi = (hi[0], 0.5 * hi[1])
+            lo, side = (t, r), -1
+        else:
+            if side > 0 and lo is not None:
+                lo = (lo[0], 0.5 * lo[1])
+            hi, side = (t, r), 1
+        if lo is not None and hi is not None:
+            t_next = lo[0] - lo[1] * (hi[0] - lo[0]) / (hi[1] - lo[1])
+        else:
+            step = t - prev[0]
+            slope = (r - prev[1]) / step if step != 0.0 else 0.0
+            move = -r / slope if slope != 0.0 else step
+            t_next = t + float(np.clip(move, -4.0 * abs(step), 4.0 * abs(step)))
+        prev = (t, r)
+        t = t_next
+    return hist
+
+
 def run_adjoint_outcome(args, steps=800, fit_steps=400, iterations=3):
     """The differentiation workload where the impact has an outcome. (1) The reverse pass over `steps` KDK
-    steps from the standard start (the ejecta have left 2.9 target radii by then): loss and cost. (2) A
-    Gauss-Newton fit of (angle, speed, mass ratio) over `fit_steps` steps toward twice the nominal loss,
-    each step scaled by the parameters' nominal sizes: p += s^2 g (L* - L) / sum (s g)^2, with the dt
+    steps from the standard start (the ejecta have left 2.9 target radii by then): loss and cost. (2) A fit of
+    (angle, speed, mass ratio) over `fit_steps` steps toward twice the nominal loss (`fit_along_line`: a
+    Gauss-Newton step on the reverse pass's gradient, then a bracketed secant search along it), with the dt
     sequence of the first evaluation held fixed."""
     from botheadstudios_amd import isph, scene
     n = args.adjoint_n
@@ -419,15 +477,14 @@ def run_adjoint_outcome(args, steps=800, fit_steps=400, iterations=3):
     rec, _, _ = fam.state(*p0)
     out["loss_over_total_mass"] = l_long / float(rec["mass"].astype(np.float64).sum())
     scale = np.array([45.0, 1.2, 0.11])
-    p = p0.copy()
-    l, g, dts, _ = _impact_gradient(fam, loss, fit_steps, p, args.adjoint_every, args.cfl, args.theta)
+    l, g, dts, _ = _impact_gradient(fam, loss, fit_steps, p0, args.adjoint_every, args.cfl, args.theta)
     target = 2.0 * l
-    hist = [{"params": p.tolist(), "loss_kg": l}]
-    for _ in range(iterations):
-        sg = scale * g
-        p = p + scale * sg * (target - l) / float(sg @ sg)
-        l, g, _, _ = _impact_gradient(fam, loss, fit_steps, p, args.adjoint_every, args.cfl, args.theta, dts)
-        hist.append({"params": p.tolist(), "loss_kg": l})
+
+    def evaluate(q):
+        lq, gq, _, _ = _impact_gradient(fam, loss, fit_steps, q, args.adjoint_every, args.cfl, args.theta, dts)
+        return lq, gq
+
+    hist = fit_along_line(evaluate, p0, scale, target, iterations, first=(l, g))
     out["fit"] = {"steps": fit_steps, "target_kg": target, "iterations": hist,
                   "final_over_target": hist[-1]["loss_kg"] / target}
     return out
