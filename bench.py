@@ -558,6 +558,17 @@ def run_batch(args):
                        "from_cache": from_cache}}
 
 
+def _leg(fn, *a):
+    """An optional leg of the line: its result, or the error it raised, so that the headline measurement is
+    still printed (the traceback goes to stderr)."""
+    try:
+        return fn(*a)
+    except Exception as e:  # noqa: BLE001 - reported in the line
+        import traceback
+        traceback.print_exc(file=sys.stderr)
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -619,41 +630,53 @@ def main():
         line["dE_ok"] = bool(r["dE_rel"] < DE_BOUND)
         line["clamp_energy_rel"] = r["clamp_energy_rel"]
     line["measure"] = {"impact_state": r["measure"]}
-    debris, debris_rec = run_debris(n)
+    debris = _leg(run_debris, n)
+    debris_rec = None
+    if isinstance(debris, tuple):
+        debris, debris_rec = debris
     line["measure"]["debris_field"] = debris
-    for v in [x for x in args.variants.split(",") if x]:
+    variants = [x for x in args.variants.split(",") if x]
+    for v in variants:
         if v not in ("mixed", "reuse"):
             sys.exit(f"unknown variant {v!r}")
+
+    def variant(v):
         prec = "mixed" if v == "mixed" else args.precision
         rv = run_gpu(sc, args, prec, args.steps, args.warmup, args.reuse or v == "reuse")
-        line[v] = {"value": n * args.steps / rv["elapsed"], "ms_per_step": rv["elapsed"] * 1e3 / args.steps,
-                   "ratio_to_value": r["elapsed"] / rv["elapsed"],
-                   "phases_ms_per_step": {k: x / args.steps for k, x in rv["phases"].items()},
-                   "dE_rel": rv.get("dE_rel"), "dL_rel": rv.get("dL_rel"),
-                   "dE_ok": None if rv.get("dE_rel") is None else bool(rv["dE_rel"] < DE_BOUND)}
+        return {"value": n * args.steps / rv["elapsed"], "ms_per_step": rv["elapsed"] * 1e3 / args.steps,
+                "ratio_to_value": r["elapsed"] / rv["elapsed"],
+                "phases_ms_per_step": {k: x / args.steps for k, x in rv["phases"].items()},
+                "dE_rel": rv.get("dE_rel"), "dL_rel": rv.get("dL_rel"),
+                "dE_ok": None if rv.get("dE_rel") is None else bool(rv["dE_rel"] < DE_BOUND)}
+
+    for v in variants:
+        line[v] = _leg(variant, v)
     if not args.no_adjoint:
-        line["adjoint"] = run_adjoint(args)
-        line["adjoint"]["outcome"] = run_adjoint_outcome(args)
+        line["adjoint"] = _leg(run_adjoint, args)
+        if "error" not in line["adjoint"]:
+            line["adjoint"]["outcome"] = _leg(run_adjoint_outcome, args)
     if not args.no_batch:
-        line["batch"] = run_batch(args)
+        line["batch"] = _leg(run_batch, args)
     if not args.no_4m:
-        line["precision_4m"] = run_4m(args)
+        line["precision_4m"] = _leg(run_4m, args)
     if not args.no_precision_study:
-        line["precision_study"] = run_precision_study(args, sc)
+        line["precision_study"] = _leg(run_precision_study, args, sc)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sc)
-        # the roofline on the reference walk's interaction count (its per-particle MAC needs fewer
-        # interactions than the group MAC, which accepts a node only for the whole walk group)
-        rw = line["cpu_baseline"]["reference_walk"]
-        ref_inter = rw["cells_per_particle"] + rw["particles_per_particle"]
-        walk_inter = line["interactions_per_particle_eval"]
-        rf = line["roofline"]
-        rf["reference_walk_interactions_per_particle"] = ref_inter
-        rf["walk_interactions_per_particle"] = walk_inter
-        rf["achieved_reference_walk"] = rf["achieved"] * ref_inter / walk_inter
-        rf["frac_reference_walk"] = rf["achieved_reference_walk"] / FP32_PEAK_TFLOPS
-        line["measure"]["impact_state"]["cpu_oracle_ms"] = measure_cpu(r["measure_rec"])
-        line["measure"]["debris_field"]["cpu_oracle_ms"] = measure_cpu(debris_rec)
+        line["cpu_baseline"] = _leg(cpu_baseline, sc)
+        if "error" not in line["cpu_baseline"]:
+            # the roofline on the reference walk's interaction count (its per-particle MAC needs fewer
+            # interactions than the group MAC, which accepts a node only for the whole walk group)
+            rw = line["cpu_baseline"]["reference_walk"]
+            ref_inter = rw["cells_per_particle"] + rw["particles_per_particle"]
+            walk_inter = line["interactions_per_particle_eval"]
+            rf = line["roofline"]
+            rf["reference_walk_interactions_per_particle"] = ref_inter
+            rf["walk_interactions_per_particle"] = walk_inter
+            rf["achieved_reference_walk"] = rf["achieved"] * ref_inter / walk_inter
+            rf["frac_reference_walk"] = rf["achieved_reference_walk"] / FP32_PEAK_TFLOPS
+        line["measure"]["impact_state"]["cpu_oracle_ms"] = _leg(measure_cpu, r["measure_rec"])
+        if debris_rec is not None:
+            line["measure"]["debris_field"]["cpu_oracle_ms"] = _leg(measure_cpu, debris_rec)
     print(json.dumps(line), flush=True)
 
 
