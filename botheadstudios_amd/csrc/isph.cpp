@@ -1129,7 +1129,16 @@ int isph_force_eval(isph_t* h, float* acc4, float* rho) {
     return guarded(h, [&]() -> int {
         if (h->d.n == 0) return ISPH_OK;
         HIP_OK(hipSetDevice(h->device));
-        eval(h, true);
+        // The plain form's EOS pass must not take the f64 density a mixed evaluation left behind.
+        Dev saved = h->d;
+        h->d = integ_view(h);
+        try {
+            eval(h, true);
+        } catch (...) {
+            h->d = saved;
+            throw;
+        }
+        h->d = saved;
         h->reuse_valid = false;
         HIP_OK(hipStreamSynchronize(h->stream));
         HIP_OK(hipGetLastError());

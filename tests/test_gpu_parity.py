@@ -322,6 +322,24 @@ def test_tree_gravity_in_step_theta_bound(c1):
     assert rms_rel(acc, acc_ref)[0] < 1e-2
 
 
+def test_plain_evaluation_after_mixed_takes_no_f64_state(c1):
+    """A handle switched back from the mixed form to the plain one evaluates as a plain handle that never
+    left it, bit for bit: the f64 density, pressure and sound speed of the mixed evaluation stay unused."""
+    rec = with_random_velocities(c1)
+    with _sim(rec, c1.softening) as s:
+        s.set_gravity(isph.GRAVITY_NONE, 0.5, 1)
+        ref = s.force_eval()
+    with _sim(rec, c1.softening) as s:
+        s.set_gravity(isph.GRAVITY_NONE, 0.5, 1)
+        s.set_precision(isph.PRECISION_MIXED)
+        mixed = s.force_eval()
+        s.set_precision(isph.PRECISION_FP32)
+        back = s.force_eval()
+    for a, b in zip(ref, back):
+        np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(ref[1], mixed[1])  # the mixed du/dt differs, so the check has teeth
+
+
 def test_courant_dt_matches_cpu(c1):
     """Device Courant reduction vs HydroBody::courant_dt (hydrostatic.rs:472-480)."""
     rec = with_random_velocities(c1)
