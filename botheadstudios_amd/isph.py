@@ -135,6 +135,41 @@ class Clump(ctypes.Structure):
 
 
 _lib = None
+_warned_runtimes = False
+
+
+def hip_runtimes(maps_text=None):
+    """The HIP runtime libraries (libamdhip64) mapped into this process, as distinct file paths.
+
+    libisph.so binds to the HIP runtime already loaded under the soname libamdhip64.so.7, else to the
+    system's. A Python package that ships its own runtime and asks for it by another name (torch's wheel loads
+    its bundled libamdhip64.so this way) then adds a second runtime, with its own HSA runtime, when it is
+    imported after this library. Streams and device pointers of one runtime mean nothing to the other."""
+    if maps_text is None:
+        try:
+            with open("/proc/self/maps") as f:
+                maps_text = f.read()
+        except OSError:
+            return []
+    found = set()
+    for line in maps_text.splitlines():
+        parts = line.split(None, 5)
+        if len(parts) == 6 and os.path.basename(parts[5]).startswith("libamdhip64.so"):
+            found.add(os.path.realpath(parts[5]))
+    return sorted(found)
+
+
+def _warn_if_two_runtimes():
+    global _warned_runtimes
+    if _warned_runtimes:
+        return
+    libs = hip_runtimes()
+    if len(libs) > 1:
+        import warnings
+        _warned_runtimes = True
+        warnings.warn("two HIP runtimes are loaded in this process (" + ", ".join(libs) + "): device pointers "
+                      "and streams of one are invalid in the other. Import torch before this library when both "
+                      "are used.", RuntimeWarning, stacklevel=3)
 
 
 def load_library(path=LIB_PATH):
@@ -165,6 +200,7 @@ class GpuSph:
 
     def __init__(self, capacity, device=0):
         self._lib = load_library()
+        _warn_if_two_runtimes()
         self._h = self._lib.isph_create(int(capacity), int(device))
         if not self._h:
             raise IsphError("isph_create failed: " + self._lib.isph_last_error(None).decode())
