@@ -83,13 +83,15 @@ def lscpu_model():
 
 def measured_traffic():
     """HBM bytes per gravity launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the
-    factors measured on plain streaming kernels); None when that summary is absent."""
+    factors measured on plain streaming kernels), and the commit whose code those passes measured; (None,
+    None) when that summary is absent."""
     path = os.path.join(ROOT, "profiles", "k_gravity_tree_two_body_1m_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f)["traffic_bytes_per_launch"]
+            d = json.load(f)
+        return d["traffic_bytes_per_launch"], d.get("measured_on")
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def host_cores():
@@ -592,6 +594,7 @@ def main():
     grav_s = st["gravity_ms"] / 1e3
     achieved = flops / grav_s / 1e12 if grav_s > 0 else 0.0
     value = n * args.steps / r["elapsed"]
+    traffic, traffic_code = measured_traffic() if (n == 1048576 and args.precision == "fp32") else (None, None)
     line = {
         "metric": "particle-steps/s @N=1M (plain FP32 vs mixed, dE<1e-4); adjoint/forward cost",
         "value": value,
@@ -615,7 +618,8 @@ def main():
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS,
                      "peak_nonpacked": FP32_NONPACKED_TFLOPS, "frac_nonpacked": achieved / FP32_NONPACKED_TFLOPS,
-                     "traffic": measured_traffic() if (n == 1048576 and args.precision == "fp32") else None,
+                     "traffic": traffic,
+                     "traffic_measured_on": traffic_code,
                      "flops_per_launch": flops / launches,
                      "avg_launch_ms": st["gravity_ms"] / launches},
         "phases_ms_per_step": {k: v / args.steps for k, v in r["phases"].items()},
