@@ -115,6 +115,13 @@ template <class T>
 void dalloc(T** p, size_t count) {
     HIP_OK(hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T)));
 }
+// For the buffer sets allocated on first use: a member that an interrupted earlier call (out of memory)
+// already allocated is kept. Each set is allocated with the member its ensure_* function tests last, so a
+// set is either complete or retried, never taken as complete with members missing.
+template <class T>
+void dalloc_once(T** p, size_t count) {
+    if (!*p) dalloc(p, count);
+}
 
 // Host-to-device transfers and fills that the handle's kernels read are ordered on the handle's stream. That
 // stream is non-blocking, so it is not ordered after work on the legacy null stream, and a synchronous
@@ -235,26 +242,27 @@ void alloc_all(isph_ctx* c, uint32_t cap) {
     HIP_OK(hipStreamSynchronize(c->stream));
 }
 
+// pos64 marks the f64 state as present (the forms and getters test it), so it is allocated last.
 void ensure64(isph_ctx* c) {
     if (!c->d.pos64) {
-        dalloc(&c->d.pos64, 3 * (size_t)c->d.cap);
-        dalloc(&c->d.vel64, 3 * (size_t)c->d.cap);
-        dalloc(&c->d.srho64, c->d.cap);
-        dalloc(&c->d.spr64, c->d.cap);
-        dalloc(&c->d.scs64, c->d.cap);
-        dalloc(&c->d.acc64, 3 * (size_t)c->d.cap);
+        dalloc_once(&c->d.vel64, 3 * (size_t)c->d.cap);
+        dalloc_once(&c->d.srho64, c->d.cap);
+        dalloc_once(&c->d.spr64, c->d.cap);
+        dalloc_once(&c->d.scs64, c->d.cap);
+        dalloc_once(&c->d.acc64, 3 * (size_t)c->d.cap);
         fill0(c->stream, c->d.acc64, 3 * (size_t)c->d.cap * sizeof(double));
+        dalloc(&c->d.pos64, 3 * (size_t)c->d.cap);
     }
 }
 
 void ensure_ref(isph_ctx* c) {
     Dev& d = c->d;
     if (!d.slo) {
+        dalloc_once(&d.svlo, d.cap);
+        dalloc_once(&d.ndepth, d.cap);
+        dalloc_once(&d.nA64, d.cap);
+        dalloc_once(&d.sgrav64, d.cap);
         dalloc(&d.slo, d.cap);
-        dalloc(&d.svlo, d.cap);
-        dalloc(&d.ndepth, d.cap);
-        dalloc(&d.nA64, d.cap);
-        dalloc(&d.sgrav64, d.cap);
     }
 }
 
@@ -497,20 +505,20 @@ void ensure_adj(isph_ctx* c) {
     Adj& a = c->adj;
     if (a.xb) return;
     const size_t cap = c->d.cap;
-    dalloc(&a.xb, 3 * cap);
-    dalloc(&a.vb, 3 * cap);
-    dalloc(&a.ub, cap);
-    dalloc(&a.mb, cap);
-    dalloc(&a.ab, cap);
-    dalloc(&a.sa, cap);
-    dalloc(&a.sxb, cap);
-    dalloc(&a.svb, cap);
-    dalloc(&a.srb, cap);
-    dalloc(&a.sqb, cap);
-    dalloc(&a.scb, cap);
-    dalloc(&a.nbar, kNbarSlots * 2 * (size_t)cap);
-    dalloc(&a.pbar, kMaxMaterials * 4);
-    dalloc(&a.red, 16);
+    dalloc_once(&a.vb, 3 * cap);
+    dalloc_once(&a.ub, cap);
+    dalloc_once(&a.mb, cap);
+    dalloc_once(&a.ab, cap);
+    dalloc_once(&a.sa, cap);
+    dalloc_once(&a.sxb, cap);
+    dalloc_once(&a.svb, cap);
+    dalloc_once(&a.srb, cap);
+    dalloc_once(&a.sqb, cap);
+    dalloc_once(&a.scb, cap);
+    dalloc_once(&a.nbar, kNbarSlots * 2 * (size_t)cap);
+    dalloc_once(&a.pbar, kMaxMaterials * 4);
+    dalloc_once(&a.red, 16);
+    dalloc(&a.xb, 3 * cap);  // tested above: last
 }
 
 // The per-evaluation arrays of Dev: one set per stored evaluation of a checkpoint segment.
@@ -1156,39 +1164,39 @@ void ensure_meas(isph_ctx* h) {
     if (m.cap == h->d.cap && m.word) return;
     const size_t cap = std::max<uint32_t>(h->d.cap, 1u);
     const size_t cap_inst = 256, cap_chunks = (cap + kMeasChunk - 1) / kMeasChunk + cap_inst;
-    dalloc(&m.chunks, cap_chunks);
-    dalloc(&m.cinst, cap_chunks);
-    dalloc(&m.ichunk, cap_inst + 1);
-    dalloc(&m.part, cap_chunks * kPartFields);
-    dalloc(&m.coff, cap_chunks);
-    dalloc(&m.view, cap_inst * kViewFields);
-    dalloc(&m.cross, cap_inst);
-    dalloc(&m.k0, cap);
-    dalloc(&m.k1, cap);
-    dalloc(&m.v0, cap);
-    dalloc(&m.v1, cap);
-    dalloc(&m.khi, cap);
-    dalloc(&m.scum, cap);
-    dalloc(&m.slot, cap);
-    dalloc(&m.mbase, cap_chunks);
-    dalloc(&m.moff, cap_inst + 1);
-    dalloc(&m.mp, cap);
-    dalloc(&m.mv, cap);
-    dalloc(&m.minst, cap);
-    dalloc(&m.midx, cap);
-    dalloc(&m.iparam, cap_inst * kParamFields);
-    dalloc(&m.parent, cap);
-    dalloc(&m.label, cap);
-    dalloc(&m.cflag, cap + 1);
-    dalloc(&m.cstart, cap + 1);
-    dalloc(&m.tcnt, cap + 1);
-    dalloc(&m.cp, cap);
-    dalloc(&m.cv, cap);
-    dalloc(&m.crec, cap * kClumpFields);
-    dalloc(&m.word, 24);
-    dalloc(&m.csum, 3 * cap_inst);
-    dalloc(&m.sel, cap);
-    dalloc(&m.srec, cap * kClumpFields);
+    dalloc_once(&m.chunks, cap_chunks);
+    dalloc_once(&m.cinst, cap_chunks);
+    dalloc_once(&m.ichunk, cap_inst + 1);
+    dalloc_once(&m.part, cap_chunks * kPartFields);
+    dalloc_once(&m.coff, cap_chunks);
+    dalloc_once(&m.view, cap_inst * kViewFields);
+    dalloc_once(&m.cross, cap_inst);
+    dalloc_once(&m.k0, cap);
+    dalloc_once(&m.k1, cap);
+    dalloc_once(&m.v0, cap);
+    dalloc_once(&m.v1, cap);
+    dalloc_once(&m.khi, cap);
+    dalloc_once(&m.scum, cap);
+    dalloc_once(&m.slot, cap);
+    dalloc_once(&m.mbase, cap_chunks);
+    dalloc_once(&m.moff, cap_inst + 1);
+    dalloc_once(&m.mp, cap);
+    dalloc_once(&m.mv, cap);
+    dalloc_once(&m.minst, cap);
+    dalloc_once(&m.midx, cap);
+    dalloc_once(&m.iparam, cap_inst * kParamFields);
+    dalloc_once(&m.parent, cap);
+    dalloc_once(&m.label, cap);
+    dalloc_once(&m.cflag, cap + 1);
+    dalloc_once(&m.cstart, cap + 1);
+    dalloc_once(&m.tcnt, cap + 1);
+    dalloc_once(&m.cp, cap);
+    dalloc_once(&m.cv, cap);
+    dalloc_once(&m.crec, cap * kClumpFields);
+    dalloc_once(&m.word, 24);
+    dalloc_once(&m.csum, 3 * cap_inst);
+    dalloc_once(&m.sel, cap);
+    dalloc_once(&m.srec, cap * kClumpFields);
     m.cap = h->d.cap;
     m.cap_inst = (uint32_t)cap_inst;
     m.cap_chunks = (uint32_t)cap_chunks;
@@ -1478,8 +1486,8 @@ int isph_gravity_accelerations(isph_t* h, const double* pos3, const double* mass
         if (mode != ISPH_GRAVITY_TREE && mode != ISPH_GRAVITY_DIRECT) return fail(h, ISPH_EINVAL, "bad mode");
         HIP_OK(hipSetDevice(h->device));
         if (!h->gpos) {
+            dalloc_once(&h->gmass, h->d.cap);
             dalloc(&h->gpos, h->d.cap);
-            dalloc(&h->gmass, h->d.cap);
         }
         // gpu_gravity.rs:340-345: bodies as f32 (pos.x, pos.y, pos.z, mass); soft2 = (s*s) as f32.
         std::vector<float4> b(n);
