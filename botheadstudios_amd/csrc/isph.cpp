@@ -1575,6 +1575,7 @@ int isph_set_timing(isph_t* h, int enable) {
 
 int isph_phase_times(isph_t* h, double* ms, size_t n) {
     return guarded(h, [&]() -> int {
+        if (n > 0 && !ms) return fail(h, ISPH_EINVAL, "NULL output");
         HIP_OK(hipSetDevice(h->device));
         flush_timing(h);
         for (size_t k = 0; k < n && k < (size_t)ISPH_NUM_PHASES; ++k) ms[k] = h->phase_ms[k];
@@ -1595,27 +1596,28 @@ static void read_counters(isph_t* h, unsigned long long* cnt) {
         }
 }
 
-int isph_kernel_stats(isph_t* h, double* out7) {
-    // out7 holds 8 entries (see isph.h)
+int isph_kernel_stats(isph_t* h, double* out8) {
     return guarded(h, [&]() -> int {
+        if (!out8) return fail(h, ISPH_EINVAL, "NULL output");
         HIP_OK(hipSetDevice(h->device));
         flush_timing(h);
         unsigned long long cnt[kCounterSlots];
         read_counters(h, cnt);
-        out7[0] = h->grav_launches;
-        out7[1] = h->phase_ms[ISPH_PHASE_GRAVITY];
-        out7[2] = (double)cnt[0];
-        out7[3] = (double)cnt[1];
-        out7[4] = (double)cnt[2];
-        out7[5] = (double)cnt[3];
-        out7[6] = (double)cnt[4];
-        out7[7] = ((double)cnt[0] + (double)cnt[1]) * kFlopsMonopole + (double)cnt[2] * kFlopsQuadrupole;
+        out8[0] = h->grav_launches;
+        out8[1] = h->phase_ms[ISPH_PHASE_GRAVITY];
+        out8[2] = (double)cnt[0];
+        out8[3] = (double)cnt[1];
+        out8[4] = (double)cnt[2];
+        out8[5] = (double)cnt[3];
+        out8[6] = (double)cnt[4];
+        out8[7] = ((double)cnt[0] + (double)cnt[1]) * kFlopsMonopole + (double)cnt[2] * kFlopsQuadrupole;
         return ISPH_OK;
     });
 }
 
 int isph_walk_stats(isph_t* h, double* out9) {
     return guarded(h, [&]() -> int {
+        if (!out9) return fail(h, ISPH_EINVAL, "NULL output");
         HIP_OK(hipSetDevice(h->device));
         HIP_OK(hipStreamSynchronize(h->stream));
         unsigned long long cnt[kCounterSlots];
