@@ -1964,6 +1964,13 @@ struct Snapshot {
     }
 };
 
+// A shape read from the cache file is taken only if isph_set_gravity and isph_set_walk would accept it for this
+// handle: the file is outside the library's control, and a leaf size of 0 or above 64 would break the tree.
+bool valid_tuning(const isph_ctx* c, const Tuning& t) {
+    return t.K >= 1 && t.K <= 64 && (c->d.ninst <= 1 || t.K == 1) && std::isfinite(t.extent) && t.extent > 0.0f &&
+           (t.ppl == 1 || t.ppl == 2) && t.expand >= 1 && t.expand <= 64;
+}
+
 void apply(isph_ctx* c, const Tuning& t) {
     drop_graph(c);
     c->d.K = t.K;
@@ -2002,7 +2009,7 @@ int isph_autotune(isph_t* h, float cfl, uint32_t trial_steps, const char* cache_
                 std::istringstream v(line.substr(bar + 3));
                 Tuning t{};
                 double ms = 0.0;
-                if (!(v >> t.K >> t.extent >> t.ppl >> t.expand >> ms)) continue;
+                if (!(v >> t.K >> t.extent >> t.ppl >> t.expand >> ms) || !valid_tuning(h, t)) continue;
                 apply(h, t);
                 if (chosen) *chosen = isph_tuning{t.K, t.extent, t.ppl, t.expand, ms};
                 return 1;

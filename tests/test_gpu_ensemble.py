@@ -110,6 +110,18 @@ def test_autotune_picks_and_caches_a_walk_shape(tmp_path):
     assert from_cache
     assert (t2.leaf_size, t2.group_extent, t2.grav_particles_per_lane, t2.sph_expand) == \
         (t.leaf_size, t.group_extent, t.grav_particles_per_lane, t.sph_expand)
+    # entries for the same key that the setters would refuse are passed over, not applied
+    line = cache.read_text().splitlines()[0]
+    key = line.split(" | ")[0]
+    bad = [f"{key} | 0 4 1 4 1.0", f"{key} | 65 4 1 4 1.0", f"{key} | 1 -2 1 4 1.0", f"{key} | 1 nan 1 4 1.0",
+           f"{key} | 1 4 3 4 1.0", f"{key} | 1 4 1 0 1.0", f"{key} | 1 4 1 65 1.0", f"{key} | 1 4"]
+    cache.write_text("\n".join(bad + [line]) + "\n")
+    with isph.GpuSph(len(sc.particles)) as s:
+        s.upload(sc.particles, EOS, sc.softening)
+        from_cache, t3 = s.autotune(0.3, 2, cache)
+    assert from_cache
+    assert (t3.leaf_size, t3.group_extent, t3.grav_particles_per_lane, t3.sph_expand) == \
+        (t.leaf_size, t.group_extent, t.grav_particles_per_lane, t.sph_expand)
 
 
 @pytest.fixture(scope="module")
