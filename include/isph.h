@@ -172,7 +172,8 @@ int isph_set_graph(isph_t* h, int enable);
  * about 1.2 % of a 1M-particle evaluation, the most a 3-pass 10-bit sort could save). Results are
  * cached per size bucket (floor(log2 n)), ensemble flag, precision and gravity form in the text file
  * `cache_path` (may be NULL): a cached entry is applied without trials (returns 1), a tuned one is appended
- * (returns 0). The chosen shape is applied to the handle and written to `chosen` (may be NULL). */
+ * (returns 0); an entry that isph_set_gravity / isph_set_walk would refuse for this handle is skipped. The
+ * chosen shape is applied to the handle and written to `chosen` (may be NULL). */
 typedef struct isph_tuning {
     uint32_t leaf_size;
     float group_extent;
