@@ -21,6 +21,17 @@ PROGRAM = r"""
 static_assert(std::is_same<isph_engine::SphParticle, isph_particle>::value, "record type");
 static_assert(!std::is_copy_constructible<isph_engine::GpuSph>::value, "one owner per handle");
 static_assert(std::is_move_constructible<isph_engine::GpuSph>::value, "movable");
+// every method of the engine's GpuSph (gpu_sph.rs:1115-1505) and GpuGravity (gpu_gravity.rs:187, 207), by name
+#define HAS(C, m) static_assert(std::is_member_function_pointer<decltype(&isph_engine::C::m)>::value, #m)
+HAS(GpuSph, upload); HAS(GpuSph, set_dt); HAS(GpuSph, set_av); HAS(GpuSph, set_bulk); HAS(GpuSph, set_bulk_planar);
+HAS(GpuSph, set_external_masses); HAS(GpuSph, set_merge_budget); HAS(GpuSph, encode_merge); HAS(GpuSph, count);
+HAS(GpuSph, particle_buffer); HAS(GpuSph, encode_relax); HAS(GpuSph, encode_kdk); HAS(GpuSph, begin_readback);
+HAS(GpuSph, take_readback); HAS(GpuGravity, accelerations); HAS(GpuGravity, accelerations_tree);
+HAS(GravityField, accelerations);
+static_assert(std::is_same<decltype(std::declval<isph_engine::GpuSph&>().take_readback()),
+                           std::optional<std::vector<isph_engine::SphParticle>>>::value, "Option<Vec<SphParticle>>");
+static_assert(std::is_same<decltype(std::declval<isph_engine::GravityField&>().accelerations({}, {}, 0.0)),
+                           std::optional<std::vector<isph_engine::DVec3>>>::value, "Option<Vec<DVec3>>");
 
 int main() {
     try {
