@@ -5,7 +5,10 @@ hydrostatic.rs:78-102) relaxed for 4000 damped steps (damping 0.96, dt = relax_d
 oracle's HydroBody::relax_step (self-gravity + Tillotson pressure). The reference's bounds: the mass-weighted
 RMS radius settles (spread < 5 % over its last four samples, 200 steps apart) at 0.3-1.2 R0, and at 0.35
 and 0.55 of it the shell pressure gradient is negative and within 50 % of -rho(r) G M(<r) / r^2."""
+import os
+
 import numpy as np
+import pytest
 
 from botheadstudios_amd import bodies, materials
 from oracle import oracle
@@ -84,3 +87,92 @@ def test_a_self_gravitating_basalt_sphere_settles_into_hydrostatic_balance():
         assert rel < 0.5
         checked += 1
     assert checked >= 1
+
+
+def _new_differentiated(core, mantle, core_radius, total_radius, u_specific, target_n):
+    """HydroBody::new_differentiated (hydrostatic.rs:110-151): equal-mass particles, an iron core and a rock
+    mantle each filled at its own rho0 on Fibonacci spirals of equal-volume radii."""
+    v_core = 4.0 / 3.0 * np.pi * core_radius ** 3
+    v_mantle = 4.0 / 3.0 * np.pi * (total_radius ** 3 - core_radius ** 3)
+    m_core, m_mantle = core[0] * v_core, mantle[0] * v_mantle
+    m = (m_core + m_mantle) / target_n
+    n_core = max(int(np.floor(m_core / m + 0.5)), 1)
+    n_mantle = max(int(np.floor(m_mantle / m + 0.5)), 1)
+    i = np.arange(n_core, dtype=np.float64)
+    pc = bodies.fib_dir(i, n_core, 0.0) * (core_radius * np.cbrt((i + 0.5) / n_core))[:, None]
+    j = np.arange(n_mantle, dtype=np.float64)
+    rc3, rt3 = core_radius ** 3, total_radius ** 3
+    pm = bodies.fib_dir(j, n_mantle, 1.7) * np.cbrt(rc3 + (rt3 - rc3) * (j + 0.5) / n_mantle)[:, None]
+    n = n_core + n_mantle
+    mat = np.r_[np.zeros(n_core, np.uint32), np.ones(n_mantle, np.uint32)]
+    h = np.r_[np.full(n_core, bodies.smoothing_for(m, core[0])), np.full(n_mantle, bodies.smoothing_for(m, mantle[0]))]
+    return dict(pos=np.r_[pc, pm], vel=np.zeros((n, 3)), mass=np.full(n, m), u=np.full(n, u_specific), h=h,
+                mat=mat, softening=0.5 * np.cbrt(m / core[0]))
+
+
+@pytest.mark.skipif(not os.environ.get("ISPH_SLOW_ORACLE"),
+                    reason="about 3 minutes of CPU (ignored in the reference too): set ISPH_SLOW_ORACLE=1")
+def test_a_differentiated_iron_core_planet_settles_compresses_and_stratifies():
+    """`a_differentiated_iron_core_earth_settles_compresses_and_stratifies` (hydrostatic.rs:1715-1838, ignored
+    there for its cost): an Earth-mass iron-core / basalt-mantle body of 3000 equal-mass particles, 5000
+    damped relaxation steps (damping 0.95, dt = relax_dt(0.2)). The reference's bounds: the RMS radius
+    settles (spread < 6 %) without puffing up (<= 1.05 of the uniform sphere's sqrt(3/5) R); the core stays
+    inside (mean radius < 0.7 of the mantle's), denser than the mantle by 1.5x and compressed above its rho0;
+    hydrostatic balance within 60 % at half the RMS radius; central pressure between 5e10 and 2e12 Pa (Earth's
+    is 3.64e11)."""
+    core, mantle = materials.IRON, materials.BASALT
+    total_r = 7.37e6
+    b = _new_differentiated(core, mantle, 0.55 * total_r, total_r, 1.0e6, 3000)
+    n = len(b["mass"])
+    is_core = b["mat"] == 0
+    c_max = max(1.0, max(np.sqrt(oracle.till_c2(e, e[0], b["u"][0])) for e in (core, mantle)))
+    dt = 0.2 * b["h"].min() / c_max  # relax_dt (:483-492)
+    hb = oracle.Hydro(b["pos"], b["vel"], b["mass"], b["u"], b["h"], b["mat"], materials.eos_f64(("iron", "basalt")),
+                      b["softening"])
+    mass = b["mass"]
+
+    def rms_radius():
+        pos = hb.state()[0]
+        c = (mass[:, None] * pos).sum(0) / mass.sum()
+        return np.sqrt((mass * ((pos - c) ** 2).sum(1)).sum() / mass.sum())
+
+    rms = []  # sampled after steps 0, 250, ..., 4750
+    hb.relax(dt, 0.95)
+    rms.append(rms_radius())
+    for _ in range(19):
+        hb.relax(dt, 0.95, 250)
+        rms.append(rms_radius())
+    hb.relax(dt, 0.95, 249)
+    hb.compute_density()
+    pos, _, u, rho = hb.state()
+    p, _ = hb.eos()
+    hb.close()
+    last = np.array(rms[-4:])
+    mean = last.mean()
+    spread = np.abs(last - mean).max() / mean
+    rms_init = np.sqrt(3.0 / 5.0) * total_r
+    c = (mass[:, None] * pos).sum(0) / mass.sum()
+    d = np.linalg.norm(pos - c, axis=1)
+    rc, rm = d[is_core].mean(), d[~is_core].mean()
+    dc, dm = rho[is_core].mean(), rho[~is_core].mean()
+    print(f"N={n}, M={mass.sum():.2e} kg: settled RMS {mean / 1e3:.0f} km (init {rms_init / 1e3:.0f} km, spread "
+          f"{spread * 100:.1f} %), mean radius core {rc / 1e3:.0f} km mantle {rm / 1e3:.0f} km, density core "
+          f"{dc:.0f} mantle {dm:.0f}")
+    assert spread < 0.06
+    assert mean <= 1.05 * rms_init
+    assert rc < 0.7 * rm
+    assert dc > 1.5 * dm
+    assert dc > core[0]
+    dr, r = 0.14 * mean, 0.5 * mean
+    p_lo, _, n_lo = _shell(pos, c, p, rho, r - dr, dr)
+    p_hi, _, n_hi = _shell(pos, c, p, rho, r + dr, dr)
+    _, rho_mid, n_mid = _shell(pos, c, p, rho, r, dr)
+    assert n_lo >= 20 and n_hi >= 20 and n_mid >= 20
+    dpdr = (p_hi - p_lo) / (2.0 * dr)
+    expect = -rho_mid * G * mass[d <= r].sum() / (r * r)
+    rel = abs(dpdr - expect) / max(abs(expect), 1.0)
+    p_center, _, _ = _shell(pos, c, p, rho, 0.12 * mean, 0.12 * mean)
+    print(f"hydrostatic at r = {r / 1e3:.0f} km: dP/dr {dpdr:.3e} against -rho g {expect:.3e} (rel {rel:.2f}); "
+          f"central P {p_center:.3e} Pa")
+    assert dpdr < 0.0 and rel < 0.6
+    assert 5.0e10 < p_center < 2.0e12
