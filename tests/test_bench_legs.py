@@ -62,3 +62,11 @@ def test_the_headline_line_is_printed_when_optional_legs_fail(monkeypatch, capsy
     assert "error" in line["precision_4m"] and line["precision_study"] == {"rows": []}
     assert "error" in line["cpu_baseline"] and "frac_reference_walk" not in line["roofline"]
     assert line["mixed"]["value"] == pytest.approx(3072.0) and line["reuse"]["ratio_to_value"] == 1.0
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    import subprocess
+    r = subprocess.run([sys.executable, bench.__file__, "--gpus", "2", "--steps", "1"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and r.stdout == ""
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
