@@ -75,3 +75,33 @@ def test_rust_binding_declares_every_entry_point_with_the_header_arity():
         return 0 if params in ("", "void") else params.count(",") + 1
     for name in c:
         assert arity(rust[name]) == arity(c[name]), name
+
+
+def test_header_is_plain_c_and_a_c_program_links_against_the_library(tmp_path):
+    """A cgo, JNI or N-API binding includes the header from C: it compiles as C99 with -pedantic and
+    warnings as errors, and a C program that uses it links and runs (without a device, isph_create reports
+    NULL and the handle-free error message)."""
+    import shutil
+
+    import pytest
+    gcc = shutil.which("gcc")
+    if gcc is None or not os.path.exists(isph.LIB_PATH):
+        pytest.skip("gcc or the built library missing")
+    src = tmp_path / "abi.c"
+    src.write_text('#include <stdio.h>\n#include "isph.h"\n'
+                   "int main(void) {\n"
+                   "    isph_t* h = isph_create(16, 0);\n"
+                   "    if (h) { isph_destroy(h); puts(\"created\"); return 0; }\n"
+                   "    printf(\"null %u %s\\n\", isph_count(NULL), isph_last_error(NULL));\n"
+                   "    return 0;\n}\n")
+    exe = tmp_path / "abi"
+    libdir = os.path.dirname(isph.LIB_PATH)
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-Werror", "-pedantic", "-I",
+                        os.path.dirname(HEADER), str(src), "-o", str(exe), "-L", libdir,
+                        "-l:" + os.path.basename(isph.LIB_PATH), "-Wl,-rpath," + libdir,
+                        "-Wl,-rpath-link,/opt/rocm/lib"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300,
+                       env={**os.environ, "HIP_VISIBLE_DEVICES": ""})
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("null 0 ") and len(r.stdout.strip()) > len("null 0")
