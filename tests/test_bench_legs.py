@@ -4,6 +4,7 @@ import json
 import sys
 import types
 
+import numpy as np
 import pytest
 
 import bench
@@ -70,3 +71,44 @@ def test_more_than_one_gpu_is_refused_with_a_message():
                        text=True, timeout=120)
     assert r.returncode == 2 and r.stdout == ""
     assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
+
+
+class _FakeSph:
+    """Stands in for isph.GpuSph: every call succeeds with fixed figures."""
+
+    def __init__(self, n, device=0):
+        self.n = n
+
+    def __getattr__(self, name):
+        return lambda *a, **k: None
+
+    def diagnostics(self, with_pe=False):
+        return {"ke": 1.0, "ie": 2.0, "pe": -4.0, "L": np.array([0.0, 0.0, 1.0])}
+
+    def sim_time(self):
+        return 0.0, 1.0
+
+    def phase_times(self):
+        return {"gravity": 1.0}
+
+    def kernel_stats(self):
+        return _stats()
+
+    def walk_stats(self):
+        return {}
+
+    def clamp_energy(self):
+        return 0.0
+
+
+def test_a_failing_disk_measurement_keeps_the_timed_run(monkeypatch):
+    import argparse
+    from botheadstudios_amd import isph
+    monkeypatch.setattr(isph, "GpuSph", _FakeSph)
+    monkeypatch.setattr(bench, "time_disk_view", _boom)
+    sc = types.SimpleNamespace(particles=[0] * 64, eos=None, eos64=None, softening=1.0)
+    args = argparse.Namespace(gravity="tree", theta=0.5, leaf=1, quadrupole=False, group_extent=4.0, ppl=1,
+                              expand=4, cfl=0.3, no_energy=False)
+    out = bench.run_gpu(sc, args, "fp32", 2, 1, False, measure=True)
+    assert out["measure"] == {"error": "RuntimeError: leg failed"} and out["measure_rec"] is None
+    assert out["elapsed"] > 0 and out["dE_rel"] == 0.0
