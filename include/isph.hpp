@@ -198,6 +198,123 @@ public:
     void* stream() const { return isph_stream(h()); }
     isph_t* handle() const { return h_.get(); }
 
+    // ---- batching: ensembles, captured steps, autotuner ----
+    /* Independent systems in one handle (isph_upload_ensemble); returns the offsets of each instance. */
+    std::vector<uint32_t> upload_ensemble(const std::vector<std::vector<SphParticle>>& instances,
+                                          const std::vector<SphEos>& eos, float softening) {
+        std::vector<uint32_t> off{0u};
+        std::vector<SphParticle> all;
+        for (const auto& r : instances) {
+            all.insert(all.end(), r.begin(), r.end());
+            off.push_back((uint32_t)all.size());
+        }
+        detail::check(h(), isph_upload_ensemble(h(), all.data(), off.data(), (uint32_t)instances.size(),
+                                                eos.data(), eos.size(), softening));
+        return off;
+    }
+    void set_graph(bool enable) { detail::check(h(), isph_set_graph(h(), enable ? 1 : 0)); }
+    void set_reuse(bool enable) { detail::check(h(), isph_set_reuse(h(), enable ? 1 : 0)); }
+    void set_candidate_cache(uint32_t max_blocks) { detail::check(h(), isph_set_candidate_cache(h(), max_blocks)); }
+    /* Returns (taken from the cache file, the shape applied). */
+    std::pair<bool, isph_tuning> autotune(float cfl, uint32_t trial_steps, const char* cache_path = nullptr) {
+        isph_tuning t{};
+        const int rc = detail::check(h(), isph_autotune(h(), cfl, trial_steps, cache_path, &t));
+        return {rc == 1, t};
+    }
+
+    // ---- measurement (DiskView gpu_sph.rs:652-851, accretion::find_clumps accretion.rs:158-301) ----
+    void set_measure(double clump_softening, double link_factor) {
+        detail::check(h(), isph_set_measure(h(), clump_softening, link_factor));
+    }
+    /* DiskView::of of every instance (one record for a single system); `valid` == 0 is the reference's None. */
+    std::vector<isph_disk> disk_view() {
+        std::vector<isph_disk> out(256);
+        out.resize((size_t)detail::check(h(), isph_disk_view(h(), out.data(), (uint32_t)out.size())));
+        return out;
+    }
+    /* DiskView::clumps (min_members 1) and moonlets(k) (the accreting ones with >= k members). */
+    std::vector<isph_clump> disk_clumps(uint32_t inst = 0, uint32_t min_members = 1) {
+        const int n = detail::check(h(), isph_disk_clumps(h(), inst, min_members, nullptr, 0));
+        std::vector<isph_clump> out((size_t)n);
+        detail::check(h(), isph_disk_clumps(h(), inst, min_members, out.data(), (uint32_t)out.size()));
+        return out;
+    }
+    std::vector<isph_clump> moonlets(uint32_t k, uint32_t inst = 0) {
+        std::vector<isph_clump> out;
+        for (const auto& c : disk_clumps(inst, k))
+            if (c.n_members >= 2 && c.bound && c.outside_roche) out.push_back(c);
+        return out;
+    }
+    /* DiskView's member indices and each member's clump id. */
+    std::pair<std::vector<uint32_t>, std::vector<uint32_t>> disk_members(uint32_t inst = 0) {
+        const int n = detail::check(h(), isph_disk_members(h(), inst, nullptr, nullptr, 0));
+        std::vector<uint32_t> idx((size_t)n), clump((size_t)n);
+        detail::check(h(), isph_disk_members(h(), inst, idx.data(), clump.data(), (uint32_t)n));
+        return {idx, clump};
+    }
+    /* disk_stats_json (gpu_sph.rs:825-851). */
+    std::string disk_stats_json(uint32_t inst = 0) {
+        const int n = detail::check(h(), isph_disk_stats_json(h(), inst, nullptr, 0));
+        std::string buf((size_t)n + 1, '\0');
+        detail::check(h(), isph_disk_stats_json(h(), inst, &buf[0], buf.size()));
+        buf.resize((size_t)n);
+        return buf;
+    }
+
+    // ---- differentiation (mixed form) ----
+    void set_state_f64(const std::vector<DVec3>& pos, const std::vector<DVec3>& vel) {
+        if (pos.size() != count() || vel.size() != count()) throw Error(ISPH_EINVAL, "state size differs from count()");
+        detail::check(h(), isph_set_state_f64(h(), pos.empty() ? nullptr : pos[0].data(),
+                                              vel.empty() ? nullptr : vel[0].data()));
+    }
+    double loss_value(const isph_loss& loss) {
+        double v = 0.0;
+        detail::check(h(), isph_loss_value(h(), &loss, &v));
+        return v;
+    }
+    struct Vjp {
+        double loss = 0.0;
+        std::vector<DVec3> dpos, dvel;
+        std::vector<double> du, dmass;
+        std::vector<std::array<double, 4>> deos;  // d loss / d (a, b, A, B) per material
+        std::vector<float> dts;
+        double forward_ms = 0.0, reverse_ms = 0.0;
+    };
+    /* Reverse-mode gradient of `loss` after `steps` KDK steps (isph_vjp_run); dts empty = Courant at cfl. */
+    Vjp vjp_run(const isph_loss& loss, uint32_t steps, uint32_t checkpoint_every, float cfl,
+                const std::vector<float>& dts = {}) {
+        if (!dts.empty() && dts.size() != steps) throw Error(ISPH_EINVAL, "dts must hold one dt per step");
+        const size_t n = count();
+        Vjp r;
+        r.dpos.resize(n);
+        r.dvel.resize(n);
+        r.du.resize(n);
+        r.dmass.resize(n);
+        r.deos.resize(16);
+        r.dts.resize(steps);
+        double ms[2] = {0.0, 0.0};
+        detail::check(h(), isph_vjp_run(h(), cfl, dts.empty() ? nullptr : dts.data(), steps, checkpoint_every, &loss,
+                                        &r.loss, n ? r.dpos[0].data() : nullptr, n ? r.dvel[0].data() : nullptr,
+                                        r.du.data(), r.dmass.data(), r.deos[0].data(), r.dts.data(), ms));
+        r.forward_ms = ms[0];
+        r.reverse_ms = ms[1];
+        return r;
+    }
+
+    // ---- timing and statistics ----
+    void set_timing(bool enable) { detail::check(h(), isph_set_timing(h(), enable ? 1 : 0)); }
+    std::array<double, ISPH_NUM_PHASES> phase_times() {
+        std::array<double, ISPH_NUM_PHASES> ms{};
+        detail::check(h(), isph_phase_times(h(), ms.data(), ms.size()));
+        return ms;
+    }
+    void reset_stats() { detail::check(h(), isph_reset_stats(h())); }
+    double clamp_energy() {
+        double j = 0.0;
+        detail::check(h(), isph_clamp_energy(h(), &j));
+        return j;
+    }
+
 private:
     isph_t* h() const { return h_.get(); }
     detail::Handle h_;

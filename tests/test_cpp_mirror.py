@@ -28,6 +28,10 @@ HAS(GpuSph, set_external_masses); HAS(GpuSph, set_merge_budget); HAS(GpuSph, enc
 HAS(GpuSph, particle_buffer); HAS(GpuSph, encode_relax); HAS(GpuSph, encode_kdk); HAS(GpuSph, begin_readback);
 HAS(GpuSph, take_readback); HAS(GpuGravity, accelerations); HAS(GpuGravity, accelerations_tree);
 HAS(GravityField, accelerations);
+// the extensions: measurement (DiskView), differentiation, batching
+HAS(GpuSph, disk_view); HAS(GpuSph, disk_clumps); HAS(GpuSph, moonlets); HAS(GpuSph, disk_members);
+HAS(GpuSph, disk_stats_json); HAS(GpuSph, vjp_run); HAS(GpuSph, loss_value); HAS(GpuSph, set_state_f64);
+HAS(GpuSph, upload_ensemble); HAS(GpuSph, autotune); HAS(GpuSph, set_graph); HAS(GpuSph, step_cfl);
 static_assert(std::is_same<decltype(std::declval<isph_engine::GpuSph&>().take_readback()),
                            std::optional<std::vector<isph_engine::SphParticle>>>::value, "Option<Vec<SphParticle>>");
 static_assert(std::is_same<decltype(std::declval<isph_engine::GravityField&>().accelerations({}, {}, 0.0)),
