@@ -261,6 +261,31 @@ public:
         return buf;
     }
 
+    /* accretion::find_clumps (accretion.rs:158-301) on caller arrays (n <= capacity): the clumps with at least
+     * min_members members, heaviest first, and every particle's clump id (its smallest member index). */
+    struct Clumps {
+        std::vector<isph_clump> clumps;
+        std::vector<uint32_t> labels;
+    };
+    Clumps find_clumps(const std::vector<DVec3>& pos, const std::vector<DVec3>& vel, const std::vector<double>& mass,
+                       const std::vector<double>& rho, double linking_length, double g, double softening,
+                       const DVec3& central_pos, double central_mass, double central_radius,
+                       uint32_t min_members) {
+        const size_t n = pos.size();
+        if (vel.size() != n || mass.size() != n || rho.size() != n)
+            throw Error(ISPH_EINVAL, "find_clumps: arrays differ in length");
+        Clumps r;
+        r.clumps.resize(n / std::max<uint32_t>(min_members, 1u));  // no more clumps than that
+        r.labels.resize(n);
+        const int k = detail::check(
+            h(), isph_find_clumps(h(), n ? pos[0].data() : nullptr, n ? vel[0].data() : nullptr, mass.data(),
+                                  rho.data(), n, linking_length, g, softening, central_pos.data(), central_mass,
+                                  central_radius, min_members, r.clumps.data(), (uint32_t)r.clumps.size(),
+                                  r.labels.data()));
+        r.clumps.resize(std::min(r.clumps.size(), (size_t)k));
+        return r;
+    }
+
     // ---- differentiation (mixed form) ----
     void set_state_f64(const std::vector<DVec3>& pos, const std::vector<DVec3>& vel) {
         if (pos.size() != count() || vel.size() != count()) throw Error(ISPH_EINVAL, "state size differs from count()");

@@ -30,7 +30,7 @@ HAS(GpuSph, take_readback); HAS(GpuGravity, accelerations); HAS(GpuGravity, acce
 HAS(GravityField, accelerations);
 // the extensions: measurement (DiskView), differentiation, batching
 HAS(GpuSph, disk_view); HAS(GpuSph, disk_clumps); HAS(GpuSph, moonlets); HAS(GpuSph, disk_members);
-HAS(GpuSph, disk_stats_json); HAS(GpuSph, vjp_run); HAS(GpuSph, loss_value); HAS(GpuSph, set_state_f64);
+HAS(GpuSph, disk_stats_json); HAS(GpuSph, find_clumps); HAS(GpuSph, vjp_run); HAS(GpuSph, loss_value); HAS(GpuSph, set_state_f64);
 HAS(GpuSph, upload_ensemble); HAS(GpuSph, autotune); HAS(GpuSph, set_graph); HAS(GpuSph, step_cfl);
 static_assert(std::is_same<decltype(std::declval<isph_engine::GpuSph&>().take_readback()),
                            std::optional<std::vector<isph_engine::SphParticle>>>::value, "Option<Vec<SphParticle>>");
