@@ -11,7 +11,8 @@
 // The traversal stack lives in LDS; up to 64 nodes are popped per round (one per lane), which leaves at
 // most ~64 pending nodes per tree level (unrestricted, the deepest stack at N = 1M reaches ~540 entries).
 // The pop width narrows towards depth-first near capacity, which bounds the stack by capacity - margin +
-// tree depth; a 512-entry stack keeps the workgroup's LDS small enough for 6 resident waves per SIMD.
+// tree depth. With a 512-entry stack the monopole walk's LDS is 4 KB per wave, so registers set its resident
+// waves (8 per SIMD at 64 VGPRs); the quadrupole walk's extra lists take 9.5 KB, which caps it at 4.
 #include "state.h"
 
 namespace isph {
